@@ -1,0 +1,76 @@
+"""ctypes binding of libatsc3fec.so (declarations in include/atsc3_fec.h).
+
+The library is required: there is no CPU fallback anywhere in the package.
+If the .so is missing or fails to load, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libatsc3fec.so")
+
+_lock = threading.Lock()
+_lib = None
+
+c_i32, c_f32, c_vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
+
+# name -> argtypes (all pointers are passed as integers / c_void_p)
+_SIGS = {
+    "ldpc_kernel_abi": [],
+    "ldpc_minsum_f32": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
+                        c_i32, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_csr_f32_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+                                 c_vp, c_i32, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp, c_vp],
+    "atsc3_minsum_qc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+                            c_vp, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp,
+                            c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_demap_ws_doubles": [c_i32, c_i32],
+    "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
+                        c_vp, c_vp],
+}
+
+ERRORS = {-1: "check degree exceeds the supported maximum",
+          -2: "inconsistent sizes or NULL pointers",
+          -3: "HIP runtime error or kernel launch failure",
+          -4: "size exceeds a fixed capacity"}
+
+
+class FecError(RuntimeError):
+    def __init__(self, fn, rc):
+        super().__init__(f"{fn} returned {rc}: {ERRORS.get(rc, 'unknown error')}")
+        self.rc = rc
+
+
+def lib():
+    """Load the library (once).  Raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built: run python -m atsc3_amd.build "
+                    "(hipcc, gfx950)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, args in _SIGS.items():
+                f = getattr(L, name)
+                f.restype = c_i32
+                f.argtypes = args
+            if L.ldpc_kernel_abi() != 1:
+                raise RuntimeError("libatsc3fec ABI mismatch")
+            _lib = L
+        return _lib
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0 and name != "atsc3_demap_ws_doubles":
+        raise FecError(name, rc)
+    return rc
+
+
+def exported_symbols():
+    return list(_SIGS)

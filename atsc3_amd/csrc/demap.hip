@@ -1,0 +1,179 @@
+// Max-log NUC demapper with fused bit de-interleave, gfx950.
+//
+// Maths (lab/m6_bicm.py:213-225, max-log, llr > 0 means bit 0):
+//   LLR_i(z) = w * (min_{p: y_i = 1} |z - p|^2 - min_{p: y_i = 0} |z - p|^2) / sigma2
+//
+// Every A/322 alphabet handled here (QPSK and the 2D NUCs of Annex C) is
+// built by the quadrant rule x = [w, -conj w, conj w, -w] from M/4 base
+// points w in the closed first quadrant, with label bit y0 selecting the sign
+// of the imaginary part and y1 the sign of the real part (the host checks
+// this before it hands the base points over).  Hence, with z~ = (|Re z|,
+// |Im z|) and per base point
+//     A = (|zr| - wr)^2,  B = (|zr| + wr)^2,  C = (|zi| - wi)^2,  D = (|zi| + wi)^2,
+//   * the nearest point with a given index k of w is at distance A + C, so for
+//     label bits y2..y_{nb-1} (the bits of k) the subset minima are minima of
+//     A + C over k;
+//   * for y0 the subset with the same imaginary sign as z has minimum
+//     min_k (A + C) and the other min_k (A + D); likewise y1 with B + C.
+// That is the exhaustive max-log over all M points, evaluated exactly (in
+// real arithmetic) with M/4 distance evaluations, each by direct differences
+// (no |p|^2 - 2 Re(z p*) expansion, which loses relative accuracy on the
+// closely spaced 256-NUC points).
+//
+// Output position of q-stream bit q = cell*nb + i is lam_of_q[q]
+// (lab/m6_bicm.py:331-335), so the LLRs land in codeword order.
+//
+// sigma2 == NULL: per block sigma2 = max(mean(w * d2min), 1e-9) with d2min
+// = min_p |z - p|^2 (lab/m6_bicm.py:219-220), reduced in FP64 in a fixed
+// order (per-workgroup tree, then sequential over workgroups), then a second
+// pass divides the block's LLRs by it.
+#include "fec_common.h"
+
+#define DEMAP_WG 256
+
+namespace atsc3 {
+
+template <int NB>
+__global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
+    const float2 *__restrict__ cells, const float *__restrict__ csi,
+    const double *__restrict__ sigma2, int ncell, const float2 *__restrict__ base,
+    const int *__restrict__ lam_of_q, float *__restrict__ llr, double *__restrict__ partial) {
+    constexpr int NW = 1 << (NB - 2);
+    __shared__ float2 w[NW];
+    __shared__ double red[DEMAP_WG / 64];
+    const int blk = blockIdx.y;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NW; k += DEMAP_WG) w[k] = base[k];
+    __syncthreads();
+    const int s = blockIdx.x * DEMAP_WG + tid;
+    const size_t cbase = (size_t)blk * ncell;
+    const int N = ncell * NB;
+    double d2w = 0.0;
+    if (s < ncell) {
+        const float2 z = cells[cbase + s];
+        const float ar = fabsf(z.x), ai = fabsf(z.y);
+        float dmin = INFINITY, opp_im = INFINITY, opp_re = INFINITY;
+        float m0[NB > 2 ? NB - 2 : 1], m1[NB > 2 ? NB - 2 : 1];
+#pragma unroll
+        for (int i = 0; i < NB - 2; ++i) m0[i] = m1[i] = INFINITY;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const float2 p = w[k];
+            const float dr = ar - p.x, sr = ar + p.x;
+            const float di = ai - p.y, si = ai + p.y;
+            const float A = dr * dr, B = sr * sr, C = di * di, D = si * si;
+            const float ac = A + C, ad = A + D, bc = B + C;
+            dmin = fminf(dmin, ac);
+            opp_im = fminf(opp_im, ad);
+            opp_re = fminf(opp_re, bc);
+#pragma unroll
+            for (int i = 2; i < NB; ++i) {
+                if ((k >> (NB - 1 - i)) & 1)
+                    m1[i - 2] = fminf(m1[i - 2], ac);
+                else
+                    m0[i - 2] = fminf(m0[i - 2], ac);
+            }
+        }
+        const float wgt = csi ? csi[cbase + s] : 1.0f;
+        float diff[NB];
+        // y0: imaginary sign; y1: real sign (see header)
+        diff[0] = z.y >= 0.0f ? opp_im - dmin : dmin - opp_im;
+        diff[1] = z.x >= 0.0f ? opp_re - dmin : dmin - opp_re;
+#pragma unroll
+        for (int i = 2; i < NB; ++i) diff[i] = m1[i - 2] - m0[i - 2];
+        const int *lq = lam_of_q + (size_t)s * NB;
+        float *out = llr + (size_t)blk * N;
+        if (sigma2) {
+            const float s2 = (float)sigma2[blk];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) out[lq[i]] = (diff[i] * wgt) / s2;
+        } else {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) out[lq[i]] = diff[i] * wgt;
+            d2w = (double)wgt * (double)dmin;
+        }
+    }
+    if (!sigma2) {
+        for (int off = 32; off > 0; off >>= 1) d2w += __shfl_xor(d2w, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = d2w;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int i = 0; i < DEMAP_WG / 64; ++i) t += red[i];
+            partial[(size_t)blk * gridDim.x + blockIdx.x] = t;
+        }
+    }
+}
+
+// One workgroup per block: sigma2 = max(sum(partials) / ncell, 1e-9), then
+// llr /= sigma2 over the block's N values.
+__global__ __launch_bounds__(DEMAP_WG) void demap_scale_kernel(
+    float *__restrict__ llr, int N, int ncell, const double *__restrict__ partial, int nwg,
+    double *__restrict__ sigma2_out) {
+    __shared__ float s2f;
+    const int blk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < nwg; ++i) t += partial[(size_t)blk * nwg + i];
+        double s2 = t / (double)ncell;
+        s2 = s2 > 1e-9 ? s2 : 1e-9;
+        if (sigma2_out) sigma2_out[blk] = s2;
+        s2f = (float)s2;
+    }
+    __syncthreads();
+    const float d = s2f;
+    float *out = llr + (size_t)blk * N;
+    for (int i = threadIdx.x; i < N; i += DEMAP_WG) out[i] = out[i] / d;
+}
+
+__global__ void copy_sigma2_kernel(const double *src, double *dst, int nblk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nblk) dst[i] = src[i];
+}
+
+template <int NB>
+static void launch_demap(const float *cells, const float *csi, const double *sigma2,
+                         int nblk, int ncell, const float *points,
+                         const int32_t *lam_of_q, float *llr, double *ws, hipStream_t st) {
+    const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
+    hipLaunchKernelGGL((demap_kernel<NB>), dim3(nwg, nblk), dim3(DEMAP_WG), 0, st,
+                       reinterpret_cast<const float2 *>(cells), csi, sigma2, ncell,
+                       reinterpret_cast<const float2 *>(points), lam_of_q, llr, ws);
+}
+
+}  // namespace atsc3
+
+extern "C" int32_t atsc3_demap_ws_doubles(int32_t nblk, int32_t ncell) {
+    if (nblk < 0 || ncell < 0) return ATSC3_E_ARG;
+    return nblk * ((ncell + DEMAP_WG - 1) / DEMAP_WG);
+}
+
+extern "C" int32_t atsc3_demap_dev(const float *cells, const float *csi,
+                                   const double *sigma2, int32_t nblk, int32_t ncell,
+                                   const float *points, int32_t nbits,
+                                   const int32_t *lam_of_q, float *llr_out,
+                                   double *sigma2_out, double *ws, void *stream) {
+    if (nblk < 0 || ncell < 0) return ATSC3_E_ARG;
+    if (nblk == 0 || ncell == 0) return ATSC3_OK;
+    if (!cells || !points || !lam_of_q || !llr_out) return ATSC3_E_ARG;
+    if (!sigma2 && !ws) return ATSC3_E_ARG;
+    hipStream_t st = atsc3::as_stream(stream);
+    // `points` holds the M/4 first-quadrant base points (host-checked).
+    switch (nbits) {
+        case 2: atsc3::launch_demap<2>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
+        case 4: atsc3::launch_demap<4>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
+        case 6: atsc3::launch_demap<6>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
+        case 8: atsc3::launch_demap<8>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
+        default: return ATSC3_E_ARG;
+    }
+    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    if (!sigma2) {
+        const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
+        hipLaunchKernelGGL(atsc3::demap_scale_kernel, dim3(nblk), dim3(DEMAP_WG), 0, st,
+                           llr_out, ncell * nbits, ncell, ws, nwg, sigma2_out);
+    } else if (sigma2_out) {
+        hipLaunchKernelGGL(atsc3::copy_sigma2_kernel, dim3((nblk + 255) / 256), dim3(256), 0,
+                           st, sigma2, sigma2_out, nblk);
+    }
+    return atsc3::launch_status();
+}

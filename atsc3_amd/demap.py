@@ -1,0 +1,132 @@
+"""GPU max-log demapping behind the upstream demap call signatures.
+
+Mirrors:
+  * ``demap_llr(cells, mod, rate, sigma2=None)`` -- lab/m6_bicm.py:213-225:
+    one block of cells -> q-ordered LLRs (numpy).
+  * ``demap_batch(cells2d, pts, sigma2=None)`` -- lab/m9_fast.py:756,
+    lab/m44_ldm.py:2454 (demap_batch_qam), lab/m9_gpu.py:301 (GpuBicm.demap):
+    (nblk, ncell) -> (nblk, ncell*nb) q-ordered LLRs, sigma2 per block.
+  * ``Demapper(n, mod, rate).demap_dev(cells, csi, sigma2)`` -- the fused
+    form: demap plus the caller's de-interleave ``lam[lam_of_q] = q``
+    (lab/m6_bicm.py:331-335), device tensors in and out, codeword order.
+
+LLR_i = w * (min_{y_i=1} |z-p|^2 - min_{y_i=0} |z-p|^2) / sigma2, llr > 0 means
+bit 0.  w is the per-cell CSI weight |h|^2 of equalised cells (1 if none);
+sigma2 None -> per block max(mean(w * min_p |z-p|^2), 1e-9), reduced in FP64
+on the device.
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import torch
+
+from . import _lib, bicm
+
+
+def _check_quadrant_rule(pts):
+    """The kernel evaluates max-log through the quadrant rule; verify it."""
+    pts = np.asarray(pts, np.complex128)
+    m = len(pts)
+    nb = int(round(np.log2(m)))
+    if 1 << nb != m or nb not in (2, 4, 6, 8):
+        raise ValueError(f"unsupported alphabet size {m}")
+    w = pts[: m // 4]
+    ok = (np.allclose(pts[m // 4: m // 2], -np.conj(w), rtol=0, atol=0)
+          and np.allclose(pts[m // 2: 3 * m // 4], np.conj(w), rtol=0, atol=0)
+          and np.allclose(pts[3 * m // 4:], -w, rtol=0, atol=0)
+          and (w.real >= 0).all() and (w.imag >= 0).all())
+    if not ok:
+        raise ValueError("alphabet does not follow the A/322 quadrant rule "
+                         "with first-quadrant base points")
+    return nb
+
+
+def _stream(dev):
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=None):
+    """Core launcher.  cells (B, C) complex64 device tensor."""
+    if not (isinstance(cells, torch.Tensor) and cells.is_cuda):
+        raise TypeError("cells must be a device tensor")
+    cells = cells.to(torch.complex64).contiguous()
+    B, C = cells.shape
+    dev = cells.device
+    N = C * nb
+    llr = out if out is not None else torch.empty((B, N), dtype=torch.float32, device=dev)
+    s2out = torch.empty(B, dtype=torch.float64, device=dev)
+    if B == 0 or C == 0:
+        return llr, s2out
+    csi_p = None
+    if csi is not None:
+        csi = torch.as_tensor(csi, device=dev).to(torch.float32).contiguous()
+        assert csi.shape == (B, C)
+        csi_p = csi.data_ptr()
+    s2_p, ws, ws_p = None, None, None
+    if sigma2 is not None:
+        sigma2 = torch.as_tensor(sigma2, dtype=torch.float64, device=dev).reshape(-1)
+        if sigma2.numel() == 1 and B > 1:
+            sigma2 = sigma2.expand(B)
+        sigma2 = sigma2.contiguous()
+        s2_p = sigma2.data_ptr()
+    else:
+        nws = _lib.lib().atsc3_demap_ws_doubles(B, C)
+        ws = torch.empty(max(nws, 1), dtype=torch.float64, device=dev)
+        ws_p = ws.data_ptr()
+    _lib.call("atsc3_demap_dev", torch.view_as_real(cells).data_ptr(), csi_p, s2_p, B, C,
+              pts_dev.data_ptr(), nb, lam_of_q_dev.data_ptr(), llr.data_ptr(),
+              s2out.data_ptr(), ws_p, _stream(dev))
+    return llr, s2out
+
+
+@functools.lru_cache(maxsize=None)
+def _points_dev(key, device):
+    mod, rate = key
+    pts = bicm.points(mod, rate)
+    nb = _check_quadrant_rule(pts)
+    t = torch.from_numpy(np.stack([pts.real, pts.imag], 1).astype(np.float32)).to(device)
+    return t.contiguous(), nb
+
+
+@functools.lru_cache(maxsize=None)
+def _identity_dev(N, device):
+    return torch.arange(N, dtype=torch.int32, device=device)
+
+
+class Demapper:
+    """Fused demap + de-interleave for one (N, modulation, rate)."""
+
+    def __init__(self, n, mod, rate, device="cuda"):
+        self.n, self.mod, self.rate = int(n), mod, rate
+        self.device = torch.device(device)
+        self.points, self.nb = _points_dev((mod, rate), self.device)
+        self.ncell = bicm.cells_per_block(self.n, mod)
+        lq = bicm.bicm_map(self.n, mod, rate)
+        self.lam_of_q = torch.from_numpy(lq.astype(np.int32)).to(self.device)
+
+    def demap_dev(self, cells, csi=None, sigma2=None, out=None):
+        """(B, ncell) cells -> ((B, n) f32 codeword-order LLRs, sigma2 (B,) f64)."""
+        if cells.shape[1] != self.ncell:
+            raise ValueError(f"{cells.shape[1]} cells per block, expected {self.ncell}")
+        return demap_q_dev(cells, self.points, self.nb, self.lam_of_q, csi, sigma2, out)
+
+
+def demap_batch(cells2d, pts, sigma2=None, csi=None, device="cuda"):
+    """(nblk, ncell) -> ((nblk, ncell*nb) q-order float32 numpy, sigma2)."""
+    pts = np.asarray(pts)
+    nb = _check_quadrant_rule(pts)
+    dev = torch.device(device)
+    p = torch.from_numpy(np.stack([pts.real, pts.imag], 1).astype(np.float32)).to(dev)
+    z = torch.from_numpy(np.ascontiguousarray(cells2d, np.complex64)).to(dev)
+    ident = _identity_dev(z.shape[1] * nb, dev)
+    llr, s2 = demap_q_dev(z, p, nb, ident, csi, sigma2)
+    return llr.cpu().numpy(), s2.cpu().numpy()
+
+
+def demap_llr(cells, mod, rate, sigma2=None, device="cuda"):
+    """lab/m6_bicm.py:213: one block -> q-ordered LLRs (numpy float32)."""
+    llr, _ = demap_batch(np.asarray(cells)[None, :], bicm.points(mod, rate), sigma2,
+                         device=device)
+    return llr[0]

@@ -1,0 +1,287 @@
+"""GPU min-sum decoding behind the upstream call signatures.
+
+Mirrors (same arguments, same return tuples):
+  * ``min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype)``
+    -- lab/m3_ldpc.py:509-630.  float32 is the path the upstream serves with
+    its compiled kernel (lab/ldpc_kernel.c); here it runs on the GPU and is
+    bit-exact with that kernel on every field (bits, conv, its, bad).
+  * ``GpuMinSum(checks, n, iters, alpha, device, dtype).decode(llr2d, iters,
+    alpha)`` -- lab/m9_gpu.py:84-220; accepts a numpy array or a device
+    tensor and returns numpy ``(bits u8, conv bool, its i32, bad i32)``.
+  * ``ladder_decode`` -- the alpha ladder of lab/m9_fast.py:843-891 /
+    lab/m6_bicm.py:341-355 (rungs 1.0, 0.85, 0.75 on still-failing rows; a
+    rung's result is taken when it converged or has fewer unsatisfied checks).
+
+``checks`` may be an ``LdpcCode`` or the upstream list-of-lists; a list is
+matched to its A/322 code (N, rate from N - len(checks)) and verified equal,
+so the QC kernel can serve it; an unrecognised matrix runs on the CSR kernel.
+
+Message storage (``msg_dtype``): "float32" (plain, bit-exact), "float16",
+"int8" (per-block scale).  Variable sums accumulate in FP32 in every mode.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .codes import LdpcCode, get_code, RATES
+
+MSG_FORMATS = {"float32": 0, "fp32": 0, "float16": 1, "fp16": 1, "int8": 2}
+MSG_BYTES = {0: 4, 1: 2, 2: 1}
+_DEV_TABLES = {}
+_RESOLVED = {}
+
+
+def _stream_ptr(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def resolve_code(checks, n=None):
+    """LdpcCode for ``checks`` (an LdpcCode or a list of column lists)."""
+    if isinstance(checks, LdpcCode):
+        return checks
+    key = id(checks)
+    hit = _RESOLVED.get(key)
+    if hit is not None and hit[0] is checks:
+        return hit[1]
+    nc = len(checks)
+    code = None
+    if n in (16200, 64800):
+        k = n - nc
+        if (k * 15) % n == 0:
+            rate = f"{k * 15 // n}/15"
+            if rate in RATES:
+                try:
+                    cand = get_code(rate, n)
+                except ValueError:
+                    cand = None
+                if cand is not None and cand.checks == [list(c) for c in checks]:
+                    code = cand
+    if code is None:
+        code = _GenericCode(checks, n)
+    _RESOLVED[key] = (checks, code)
+    return code
+
+
+class _GenericCode:
+    """A parity-check matrix that is not an A/322 QC code: CSR only."""
+
+    def __init__(self, checks, n):
+        self.n = int(n)
+        self.checks = checks
+        self.nc = len(checks)
+        cn = np.array([len(c) for c in checks], np.int32)
+        cptr = np.concatenate([[0], np.cumsum(cn)]).astype(np.int32)
+        cidx = np.fromiter((v for c in checks for v in c), np.int32, int(cn.sum()))
+        vedge = np.argsort(cidx, kind="stable").astype(np.int32)
+        vptr = np.concatenate([[0], np.cumsum(np.bincount(cidx, minlength=self.n))]).astype(np.int32)
+        self._csr = (cptr, cidx, vptr, vedge)
+
+    def csr(self):
+        return self._csr
+
+    def qc(self):
+        raise ValueError("not a 360-lifted QC code")
+
+
+class DeviceTables:
+    """Code tables resident on one device (built once per code, like
+    GpuMinSum.__init__, lab/m9_gpu.py:95-136)."""
+
+    def __init__(self, code, device):
+        dev = torch.device(device)
+        self.code = code
+        self.n = code.n
+        cptr, cidx, vptr, vedge = code.csr()
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        self.cptr, self.cidx, self.vptr, self.vedge = t(cptr), t(cidx), t(vptr), t(vedge)
+        self.nc = len(cptr) - 1
+        self.nedges = len(cidx)
+        self.max_check_degree = int(np.diff(cptr).max(initial=0))
+        try:
+            q = code.qc()
+        except ValueError:
+            q = None
+        self.qc = None
+        if q is not None and q["max_vdeg"] <= 32:
+            self.qc = dict(q)
+            for k in ("cg_ptr", "be_word", "vg_ptr", "ve_word"):
+                self.qc[k] = t(q[k])
+
+
+def device_tables(code, device):
+    dev = torch.device(device)
+    if dev.index is None:
+        dev = torch.device(dev.type, torch.cuda.current_device())
+    key = (id(code), str(dev))
+    hit = _DEV_TABLES.get(key)
+    if hit is None or hit.code is not code:
+        hit = DeviceTables(code, dev)
+        _DEV_TABLES[key] = hit
+    return hit
+
+
+def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="auto",
+               msg_scale=None):
+    """Decode device-resident LLRs (nblk, n) float32; no host sync.
+
+    Returns device tensors (bits u8 (nblk,n), conv u8, its i32, bad i32).
+    engine: "qc" (360-lifted kernel), "csr" (generic kernel; FP32 only) or
+    "auto" (qc when the code has a QC form).
+    """
+    if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
+        raise TypeError("decode_dev needs a CUDA (HIP) tensor")
+    llr = llr.contiguous()
+    if llr.dtype != torch.float32:
+        llr = llr.float()
+    dev = llr.device
+    tabs = device_tables(code, dev)
+    nblk, n = llr.shape
+    if n != tabs.n:
+        raise ValueError(f"LLR width {n} != code length {tabs.n}")
+    fmt = MSG_FORMATS[msg_dtype]
+    bits = torch.empty((nblk, n), dtype=torch.uint8, device=dev)
+    conv = torch.empty(nblk, dtype=torch.uint8, device=dev)
+    its = torch.empty(nblk, dtype=torch.int32, device=dev)
+    bad = torch.empty(nblk, dtype=torch.int32, device=dev)
+    if nblk == 0:
+        return bits, conv, its, bad
+    use_qc = engine == "qc" or (engine == "auto" and tabs.qc is not None)
+    if use_qc and tabs.qc is None:
+        raise ValueError("this code has no QC form with variable degree <= 32; "
+                         "use engine='csr'")
+    stream = _stream_ptr(dev)
+    tot = torch.empty((nblk, n), dtype=torch.float32, device=dev)
+    if use_qc:
+        q = tabs.qc
+        msg = torch.empty(nblk * q["nbe"] * 360 * MSG_BYTES[fmt], dtype=torch.uint8, device=dev)
+        scale_ptr = None
+        if msg_scale is not None:
+            msg_scale = torch.as_tensor(msg_scale, dtype=torch.float32, device=dev).contiguous()
+            scale_ptr = msg_scale.data_ptr()
+        _lib.call("atsc3_minsum_qc_dev", llr.data_ptr(), nblk, n, q["ncg"], q["nvg"],
+                  q["nbe"], q["cg_ptr"].data_ptr(), q["be_word"].data_ptr(),
+                  q["vg_ptr"].data_ptr(), q["ve_word"].data_ptr(), q["max_cdeg"],
+                  q["max_vdeg"], int(iters), float(np.float32(alpha)), fmt, scale_ptr,
+                  tot.data_ptr(), msg.data_ptr(), bits.data_ptr(), conv.data_ptr(),
+                  its.data_ptr(), bad.data_ptr(), stream)
+    else:
+        if fmt != 0:
+            raise ValueError("the CSR engine stores FP32 messages only")
+        msg = torch.empty((nblk, tabs.nedges), dtype=torch.float32, device=dev)
+        _lib.call("atsc3_minsum_csr_f32_dev", llr.data_ptr(), nblk, n, tabs.nc,
+                  tabs.nedges, tabs.cptr.data_ptr(), tabs.cidx.data_ptr(),
+                  tabs.vptr.data_ptr(), tabs.vedge.data_ptr(), int(iters),
+                  float(np.float32(alpha)), tot.data_ptr(), msg.data_ptr(),
+                  bits.data_ptr(), conv.data_ptr(), its.data_ptr(), bad.data_ptr(),
+                  stream)
+    return bits, conv, its, bad
+
+
+def _to_numpy(bits, conv, its, bad):
+    return (bits.cpu().numpy(), conv.cpu().numpy().astype(bool),
+            its.cpu().numpy(), bad.cpu().numpy())
+
+
+def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float32,
+                         msg_dtype="float32", device="cuda", engine="auto"):
+    """lab/m3_ldpc.py:509: (nblk, n) LLRs -> (bits, conv, its, bad) numpy.
+
+    dtype must be float32 (the upstream's compiled-kernel path); the float64
+    update-then-check variant is not served on the GPU.
+    """
+    if np.dtype(dtype) != np.float32:
+        raise NotImplementedError("GPU min-sum serves the float32 (compiled-kernel) semantics")
+    code = resolve_code(checks, n)
+    lam = np.asarray(lam2d) if not isinstance(lam2d, torch.Tensor) else lam2d
+    nblk = int(lam.shape[0])
+    if nblk == 0:
+        return (np.zeros((0, n), np.uint8), np.zeros(0, bool),
+                np.full(0, iters, np.int32), np.full(0, -1, np.int32))
+    t = torch.as_tensor(np.ascontiguousarray(lam, np.float32)) if not isinstance(lam, torch.Tensor) else lam
+    t = t.to(device=device, dtype=torch.float32)
+    return _to_numpy(*decode_dev(t, code, iters, alpha, msg_dtype, engine))
+
+
+class GpuMinSum:
+    """lab/m9_gpu.py:84: batched min-sum for ONE code, built once, reused."""
+
+    def __init__(self, checks, n=None, iters=50, alpha=0.75, device="cuda",
+                 dtype="float32", msg_dtype="float32", engine="auto"):
+        if dtype not in ("float32",):
+            raise NotImplementedError("GPU min-sum serves float32 semantics")
+        self.code = resolve_code(checks, n if n is not None else getattr(checks, "n", None))
+        self.n = self.code.n
+        self.iters, self.alpha = iters, alpha
+        self.device = torch.device(device)
+        self.msg_dtype, self.engine = msg_dtype, engine
+        self.tables = device_tables(self.code, self.device)
+        self.stats = dict(calls=0, iters=0, blocks=0)
+
+    def decode(self, llr2d, iters=None, alpha=None):
+        iters = self.iters if iters is None else iters
+        alpha = self.alpha if alpha is None else alpha
+        if isinstance(llr2d, torch.Tensor):
+            t = llr2d.to(self.device, dtype=torch.float32)
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(llr2d, np.float32)).to(self.device)
+        assert t.shape[1] == self.n
+        out = _to_numpy(*decode_dev(t, self.code, iters, alpha, self.msg_dtype, self.engine))
+        self.stats["calls"] += 1
+        self.stats["blocks"] += int(t.shape[0])
+        self.stats["iters"] += int(out[2].sum())
+        return out
+
+
+def ladder_dev(llr, code, ladder=(1.0, 0.85, 0.75), iters=50, msg_dtype="float32",
+               engine="auto"):
+    """Alpha ladder on device tensors; returns device (bits, conv, its, bad).
+
+    One host sync per rung (the number of rows still failing)."""
+    nb = llr.shape[0]
+    dev = llr.device
+    bits = torch.empty((nb, code.n), dtype=torch.uint8, device=dev)
+    conv = torch.zeros(nb, dtype=torch.uint8, device=dev)
+    its = torch.zeros(nb, dtype=torch.int32, device=dev)
+    bad = torch.full((nb,), 1 << 30, dtype=torch.int32, device=dev)
+    todo = torch.arange(nb, device=dev)
+    for a in ladder:
+        if todo.numel() == 0:
+            break
+        sub = llr if todo.numel() == nb else llr.index_select(0, todo)
+        b, c, it, d = decode_dev(sub, code, iters, a, msg_dtype, engine)
+        take = (c != 0) | (d < bad.index_select(0, todo))
+        sel = todo[take]
+        bits[sel] = b[take]
+        its[sel] = it[take]
+        bad[sel] = d[take]
+        conv[sel] = c[take]
+        todo = todo[c == 0]
+    return bits, conv, its, bad
+
+
+def ladder_decode(lam2d, checks, n, ladder=(1.0, 0.85, 0.75), iters=50,
+                  msg_dtype="float32", device="cuda"):
+    code = resolve_code(checks, n)
+    t = torch.as_tensor(np.ascontiguousarray(lam2d, np.float32)).to(device)
+    return _to_numpy(*ladder_dev(t, code, ladder, iters, msg_dtype))
+
+
+def minsum_f32_host(lam2d, checks, n, iters=50, alpha=0.75):
+    """The host-pointer drop-in ``ldpc_minsum_f32`` (lab/ldpc_kernel.c:65),
+    driven exactly the way lab/m3_ldpc.py:485-506 drives the upstream one."""
+    code = resolve_code(checks, n)
+    cptr, cidx, vptr, vedge = code.csr()
+    llr = np.ascontiguousarray(lam2d, np.float32)
+    nblk = llr.shape[0]
+    bits = np.empty((nblk, n), np.uint8)
+    conv = np.zeros(nblk, np.uint8)
+    its = np.empty(nblk, np.int32)
+    bad = np.empty(nblk, np.int32)
+    work = np.empty(n + len(cidx), np.float32)
+    p = lambda a: a.ctypes.data
+    _lib.call("ldpc_minsum_f32", p(llr), nblk, n, len(cptr) - 1, len(cidx), p(cptr),
+              p(cidx), p(vptr), p(vedge), int(iters), float(np.float32(alpha)), p(bits),
+              p(conv), p(its), p(bad), p(work))
+    return bits, conv.view(bool), its, bad

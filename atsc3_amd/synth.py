@@ -1,0 +1,75 @@
+"""Seeded synthetic workloads for the demap + decode path.
+
+Everything here is deterministic given (seed, shapes): numpy's PCG64
+``default_rng``.  Used by the tests, the smoke check and bench.py; none of it
+is on the timed path.
+
+Transmit chain (the shape of lab/m6_bicm.py:367-373 with BCH and scrambler
+bypassed): uniform info bits -> LDPC encode -> q = cw[lam_of_q] -> NUC map.
+
+Channels:
+  awgn      y = x + n,  n ~ CN(0, N0),  N0 = 10^(-Es/N0 / 10), E|x|^2 = 1
+  rayleigh  y = h x + n, h = g1 + j g2 with g ~ N(0, 1) per cell, i.e. |h| is
+            Rayleigh(sigma = 1) with uniform phase (E|h|^2 = 2); perfect CSI:
+            the receiver gets z = y / h and w = |h|^2
+  ldm       y = (x_core + 10^(inj/20) x_enh) / sqrt(1 + 10^(inj/10)) + n
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import bicm
+from .codes import get_code
+
+
+def n0_of(esn0_db):
+    return 10.0 ** (-esn0_db / 10.0)
+
+
+def codewords(rate, n, nblk, seed):
+    code = get_code(rate, n)
+    rng = np.random.default_rng(seed)
+    info = rng.integers(0, 2, (nblk, code.K), dtype=np.uint8)
+    return code.encode(info)
+
+
+def modulate(cw, n, mod, rate):
+    lq = bicm.bicm_map(n, mod, rate)
+    return bicm.map_cells(np.asarray(cw)[:, lq], mod, rate)
+
+
+def complex_noise(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)
+
+
+def awgn_blocks(rate, n, mod, nblk, esn0_db, seed):
+    """(cw uint8 (B,n), cells complex128 (B,C), N0)."""
+    cw = codewords(rate, n, nblk, seed)
+    x = modulate(cw, n, mod, rate)
+    rng = np.random.default_rng(seed + 7919)
+    n0 = n0_of(esn0_db)
+    y = x + complex_noise(rng, x.shape) * np.sqrt(n0)
+    return cw, y, n0
+
+
+def rayleigh_blocks(rate, n, mod, nblk, esn0_db, seed):
+    """(cw, equalised cells z, csi w = |h|^2, N0) under per-cell Rayleigh."""
+    cw = codewords(rate, n, nblk, seed)
+    x = modulate(cw, n, mod, rate)
+    rng = np.random.default_rng(seed + 104729)
+    h = rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape)
+    n0 = n0_of(esn0_db)
+    y = h * x + complex_noise(rng, x.shape) * np.sqrt(n0)
+    return cw, y / h, np.abs(h) ** 2, n0
+
+
+def llr_bpsk_like(cw, snr_db, seed, scale=None):
+    """Quick float32 LLRs for decoder-only tests: (1-2b)*a + sigma*N(0,1)."""
+    cw = np.asarray(cw)
+    rng = np.random.default_rng(seed)
+    sigma = np.float32(10 ** (-snr_db / 20.0))
+    a = np.float32(1.0)
+    noise = rng.standard_normal(cw.shape, dtype=np.float32)
+    y = (np.float32(1) - np.float32(2) * cw.astype(np.float32)) * a + sigma * noise
+    s = np.float32(2.0) / (sigma * sigma) if scale is None else np.float32(scale)
+    return (y * s).astype(np.float32)

@@ -1,0 +1,119 @@
+/* atsc3_fec.h -- C ABI of libatsc3fec.so, the MI355X (gfx950) demap + LDPC
+ * min-sum library.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Functions whose name ends in _dev take
+ *     DEVICE pointers (memory the caller allocated on the current HIP device)
+ *     and enqueue work on `stream` (a hipStream_t passed as void*, NULL = the
+ *     null stream); they return as soon as the work is enqueued.
+ *   - Functions without the suffix take HOST pointers and are synchronous.
+ *   - Every entry point returns 0 on success or a negative ATSC3_E* code; on
+ *     error nothing is written to the outputs.
+ *   - LLR sign convention: llr > 0 means bit 0 (lab/m6_bicm.py:213).
+ *   - Thread safety: no global mutable state; concurrent calls on different
+ *     streams with disjoint buffers are safe (the host-pointer drop-in
+ *     allocates its own device buffers and stream per call).
+ */
+#ifndef ATSC3_FEC_H
+#define ATSC3_FEC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATSC3_OK 0
+#define ATSC3_E_DEGREE (-1)   /* a check degree exceeds the supported maximum */
+#define ATSC3_E_ARG (-2)      /* inconsistent sizes or NULL pointers          */
+#define ATSC3_E_HIP (-3)      /* a HIP runtime call or kernel launch failed   */
+#define ATSC3_E_SIZE (-4)     /* a size exceeds a fixed capacity             */
+
+/* ------------------------------------------------------------------------
+ * Drop-in for the upstream compiled kernel.
+ * Replaces lab/ldpc_kernel.c:52  int32_t ldpc_kernel_abi(void)
+ * ------------------------------------------------------------------------ */
+int32_t ldpc_kernel_abi(void);
+
+/* Replaces lab/ldpc_kernel.c:65-71 ldpc_minsum_f32 with identical arguments
+ * and results (bits, conv, its, bad bit-exact).  Host pointers; the decode
+ * runs on the current HIP device (H2D, kernel, D2H) and `work` is unused.
+ * Returns -1 when a check degree exceeds 128, like the upstream kernel.   */
+int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, int32_t nc,
+                        int32_t nedges, const int32_t *cptr,
+                        const int32_t *cidx, const int32_t *vptr,
+                        const int32_t *vedge, int32_t iters, float alpha,
+                        uint8_t *bits, uint8_t *conv, int32_t *its,
+                        int32_t *bad, float *work);
+
+/* ------------------------------------------------------------------------
+ * Device-pointer min-sum over the Tanner graph in CSR form.
+ * Same semantics as ldpc_minsum_f32 (lab/ldpc_kernel.c:65-146); serves any
+ * parity-check matrix.  Workspace: tot_ws [nblk*n] f32, msg_ws [nblk*nedges]
+ * f32 (contents on entry are ignored).
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
+                                 int32_t nc, int32_t nedges,
+                                 const int32_t *cptr, const int32_t *cidx,
+                                 const int32_t *vptr, const int32_t *vedge,
+                                 int32_t iters, float alpha, float *tot_ws,
+                                 float *msg_ws, uint8_t *bits, uint8_t *conv,
+                                 int32_t *its, int32_t *bad, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Device-pointer min-sum over the 360-lifted QC base graph.
+ * The fast path behind the upstream min-sum calls (lab/m3_ldpc.py:509-630
+ * min_sum_decode_batch at float32 -> lab/ldpc_kernel.c:65; lab/m9_gpu.py:138
+ * GpuMinSum.decode).  Table words are produced by atsc3_amd.codes.LdpcCode.qc():
+ *   cg_ptr[ncg+1], be_word[nbe], vg_ptr[nvg+1], ve_word[nbe].
+ * msg_fmt selects the check-to-variable message storage:
+ *   0 = FP32 (plain mode: bits, conv, its, bad bit-exact with
+ *       lab/ldpc_kernel.c on the same LLRs),
+ *   1 = FP16 storage, 2 = INT8 storage with a per-block scale
+ *       (msg_scale[b] = value of one INT8 step for block b; NULL -> derived
+ *       on the device as 8 * mean|llr| / 127, saturating at +-127 steps).
+ * Variable sums always accumulate in FP32.  Any check degree; variable
+ * degree <= 32 (every A/322 code), else ATSC3_E_DEGREE.
+ * Workspace: tot_ws [nblk*n] f32; msg_ws nblk*nbe*360 elements of the
+ * message type (4, 2 or 1 bytes each).
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n,
+                            int32_t ncg, int32_t nvg, int32_t nbe,
+                            const int32_t *cg_ptr, const int32_t *be_word,
+                            const int32_t *vg_ptr, const int32_t *ve_word,
+                            int32_t max_cdeg, int32_t max_vdeg, int32_t iters,
+                            float alpha, int32_t msg_fmt,
+                            const float *msg_scale, float *tot_ws,
+                            void *msg_ws, uint8_t *bits, uint8_t *conv,
+                            int32_t *its, int32_t *bad, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Max-log NUC demapper with fused bit de-interleave.
+ * Replaces the upstream demap calls lab/m6_bicm.py:213-225 (demap_llr),
+ * lab/m9_fast.py:756 (FrameDecoder.demap_batch), lab/m44_ldm.py:2454
+ * (demap_batch_qam) and lab/m9_gpu.py:301 (GpuBicm.demap), followed by the
+ * caller's scatter lam[lam_of_q] = q (lab/m6_bicm.py:331-335).
+ *
+ *   cells   [nblk*ncell] complex64 (interleaved re, im), equalised
+ *   csi     [nblk*ncell] f32 per-cell weight |h|^2, or NULL (weight 1)
+ *   sigma2  [nblk] f64 noise variance, or NULL: estimated per block as
+ *           max(mean(w * min_p |z - p|^2), 1e-9) in FP64 on the device
+ *   points  [M] complex64, M = 2^nbits, label = index (MSB = y0)
+ *   lam_of_q[ncell*nbits] int32 codeword position of q-stream bit
+ *   llr_out [nblk*ncell*nbits] f32 in codeword order
+ *   sigma2_out [nblk] f64 (the value used), may be NULL
+ *   ws      f64 scratch of atsc3_demap_ws_doubles(nblk, ncell) elements,
+ *           needed only when sigma2 is NULL
+ * LLR_i = w * (min_{y_i=1} |z-p|^2 - min_{y_i=0} |z-p|^2) / sigma2.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_demap_ws_doubles(int32_t nblk, int32_t ncell);
+int32_t atsc3_demap_dev(const float *cells, const float *csi,
+                        const double *sigma2, int32_t nblk, int32_t ncell,
+                        const float *points, int32_t nbits,
+                        const int32_t *lam_of_q, float *llr_out,
+                        double *sigma2_out, double *ws, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATSC3_FEC_H */

@@ -1,0 +1,80 @@
+"""GPU demapper parity against the FP64 oracle: normwise relative error
+max|got - want| / max|want| <= 1e-5 per block (the criterion of
+lab/m16_demap_fast.py:136-139), sigma2 estimate within 1e-6 relative."""
+import numpy as np
+import pytest
+
+from oracle import demap as OD
+from atsc3_amd import bicm, synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.mark.parametrize("n,mod,rate,esn0", [
+    (64800, "256QAM", "10/15", 20.0), (64800, "256QAM", "6/15", 14.0),
+    (64800, "256QAM", "8/15", 17.0), (64800, "64QAM", "10/15", 15.0),
+    (64800, "16QAM", "8/15", 10.0), (64800, "QPSK", "4/15", 0.0),
+    (16200, "64QAM", "9/15", 12.0), (16200, "QPSK", "9/15", 2.0),
+    (16200, "256QAM", "4/15", 8.0)])
+@pytest.mark.parametrize("given_sigma", [True, False])
+def test_demap_vs_fp64_oracle(device, n, mod, rate, esn0, given_sigma):
+    import torch
+    from atsc3_amd.demap import Demapper
+    print(f"[gpu] demap {n} {mod} {rate} sigma_given={given_sigma}", flush=True)
+    cw, y, n0 = synth.awgn_blocks(rate, n, mod, 3, esn0, 31)
+    y = y.astype(np.complex64).astype(np.complex128)      # the kernel's input precision
+    want, s2w = OD.demap(y, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate),
+                         sigma2=n0 if given_sigma else None)
+    dm = Demapper(n, mod, rate, device)
+    z = torch.from_numpy(y.astype(np.complex64)).to(device)
+    llr, s2 = dm.demap_dev(z, sigma2=n0 if given_sigma else None)
+    torch.cuda.synchronize()
+    got = llr.cpu().numpy().astype(np.float64)
+    err = OD.normwise_rel_err(got, want)
+    assert (err <= TOL).all(), err
+    assert np.allclose(s2.cpu().numpy(), s2w, rtol=1e-6, atol=0)
+
+
+def test_demap_with_csi_rayleigh(device):
+    import torch
+    from atsc3_amd.demap import Demapper
+    n, mod, rate = 64800, "256QAM", "10/15"
+    cw, z, w, n0 = synth.rayleigh_blocks(rate, n, mod, 2, 20.0, 5)
+    z = z.astype(np.complex64).astype(np.complex128)
+    w = w.astype(np.float32).astype(np.float64)
+    for s2 in (n0, None):
+        want, s2w = OD.demap(z, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate), csi=w, sigma2=s2)
+        dm = Demapper(n, mod, rate, device)
+        llr, s2g = dm.demap_dev(torch.from_numpy(z.astype(np.complex64)).to(device),
+                                csi=torch.from_numpy(w.astype(np.float32)).to(device), sigma2=s2)
+        torch.cuda.synchronize()
+        err = OD.normwise_rel_err(llr.cpu().numpy().astype(np.float64), want)
+        assert (err <= TOL).all(), err
+        assert np.allclose(s2g.cpu().numpy(), s2w, rtol=1e-6, atol=0)
+
+
+def test_demap_reference_signatures(device):
+    from atsc3_amd import demap as D
+    n, mod, rate = 16200, "64QAM", "11/15"
+    _, y, n0 = synth.awgn_blocks(rate, n, mod, 2, 16.0, 8)
+    y = y.astype(np.complex64).astype(np.complex128)
+    q = D.demap_llr(y[0], mod, rate)
+    want, _ = OD.demap_q(y[:1], bicm.points(mod, rate))
+    assert OD.normwise_rel_err(q[None].astype(np.float64), want)[0] <= TOL
+    qb, s2 = D.demap_batch(y, bicm.points(mod, rate), sigma2=n0)
+    want, _ = OD.demap_q(y, bicm.points(mod, rate), sigma2=n0)
+    assert (OD.normwise_rel_err(qb.astype(np.float64), want) <= TOL).all()
+
+
+def test_demap_noiseless_decodes(device):
+    """Noiseless cells -> LLR signs are the codeword bits in codeword order."""
+    import torch
+    from atsc3_amd.demap import Demapper
+    n, mod, rate = 64800, "256QAM", "10/15"
+    cw = synth.codewords(rate, n, 2, 3)
+    x = synth.modulate(cw, n, mod, rate)
+    dm = Demapper(n, mod, rate, device)
+    llr, _ = dm.demap_dev(torch.from_numpy(x.astype(np.complex64)).to(device), sigma2=0.01)
+    got = (llr.cpu().numpy() < 0).astype(np.uint8)
+    assert np.array_equal(got, cw)
