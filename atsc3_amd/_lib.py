@@ -25,9 +25,10 @@ _SIGS = {
     "atsc3_minsum_csr_f32_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
                                  c_vp, c_i32, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp],
-    "atsc3_minsum_qc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
-                            c_vp, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp,
-                            c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_qc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp,
+                            c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i32,
+                            c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                            c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],

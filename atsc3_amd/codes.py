@@ -274,6 +274,104 @@ class LdpcCode:
                     vg_ptr=vg_ptr, ve_word=ve_word,
                     max_cdeg=int(cdeg.max()), max_vdeg=int(vdeg.max()))
 
+    ROW = 384          # padded row length of the device message / tot arrays
+
+    @functools.lru_cache(maxsize=None)
+    def kernel_tables(self):
+        """Packed int32 tables of the QC HIP kernel (derived from ``qc()``).
+
+        Device layout: the message of base edge b at variable position p is
+        msg[b*ROW + p]; the posterior of variable (t, p) is tot[t*ROW + p]
+        (rows padded to ROW = 384 so every wave64 row access is 128-byte
+        aligned).
+
+          cg_info[ncg]   dc | b0 << 9 | has_missing << 20   (check groups)
+          be_off[2*nbe]  (s, t*ROW) per base edge: check (g, j) meets variable
+                         position p = (j + s) % 360 of group t
+          be_miss[nbe]   missing check position + 1 (0 = full circulant)
+          vg_info[nvg]   dv | n0 << 6 | e0 << 12 | has_missing << 23, n0 =
+                         entries in region 0 (checks < M1; 0 for Type B)
+          ve_off[2*nbe]  (b*ROW, u) per entry: variable (t, p) meets check
+                         position (p + u) % 360; entries in summation order
+                         at p = 0 (region, u, g)
+          ve_miss[nbe]   missing variable position + 1 (0 = full)
+        """
+        q = self.qc()
+        R = self.ROW
+        w = q["be_word"].astype(np.int64)
+        s, t, miss = w & 511, (w >> 9) & 511, (w >> 18) & 1023
+        cdeg = np.diff(q["cg_ptr"]).astype(np.int64)
+        b0 = q["cg_ptr"][:-1].astype(np.int64)
+        gmiss = np.zeros(q["ncg"], np.int64)
+        for b in np.flatnonzero(miss):
+            gmiss[np.searchsorted(q["cg_ptr"], b, side="right") - 1] = 1
+        cg_info = cdeg | (b0 << 9) | (gmiss << 20)
+        be_off = np.stack([s, t * R], 1).ravel()
+        vw = q["ve_word"].astype(np.int64)
+        vb, vu, vreg, vmiss = vw & 2047, (vw >> 11) & 511, (vw >> 20) & 1, (vw >> 21) & 1023
+        vdeg = np.diff(q["vg_ptr"]).astype(np.int64)
+        e0 = q["vg_ptr"][:-1].astype(np.int64)
+        n0 = np.add.reduceat(1 - vreg, e0) if len(vreg) else np.zeros(0, np.int64)
+        n0 = np.where(vdeg > 0, n0, 0)
+        vgm = np.add.reduceat((vmiss > 0).astype(np.int64), e0)
+        assert vdeg.max() < 64 and n0.max() < 64 and e0.max() < 2048
+        vg_info = vdeg | (n0 << 6) | (e0 << 12) | ((vgm > 0).astype(np.int64) << 23)
+        ve_off = np.stack([vb * R, vu], 1).ravel()
+        # processing order and steps: groups of one phase are independent, so
+        # they are sorted by degree and packed G at a time into steps (more
+        # independent loads in flight per wave for low-degree groups)
+        tvar = np.arange(q["nvg"], dtype=np.int64)
+        vg_info_t = vg_info | (tvar << 24)
+        c_order, c_steps = self._pack_steps(cdeg, gmiss, self.CHECK_PACK, 16)
+        v_order, v_steps = self._pack_steps(vdeg, (vgm > 0).astype(np.int64),
+                                            self.VAR_PACK, 32)
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)
+        return dict(ncg=q["ncg"], nvg=q["nvg"], nbe=q["nbe"], type_a=int(self.type == "A"),
+                    cg_info=i32(cg_info[c_order]), be_off=i32(be_off), be_miss=i32(miss),
+                    vg_info=i32(vg_info_t[v_order]), ve_off=i32(ve_off), ve_miss=i32(vmiss),
+                    c_steps=i32(c_steps), v_steps=i32(v_steps),
+                    max_cdeg=q["max_cdeg"], max_vdeg=q["max_vdeg"])
+
+    # groups per step by degree: check steps (degree <= 8 -> 2 groups),
+    # variable steps (<= 4 -> 4 groups, <= 8 -> 2 groups)
+    CHECK_PACK = ((8, 2),)
+    VAR_PACK = ((4, 4), (8, 2))
+
+    @staticmethod
+    def _pack_steps(deg, special, pack, fast_max):
+        """Order groups by (special, degree) and cut them into steps.
+
+        step word: D | G << 6 | start << 9 | generic << 21, where the step
+        covers processing-order entries [start, start + G) all of degree D;
+        generic steps (a missing entry, or D > fast_max) have G = 1 and a
+        degree field saturated at 63 (the kernel reads the degree from the
+        group word).
+        """
+        deg = np.asarray(deg, np.int64)
+        generic = (np.asarray(special) > 0) | (deg > fast_max)
+        order = np.lexsort((np.arange(len(deg)), -deg, generic))
+        steps = []
+        i = 0
+        while i < len(order):
+            g = order[i]
+            d = int(deg[g])
+            if generic[g]:
+                steps.append(min(d, 63) | (1 << 6) | (i << 9) | (1 << 21))
+                i += 1
+                continue
+            gmax = 1
+            for lim, gg in pack:
+                if d <= lim:
+                    gmax = max(gmax, gg)
+            run = 1
+            while (run < gmax and i + run < len(order) and not generic[order[i + run]]
+                   and deg[order[i + run]] == d):
+                run += 1
+            k = 1 << (run.bit_length() - 1)        # the kernel has G in {1, 2, 4}
+            steps.append(d | (k << 6) | (i << 9))
+            i += k
+        return order, np.asarray(steps, np.int64)
+
     # -- encoder ------------------------------------------------------------
     def encode(self, s):
         """Systematic A/322 encoder, vectorised over a batch.

@@ -6,44 +6,60 @@
 //     unsat = #checks whose XOR of (tot[v] < 0) is 1       (syndrome first)
 //     unsat == 0 -> bits = tot < 0, conv = 1, its = it, bad = 0, stop
 //     it == iters -> bits = tot < 0, conv = 0, its = iters, bad = unsat, stop
-//     per check: x = tot[v] - m; min1/min2 of |x|; the first edge (lowest
-//       variable index) attaining min1 takes min2, the others min1; times
-//       alpha; negated iff parity(x < 0) ^ (x < 0)
+//     per check: x = tot[v] - m; min1/min2 of |x|; the first edge attaining
+//       min1 takes min2, the others min1; times alpha; negated iff
+//       parity(x < 0) ^ (x < 0)
 //     per variable: s = 0; s += m in ascending check index; tot = lam + s
 //
 // Layout.  Check (g, j) -- group g, position j in 0..359 -- and variable
 // (t, p).  Base edge b joins check group g to variable group t with shift s:
-// check (g, j) meets variable t*360 + (j+s)%360.  The message of that edge
-// lives at msg[b*360 + j], so lane j of the workgroup reads 360 consecutive
-// words per base edge in the check phase, and lane p reads a rotated run of
-// 360 consecutive words in the variable phase.  HBM sees only full lines.
+// check (g, j) meets variable position p = (j + s) % 360 of group t.  The
+// message of that edge is stored at the VARIABLE position, msg[b*384 + p],
+// and the posterior at tot[t*384 + p] (rows padded to 384 words).  Lane j of
+// the check phase computes p once per edge and uses it for the tot load,
+// the message load and the message store; lane p of the variable phase
+// reads msg[b*384 + p] -- a fixed, 128-byte aligned row slice -- with no
+// address arithmetic at all.
 //
 // Bit-exactness notes.
-//  * The first-minimum rule is evaluated as a lexicographic minimum of
-//    (|x|, variable index); NaN and +inf magnitudes rank as +inf, which is
-//    exactly what the strict '<' scan from min1 = min2 = +inf does.
+//  * min1/min2 are a running minimum and a running median-of-three
+//    (fmed3(min1, min2, |x|) is the new second minimum): the multiset
+//    semantics of the strict '<' scan.  NaN magnitudes are mapped to +inf
+//    first, which is what the scan does with them (NaN < m is false).
+//  * "The first edge attaining min1 takes min2" is applied as "an edge with
+//    |x| == min1 takes min2": when min1 < min2 exactly one edge attains it;
+//    when two or more tie, min2 == min1 and every edge gets the same value,
+//    whichever is called first.  Products min*alpha are formed once per
+//    check from the same operands: identical values.
 //  * The per-variable sum must be sequential in ascending check index.  The
-//    host sorts each variable group's entries by (region, u, g); at position
-//    p the entries whose check position wrapped past 359 come first, so the
-//    lane adds (region 0: wrapped, unwrapped), then (region 1: wrapped,
-//    unwrapped) with predicated adds -- one FP32 add per message, in order.
-//  * Check degrees up to 32 keep x in registers; larger degrees (64800
-//    r5/15 47, r7/15 86) recompute x in a second pass from the same operands.
+//    host sorts each variable group's entries by (region, u, g) with u =
+//    (360 - s) % 360; at position p the entries whose check position
+//    wrapped past 359 (p + u >= 360) are the last ones of their region and
+//    come first in check order.  The lane adds, per region, the wrapped
+//    suffix then the unwrapped prefix -- one FP32 add per message, in order.
 //  * Compiled with -ffp-contract=off: no FMA contraction anywhere.
-//  * The syndrome of iteration `it` is computed from the same tot loads as
-//    the check update; when it reads zero the (speculative) message writes
-//    are simply never consumed -- tot is not touched until the variable
-//    phase, so the reported bits are those the syndrome saw.
+//  * The syndrome of iteration `it` comes from the same tot loads as the
+//    check update; when it reads zero the speculative message writes are
+//    never consumed -- tot is untouched until the variable phase.
 #include <hip/hip_fp16.h>
 
 #include "fec_common.h"
+
+#define QC_ROW 384
 
 namespace atsc3 {
 
 struct QcArgs {
     const float *llr;
     int n, ncg, nvg, nbe;
-    const int *cg_ptr, *be_word, *vg_ptr, *ve_word;
+    const int *cg_info;
+    const int2 *be_off;
+    const int *be_miss;
+    const int *vg_info;
+    const int2 *ve_off;
+    const int *ve_miss;
+    const int *c_steps, *v_steps;
+    int nsc, nsv;
     int iters;
     float alpha;
     const float *msg_scale;
@@ -98,16 +114,212 @@ __device__ float int8_step_of_block(const float *lam, int n, float *fslots) {
     return fmaxf(8.0f * mean / 127.0f, 1e-30f);
 }
 
-template <int FMT, int MAXDC, int MAXDV>
+__device__ __forceinline__ unsigned wrap360(unsigned p) { return min(p, p - 360u); }
+
+__device__ __forceinline__ float clamp_mag(float x) { return fminf(fabsf(x), INFINITY); }
+
+// G checks (g_0..g_{G-1}, j), all of degree D: every load of the G checks
+// is issued before the first store, then each check folds and emits.
+template <int FMT, int D, int G>
+__device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__restrict__ tot,
+                                                typename Msg<FMT>::T *__restrict__ msg,
+                                                unsigned j, int start, bool first, bool update,
+                                                float step, float inv_step) {
+    using M = Msg<FMT>;
+    float tv[G][D], x[G][D];
+    int b0[G];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        b0[gi] = (a.cg_info[start + gi] >> 9) & 2047;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int2 e = a.be_off[b0[gi] + k];
+            const unsigned p = wrap360(j + (unsigned)e.x);
+            tv[gi][k] = (tot + e.y)[p];
+            const float mo =
+                first ? 0.0f : M::ld((msg + (size_t)(b0[gi] + k) * QC_ROW) + p, step);
+            x[gi][k] = tv[gi][k] - mo;
+        }
+    }
+    unsigned syn_total = 0u;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        unsigned syn = 0u, par = 0u;
+        float m1 = INFINITY, m2 = INFINITY;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            syn ^= (unsigned)(tv[gi][k] < 0.0f);
+            par ^= (unsigned)(x[gi][k] < 0.0f);
+            const float mag = clamp_mag(x[gi][k]);
+            const float n2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
+            m1 = fminf(m1, mag);
+            m2 = n2;
+        }
+        syn_total += syn;
+        if (update) {
+            const float a1 = m1 * a.alpha, a2 = m2 * a.alpha;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const unsigned p = wrap360(j + (unsigned)a.be_off[b0[gi] + k].x);
+                float val = clamp_mag(x[gi][k]) == m1 ? a2 : a1;
+                if (par ^ (unsigned)(x[gi][k] < 0.0f)) val = -val;
+                M::st((msg + (size_t)(b0[gi] + k) * QC_ROW) + p, val, inv_step);
+            }
+        }
+    }
+    return syn_total;
+}
+
+// Any degree, possibly with the missing corner entry: two passes, the
+// second recomputes x from the same operands (identical FP32 value).
+template <int FMT>
+__device__ unsigned check_generic(const QcArgs &a, const float *__restrict__ tot,
+                                  typename Msg<FMT>::T *__restrict__ msg, unsigned j, int b0,
+                                  int dc, bool first, bool update, float step,
+                                  float inv_step) {
+    using M = Msg<FMT>;
+    unsigned syn = 0u, par = 0u;
+    float m1 = INFINITY, m2 = INFINITY;
+    for (int k = 0; k < dc; ++k) {
+        if (a.be_miss[b0 + k] == (int)j + 1) continue;
+        const int2 e = a.be_off[b0 + k];
+        const unsigned p = wrap360(j + (unsigned)e.x);
+        const float tv = (tot + e.y)[p];
+        const float mo = first ? 0.0f : M::ld((msg + (size_t)(b0 + k) * QC_ROW) + p, step);
+        const float xv = tv - mo;
+        syn ^= (unsigned)(tv < 0.0f);
+        par ^= (unsigned)(xv < 0.0f);
+        const float mag = clamp_mag(xv);
+        const float n2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
+        m1 = fminf(m1, mag);
+        m2 = n2;
+    }
+    if (update) {
+        const float a1 = m1 * a.alpha, a2 = m2 * a.alpha;
+        for (int k = 0; k < dc; ++k) {
+            if (a.be_miss[b0 + k] == (int)j + 1) continue;
+            const int2 e = a.be_off[b0 + k];
+            const unsigned p = wrap360(j + (unsigned)e.x);
+            const float tv = (tot + e.y)[p];
+            const float mo =
+                first ? 0.0f : M::ld((msg + (size_t)(b0 + k) * QC_ROW) + p, step);
+            const float xv = tv - mo;
+            float val = clamp_mag(xv) == m1 ? a2 : a1;
+            if (par ^ (unsigned)(xv < 0.0f)) val = -val;
+            M::st((msg + (size_t)(b0 + k) * QC_ROW) + p, val, inv_step);
+        }
+    }
+    return syn;
+}
+
+// G variable groups of degree D at position p: all loads first, then the
+// ordered sums.
+template <int FMT, int D, int G, bool TA>
+__device__ __forceinline__ void var_multi(const QcArgs &a, const float *__restrict__ lam,
+                                          float *__restrict__ tot,
+                                          const typename Msg<FMT>::T *__restrict__ msg,
+                                          unsigned p, int start, float step) {
+    using M = Msg<FMT>;
+    float val[G][D], lv[G];
+    int w0[G], w1[G], n0[G], tt[G];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        const int info = a.vg_info[start + gi];
+        const int e0 = (info >> 12) & 2047;
+        n0[gi] = (info >> 6) & 63;
+        tt[gi] = (info >> 24) & 255;
+        lv[gi] = lam[tt[gi] * ATSC3_Z + p];
+        w0[gi] = 0;
+        w1[gi] = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int2 e = a.ve_off[e0 + k];
+            val[gi][k] = M::ld((msg + e.x) + p, step);
+            const int wr = (int)(p + (unsigned)e.y >= 360u);
+            if (TA && k < n0[gi])
+                w0[gi] += wr;
+            else
+                w1[gi] += wr;
+        }
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        float s = 0.0f;
+        if (TA) {
+            const int t0 = n0[gi] - w0[gi];
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+                if (k < n0[gi] && k >= t0) s += val[gi][k];
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+                if (k < n0[gi] && k < t0) s += val[gi][k];
+        }
+        const int t1 = D - w1[gi];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if ((!TA || k >= n0[gi]) && k >= t1) s += val[gi][k];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if ((!TA || k >= n0[gi]) && k < t1) s += val[gi][k];
+        tot[tt[gi] * QC_ROW + p] = lv[gi] + s;
+    }
+}
+
+// Variable group holding the missing corner entry (or degree > 32): runtime
+// degree up to 32, per-entry validity.
+template <int FMT, int VMAX>
+__device__ void var_generic(const QcArgs &a, const float *__restrict__ lam_t,
+                            float *__restrict__ tot_t,
+                            const typename Msg<FMT>::T *__restrict__ msg, unsigned p, int e0,
+                            int dv, int n0, float step) {
+    using M = Msg<FMT>;
+    float val[VMAX];
+    unsigned wrapped = 0u, valid = 0u;
+#pragma unroll
+    for (int k = 0; k < VMAX; ++k) {
+        val[k] = 0.0f;
+        if (k < dv) {
+            const int2 e = a.ve_off[e0 + k];
+            val[k] = M::ld((msg + e.x) + p, step);
+            wrapped |= (unsigned)(p + (unsigned)e.y >= 360u) << k;
+            valid |= (unsigned)(a.ve_miss[e0 + k] != (int)p + 1) << k;
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int w = 1; w >= 0; --w) {
+#pragma unroll
+            for (int k = 0; k < VMAX; ++k) {
+                const bool in_r = r == 0 ? k < n0 : k >= n0;
+                if (((valid >> k) & 1u) && in_r && ((wrapped >> k) & 1u) == (unsigned)w)
+                    s += val[k];
+            }
+        }
+    }
+    tot_t[p] = lam_t[p] + s;
+}
+
+#define QC_CASES4(X) X(1) X(2) X(3) X(4)
+#define QC_CASES8(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#define QC_CASES16(X) \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#define QC_CASES(X) \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) \
+    X(31) X(32)
+
+template <int FMT, bool TA, int CMAX, int VMAX>
 __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
     using M = Msg<FMT>;
     using T = typename M::T;
     const int blk = blockIdx.x;
-    const int tid = threadIdx.x;
+    const unsigned tid = threadIdx.x;
     const int n = a.n;
     const float *lam = a.llr + (size_t)blk * n;
-    float *tot = a.tot_ws + (size_t)blk * n;
-    T *msg = reinterpret_cast<T *>(a.msg_ws) + (size_t)blk * a.nbe * ATSC3_Z;
+    float *tot = a.tot_ws + (size_t)blk * a.nvg * QC_ROW;
+    T *msg = reinterpret_cast<T *>(a.msg_ws) + (size_t)blk * a.nbe * QC_ROW;
     __shared__ int slots[ATSC3_WG / 64];
     __shared__ float fslots[ATSC3_WG / 64];
 
@@ -116,96 +328,55 @@ __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
         step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, n, fslots);
         inv_step = 1.0f / step;
     }
+    if (tid < ATSC3_Z)
+        for (int t = 0; t < a.nvg; ++t) tot[t * QC_ROW + tid] = lam[t * ATSC3_Z + tid];
+    __syncthreads();
 
     for (int it = 0;; ++it) {
-        const float *src = it == 0 ? lam : tot;
+        const bool first = it == 0;
         const bool update = it < a.iters;
-        int unsat = 0;
+        unsigned unsat = 0u;
         if (tid < ATSC3_Z) {
-            const int j = tid;
-            for (int g = 0; g < a.ncg; ++g) {
-                const int b0 = a.cg_ptr[g];
-                const int dc = a.cg_ptr[g + 1] - b0;
-                unsigned syn = 0u, par = 0u;
-                float lo1 = INFINITY, lo2 = INFINITY;
-                int first = 0x7fffffff;
-                // x of edge k of this check, and its variable (-1 if the
-                // edge is the missing corner entry)
-                auto edge = [&](int k, float &xv, float &tv) -> int {
-                    const int w = a.be_word[b0 + k];
-                    const int s = w & 511;
-                    const int t = (w >> 9) & 511;
-                    const int miss = (w >> 18) & 1023;
-                    if (miss == j + 1) return -1;
-                    int p = j + s;
-                    if (p >= ATSC3_Z) p -= ATSC3_Z;
-                    const int v = t * ATSC3_Z + p;
-                    tv = src[v];
-                    const float mo =
-                        it == 0 ? 0.0f : M::ld(msg + (size_t)(b0 + k) * ATSC3_Z + j, step);
-                    xv = tv - mo;
-                    return v;
-                };
-                auto fold = [&](int v, float xv, float tv) {
-                    syn ^= (unsigned)(tv < 0.0f);
-                    par ^= (unsigned)(xv < 0.0f);
-                    float mag = fabsf(xv);
-                    mag = mag < INFINITY ? mag : INFINITY;
-                    if (mag < lo1 || (mag == lo1 && v < first)) {
-                        lo2 = lo1;
-                        lo1 = mag;
-                        first = v;
-                    } else if (mag < lo2) {
-                        lo2 = mag;
-                    }
-                };
-                auto emit = [&](int k, int v, float xv) {
-                    float val = (v == first ? lo2 : lo1) * a.alpha;
-                    if (par ^ (unsigned)(xv < 0.0f)) val = -val;
-                    M::st(msg + (size_t)(b0 + k) * ATSC3_Z + j, val, inv_step);
-                };
-                if constexpr (MAXDC > 0) {
-                    // degree bounded at compile time: keep x in registers
-                    float x[MAXDC > 0 ? MAXDC : 1];
-                    int vv[MAXDC > 0 ? MAXDC : 1];
-#pragma unroll
-                    for (int k = 0; k < MAXDC; ++k) {
-                        vv[k] = -1;
-                        x[k] = 0.0f;
-                        if (k < dc) {
-                            float tv = 0.0f;
-                            vv[k] = edge(k, x[k], tv);
-                            if (vv[k] >= 0) fold(vv[k], x[k], tv);
-                        }
-                    }
-                    if (update) {
-#pragma unroll
-                        for (int k = 0; k < MAXDC; ++k)
-                            if (k < dc && vv[k] >= 0) emit(k, vv[k], x[k]);
-                    }
-                } else {
-                    // any degree: second pass recomputes x from the same
-                    // operands (identical FP32 result) instead of holding it
-                    for (int k = 0; k < dc; ++k) {
-                        float xv = 0.0f, tv = 0.0f;
-                        const int v = edge(k, xv, tv);
-                        if (v >= 0) fold(v, xv, tv);
-                    }
-                    if (update) {
-                        for (int k = 0; k < dc; ++k) {
-                            float xv = 0.0f, tv = 0.0f;
-                            const int v = edge(k, xv, tv);
-                            if (v >= 0) emit(k, v, xv);
-                        }
-                    }
+            const unsigned j = tid;
+            for (int si = 0; si < a.nsc; ++si) {
+                const int sw = a.c_steps[si];
+                const int D = sw & 63;
+                const int G = (sw >> 6) & 7;
+                const int start = (sw >> 9) & 4095;
+                if ((sw >> 21) & 1) {
+                    // generic step: the degree comes from the group word
+                    // (the step's 6-bit degree field saturates)
+                    const int info = a.cg_info[start];
+                    unsat += check_generic<FMT>(a, tot, msg, j, (info >> 9) & 2047, info & 511,
+                                                first, update, step, inv_step);
+                    continue;
                 }
-                unsat += (int)syn;
+                switch (G * 64 + D) {
+#define QC_C1(D)                                                                      \
+    case 64 + D:                                                                      \
+        if constexpr (D <= CMAX)                                                      \
+            unsat += check_multi<FMT, D, 1>(a, tot, msg, j, start, first, update, step, \
+                                            inv_step);                                \
+        break;
+#define QC_C2(D)                                                                       \
+    case 128 + D:                                                                      \
+        unsat += check_multi<FMT, D, 2>(a, tot, msg, j, start, first, update, step,    \
+                                        inv_step);                                     \
+        break;
+                    QC_CASES16(QC_C1)
+                    QC_CASES8(QC_C2)
+#undef QC_C1
+#undef QC_C2
+                    default: break;
+                }
             }
         }
-        const int total = wg_sum_i32(unsat, slots);
+        const int total = wg_sum_i32((int)unsat, slots);
         if (total == 0 || it == a.iters) {
             uint8_t *bout = a.bits + (size_t)blk * n;
-            for (int v = tid; v < n; v += ATSC3_WG) bout[v] = (uint8_t)(src[v] < 0.0f);
+            if (tid < ATSC3_Z)
+                for (int t = 0; t < a.nvg; ++t)
+                    bout[t * ATSC3_Z + tid] = (uint8_t)(tot[t * QC_ROW + tid] < 0.0f);
             if (tid == 0) {
                 a.conv[blk] = (uint8_t)(total == 0);
                 a.its[blk] = total == 0 ? it : a.iters;
@@ -214,98 +385,127 @@ __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
             return;
         }
         if (tid < ATSC3_Z) {
-            const int p = tid;
-            for (int t = 0; t < a.nvg; ++t) {
-                const int e0 = a.vg_ptr[t];
-                const int dv = a.vg_ptr[t + 1] - e0;
-                float val[MAXDV];
-                unsigned wrapped = 0u, valid = 0u, reg1 = 0u;
-#pragma unroll
-                for (int k = 0; k < MAXDV; ++k) {
-                    val[k] = 0.0f;
-                    if (k < dv) {
-                        const int w = a.ve_word[e0 + k];
-                        const int b = w & 2047;
-                        const int u = (w >> 11) & 511;
-                        const int region = (w >> 20) & 1;
-                        const int vmiss = (w >> 21) & 1023;
-                        int jj = p + u;
-                        const bool wr = jj >= ATSC3_Z;
-                        if (wr) jj -= ATSC3_Z;
-                        val[k] = M::ld(msg + (size_t)b * ATSC3_Z + jj, step);
-                        wrapped |= (unsigned)wr << k;
-                        valid |= (unsigned)(vmiss != p + 1) << k;
-                        reg1 |= (unsigned)region << k;
-                    }
+            const unsigned p = tid;
+            for (int si = 0; si < a.nsv; ++si) {
+                const int sw = a.v_steps[si];
+                const int D = sw & 63;
+                const int G = (sw >> 6) & 7;
+                const int start = (sw >> 9) & 4095;
+                if ((sw >> 21) & 1) {
+                    const int info = a.vg_info[start];
+                    const int t = (info >> 24) & 255;
+                    var_generic<FMT, VMAX>(a, lam + t * ATSC3_Z, tot + t * QC_ROW, msg, p,
+                                           (info >> 12) & 2047, info & 63, (info >> 6) & 63,
+                                           step);
+                    continue;
                 }
-                float s = 0.0f;
-#pragma unroll
-                for (int r = 0; r < 2; ++r) {
-#pragma unroll
-                    for (int w = 1; w >= 0; --w) {
-#pragma unroll
-                        for (int k = 0; k < MAXDV; ++k) {
-                            const bool take = ((valid >> k) & 1u) &&
-                                              ((reg1 >> k) & 1u) == (unsigned)r &&
-                                              ((wrapped >> k) & 1u) == (unsigned)w;
-                            if (take) s += val[k];
-                        }
-                    }
+                switch (G * 64 + D) {
+#define QC_V1(D)                                                                      \
+    case 64 + D:                                                                      \
+        if constexpr (D <= VMAX) var_multi<FMT, D, 1, TA>(a, lam, tot, msg, p, start, step); \
+        break;
+#define QC_V2(D)                                                                      \
+    case 128 + D:                                                                     \
+        var_multi<FMT, D, 2, TA>(a, lam, tot, msg, p, start, step);                   \
+        break;
+#define QC_V4(D)                                                                      \
+    case 256 + D:                                                                     \
+        var_multi<FMT, D, 4, TA>(a, lam, tot, msg, p, start, step);                   \
+        break;
+                    QC_CASES(QC_V1)
+                    QC_CASES8(QC_V2)
+                    QC_CASES4(QC_V4)
+#undef QC_V1
+#undef QC_V2
+#undef QC_V4
+                    default: break;
                 }
-                const int v = t * ATSC3_Z + p;
-                tot[v] = lam[v] + s;
             }
         }
         __syncthreads();
     }
 }
 
-template <int FMT, int DC, int DV>
-static int32_t launch_qc(const QcArgs &a, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL((minsum_qc_kernel<FMT, DC, DV>), dim3(nblk), dim3(ATSC3_WG), 0, st, a);
+template <int FMT, bool TA, int CMAX, int VMAX>
+static int32_t launch_one(const QcArgs &a, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL((minsum_qc_kernel<FMT, TA, CMAX, VMAX>), dim3(nblk), dim3(ATSC3_WG), 0,
+                       st, a);
     return launch_status();
 }
 
-template <int FMT>
-static int32_t dispatch_qc(const QcArgs &a, int nblk, int max_cdeg, int max_vdeg,
+// Degree tiers: the kernel is sized for the largest fast-path degree it
+// contains.  Check degrees above 16 (at most 4% of the checks of any code
+// here) take the generic two-pass path, which holds no per-edge registers;
+// variable degrees up to 16 or 32.
+template <int FMT, bool TA>
+static int32_t launch_tier(const QcArgs &a, int nblk, int max_cdeg, int max_vdeg,
                            hipStream_t st) {
-    if (max_cdeg <= 16 && max_vdeg <= 16) return launch_qc<FMT, 16, 16>(a, nblk, st);
-    if (max_cdeg <= 32 && max_vdeg <= 16) return launch_qc<FMT, 32, 16>(a, nblk, st);
-    if (max_cdeg <= 16 && max_vdeg <= 32) return launch_qc<FMT, 16, 32>(a, nblk, st);
-    if (max_cdeg <= 32) return launch_qc<FMT, 32, 32>(a, nblk, st);
-    if (max_vdeg <= 16) return launch_qc<FMT, 0, 16>(a, nblk, st);
-    return launch_qc<FMT, 0, 32>(a, nblk, st);
+    (void)max_cdeg;   // check degrees above 16 take the register-free path
+    if (max_vdeg <= 16) return launch_one<FMT, TA, 16, 16>(a, nblk, st);
+    return launch_one<FMT, TA, 16, 32>(a, nblk, st);
+}
+
+template <int FMT>
+static int32_t launch_qc(const QcArgs &a, int nblk, int type_a, int max_cdeg, int max_vdeg,
+                         hipStream_t st) {
+    return type_a ? launch_tier<FMT, true>(a, nblk, max_cdeg, max_vdeg, st)
+                  : launch_tier<FMT, false>(a, nblk, max_cdeg, max_vdeg, st);
 }
 
 }  // namespace atsc3
 
-extern "C" int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n,
-                                       int32_t ncg, int32_t nvg, int32_t nbe,
-                                       const int32_t *cg_ptr, const int32_t *be_word,
-                                       const int32_t *vg_ptr, const int32_t *ve_word,
+extern "C" int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n, int32_t ncg,
+                                       int32_t nvg, int32_t nbe, int32_t type_a,
+                                       const int32_t *cg_info, const int32_t *be_off,
+                                       const int32_t *be_miss, const int32_t *vg_info,
+                                       const int32_t *ve_off, const int32_t *ve_miss,
+                                       const int32_t *c_steps, int32_t nsc,
+                                       const int32_t *v_steps, int32_t nsv,
                                        int32_t max_cdeg, int32_t max_vdeg, int32_t iters,
-                                       float alpha, int32_t msg_fmt,
-                                       const float *msg_scale, float *tot_ws,
-                                       void *msg_ws, uint8_t *bits, uint8_t *conv,
-                                       int32_t *its, int32_t *bad, void *stream) {
+                                       float alpha,
+                                       int32_t msg_fmt, const float *msg_scale,
+                                       float *tot_ws, void *msg_ws, uint8_t *bits,
+                                       uint8_t *conv, int32_t *its, int32_t *bad,
+                                       void *stream) {
     if (nblk < 0 || iters < 0) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
-    if (!llr || !cg_ptr || !be_word || !vg_ptr || !ve_word || !tot_ws || !msg_ws ||
-        !bits || !conv || !its || !bad)
+    if (!llr || !cg_info || !be_off || !be_miss || !vg_info || !ve_off || !ve_miss ||
+        !c_steps || !v_steps || !tot_ws || !msg_ws || !bits || !conv || !its || !bad)
         return ATSC3_E_ARG;
-    if (n != nvg * ATSC3_Z || ncg <= 0 || nvg <= 0 || nvg > 511 || nbe <= 0 ||
-        nbe >= 2048)
+    if (nsc <= 0 || nsc > ncg || nsv <= 0 || nsv > nvg || nvg > 255) return ATSC3_E_ARG;
+    if (n != nvg * ATSC3_Z || ncg <= 0 || nvg <= 0 || nbe <= 0 || nbe >= 2048)
         return ATSC3_E_ARG;
-    if (max_cdeg < 1 || max_vdeg < 1) return ATSC3_E_ARG;
-    if (max_vdeg > 32) return ATSC3_E_DEGREE;   // check degree is unbounded
+    if (max_vdeg < 1 || max_cdeg < 1) return ATSC3_E_ARG;
+    if (max_vdeg > 32) return ATSC3_E_DEGREE;
     if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
-    atsc3::QcArgs a{llr,   n,        ncg,    nvg,      nbe,  cg_ptr, be_word,
-                    vg_ptr, ve_word, iters,  alpha,    msg_scale, tot_ws, msg_ws,
-                    bits,  conv,     its,    bad};
+    atsc3::QcArgs a{llr,
+                    n,
+                    ncg,
+                    nvg,
+                    nbe,
+                    cg_info,
+                    reinterpret_cast<const int2 *>(be_off),
+                    be_miss,
+                    vg_info,
+                    reinterpret_cast<const int2 *>(ve_off),
+                    ve_miss,
+                    c_steps,
+                    v_steps,
+                    nsc,
+                    nsv,
+                    iters,
+                    alpha,
+                    msg_scale,
+                    tot_ws,
+                    msg_ws,
+                    bits,
+                    conv,
+                    its,
+                    bad};
     hipStream_t st = atsc3::as_stream(stream);
     switch (msg_fmt) {
-        case 0: return atsc3::dispatch_qc<0>(a, nblk, max_cdeg, max_vdeg, st);
-        case 1: return atsc3::dispatch_qc<1>(a, nblk, max_cdeg, max_vdeg, st);
-        default: return atsc3::dispatch_qc<2>(a, nblk, max_cdeg, max_vdeg, st);
+        case 0: return atsc3::launch_qc<0>(a, nblk, type_a, max_cdeg, max_vdeg, st);
+        case 1: return atsc3::launch_qc<1>(a, nblk, type_a, max_cdeg, max_vdeg, st);
+        default: return atsc3::launch_qc<2>(a, nblk, type_a, max_cdeg, max_vdeg, st);
     }
 }

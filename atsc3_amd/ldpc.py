@@ -100,13 +100,14 @@ class DeviceTables:
         self.nedges = len(cidx)
         self.max_check_degree = int(np.diff(cptr).max(initial=0))
         try:
-            q = code.qc()
-        except ValueError:
+            q = code.kernel_tables()
+        except (ValueError, AttributeError):
             q = None
         self.qc = None
         if q is not None and q["max_vdeg"] <= 32:
             self.qc = dict(q)
-            for k in ("cg_ptr", "be_word", "vg_ptr", "ve_word"):
+            for k in ("cg_info", "be_off", "be_miss", "vg_info", "ve_off", "ve_miss",
+                      "c_steps", "v_steps"):
                 self.qc[k] = t(q[k])
 
 
@@ -152,23 +153,29 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
         raise ValueError("this code has no QC form with variable degree <= 32; "
                          "use engine='csr'")
     stream = _stream_ptr(dev)
-    tot = torch.empty((nblk, n), dtype=torch.float32, device=dev)
     if use_qc:
         q = tabs.qc
-        msg = torch.empty(nblk * q["nbe"] * 360 * MSG_BYTES[fmt], dtype=torch.uint8, device=dev)
+        row = LdpcCode.ROW
+        tot = torch.empty(nblk * q["nvg"] * row, dtype=torch.float32, device=dev)
+        msg = torch.empty(nblk * q["nbe"] * row * MSG_BYTES[fmt], dtype=torch.uint8, device=dev)
         scale_ptr = None
         if msg_scale is not None:
             msg_scale = torch.as_tensor(msg_scale, dtype=torch.float32, device=dev).contiguous()
+            if msg_scale.numel() != nblk:
+                raise ValueError("msg_scale needs one value per block")
             scale_ptr = msg_scale.data_ptr()
         _lib.call("atsc3_minsum_qc_dev", llr.data_ptr(), nblk, n, q["ncg"], q["nvg"],
-                  q["nbe"], q["cg_ptr"].data_ptr(), q["be_word"].data_ptr(),
-                  q["vg_ptr"].data_ptr(), q["ve_word"].data_ptr(), q["max_cdeg"],
-                  q["max_vdeg"], int(iters), float(np.float32(alpha)), fmt, scale_ptr,
-                  tot.data_ptr(), msg.data_ptr(), bits.data_ptr(), conv.data_ptr(),
-                  its.data_ptr(), bad.data_ptr(), stream)
+                  q["nbe"], q["type_a"], q["cg_info"].data_ptr(), q["be_off"].data_ptr(),
+                  q["be_miss"].data_ptr(), q["vg_info"].data_ptr(), q["ve_off"].data_ptr(),
+                  q["ve_miss"].data_ptr(), q["c_steps"].data_ptr(), len(q["c_steps"]),
+                  q["v_steps"].data_ptr(), len(q["v_steps"]), q["max_cdeg"], q["max_vdeg"],
+                  int(iters),
+                  float(np.float32(alpha)), fmt, scale_ptr, tot.data_ptr(), msg.data_ptr(),
+                  bits.data_ptr(), conv.data_ptr(), its.data_ptr(), bad.data_ptr(), stream)
     else:
         if fmt != 0:
             raise ValueError("the CSR engine stores FP32 messages only")
+        tot = torch.empty((nblk, n), dtype=torch.float32, device=dev)
         msg = torch.empty((nblk, tabs.nedges), dtype=torch.float32, device=dev)
         _lib.call("atsc3_minsum_csr_f32_dev", llr.data_ptr(), nblk, n, tabs.nc,
                   tabs.nedges, tabs.cptr.data_ptr(), tabs.cidx.data_ptr(),

@@ -73,3 +73,41 @@ def llr_bpsk_like(cw, snr_db, seed, scale=None):
     y = (np.float32(1) - np.float32(2) * cw.astype(np.float32)) * a + sigma * noise
     s = np.float32(2.0) / (sigma * sigma) if scale is None else np.float32(scale)
     return (y * s).astype(np.float32)
+
+
+class FadingStream:
+    """Device-side generator for the live-frame workload: a stream of 256-NUC
+    (or any NUC) blocks under per-cell Rayleigh fading with perfect CSI.
+
+    Codewords: ``ncw`` distinct seeded codewords, block i carries codeword
+    i % ncw; every block gets its own fading and noise, drawn on the device
+    from a seeded torch generator, so FER statistics are those of independent
+    channel realisations.  Produces equalised cells z = y / h (complex64) and
+    CSI weights w = |h|^2 (float32) resident in device memory.
+    """
+
+    def __init__(self, rate, n, mod, ncw, seed, device):
+        import torch
+        self.rate, self.n, self.mod = rate, n, mod
+        self.device = torch.device(device)
+        self.cw = codewords(rate, n, ncw, seed)
+        x = modulate(self.cw, n, mod, rate).astype(np.complex64)
+        self.x = torch.from_numpy(x).to(self.device)
+        self.cw_dev = torch.from_numpy(self.cw).to(self.device)
+        self.ncw = ncw
+        self.gen = torch.Generator(device=self.device)
+        self.gen.manual_seed(seed + 31337)
+
+    def batch(self, nblk, esn0_db, start=0):
+        import torch
+        idx = (torch.arange(nblk, device=self.device) + start) % self.ncw
+        x = self.x.index_select(0, idx)
+        shp = x.shape
+        g = lambda: torch.randn(shp, generator=self.gen, device=self.device,
+                                dtype=torch.float32)
+        h = torch.complex(g(), g())
+        n0 = n0_of(esn0_db)
+        nz = torch.complex(g(), g()) * float(np.sqrt(n0 / 2.0))
+        z = x + nz / h
+        w = (h.real * h.real + h.imag * h.imag).contiguous()
+        return z.contiguous(), w, idx, n0

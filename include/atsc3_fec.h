@@ -64,8 +64,11 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
  * Device-pointer min-sum over the 360-lifted QC base graph.
  * The fast path behind the upstream min-sum calls (lab/m3_ldpc.py:509-630
  * min_sum_decode_batch at float32 -> lab/ldpc_kernel.c:65; lab/m9_gpu.py:138
- * GpuMinSum.decode).  Table words are produced by atsc3_amd.codes.LdpcCode.qc():
- *   cg_ptr[ncg+1], be_word[nbe], vg_ptr[nvg+1], ve_word[nbe].
+ * GpuMinSum.decode).  Tables come from atsc3_amd.codes.LdpcCode
+ * .kernel_tables(): cg_info[ncg], be_off[2*nbe], be_miss[nbe],
+ * vg_info[nvg], ve_off[2*nbe], ve_miss[nbe], c_steps[nsc], v_steps[nsv]
+ * (groups in processing order, packed into steps of equal degree);
+ * type_a = 1 for A/322 Type A codes (two check regions).
  * msg_fmt selects the check-to-variable message storage:
  *   0 = FP32 (plain mode: bits, conv, its, bad bit-exact with
  *       lab/ldpc_kernel.c on the same LLRs),
@@ -74,15 +77,19 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
  *       on the device as 8 * mean|llr| / 127, saturating at +-127 steps).
  * Variable sums always accumulate in FP32.  Any check degree; variable
  * degree <= 32 (every A/322 code), else ATSC3_E_DEGREE.
- * Workspace: tot_ws [nblk*n] f32; msg_ws nblk*nbe*360 elements of the
- * message type (4, 2 or 1 bytes each).
+ * Workspace (contents on entry ignored): tot_ws [nblk * nvg * 384] f32;
+ * msg_ws nblk * nbe * 384 elements of the message type (4, 2 or 1 bytes).
  * ------------------------------------------------------------------------ */
 int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n,
                             int32_t ncg, int32_t nvg, int32_t nbe,
-                            const int32_t *cg_ptr, const int32_t *be_word,
-                            const int32_t *vg_ptr, const int32_t *ve_word,
-                            int32_t max_cdeg, int32_t max_vdeg, int32_t iters,
-                            float alpha, int32_t msg_fmt,
+                            int32_t type_a, const int32_t *cg_info,
+                            const int32_t *be_off, const int32_t *be_miss,
+                            const int32_t *vg_info, const int32_t *ve_off,
+                            const int32_t *ve_miss, const int32_t *c_steps,
+                            int32_t nsc, const int32_t *v_steps, int32_t nsv,
+                            int32_t max_cdeg,
+                            int32_t max_vdeg, int32_t iters, float alpha,
+                            int32_t msg_fmt,
                             const float *msg_scale, float *tot_ws,
                             void *msg_ws, uint8_t *bits, uint8_t *conv,
                             int32_t *its, int32_t *bad, void *stream);

@@ -143,3 +143,79 @@ def test_qc_layout_reproduces_csr_orders(n, rate):
                     order += (b[sel] * Z + jj[sel]).tolist()
             want = [slot_of[(int(echk[e]), v)] for e in vedge[vptr[v]:vptr[v + 1]]]
             assert order == want, (t, p)
+
+
+@pytest.mark.parametrize("n", [16200, 64800])
+@pytest.mark.parametrize("rate", codes.RATES)
+def test_kernel_tables_reproduce_csr_orders(n, rate):
+    """The packed kernel tables (kernel_tables) traversed the way the HIP
+    kernel traverses them -- step by step, group by group: message slot
+    b*384 + p is written by exactly one (check, variable) edge, every check
+    and variable group is visited exactly once per phase, and lane p of each
+    variable group reads its slots in ascending check index (per region:
+    wrapped suffix, then prefix)."""
+    c = get_code(rate, n)
+    q = c.qc()
+    k = c.kernel_tables()
+    R = c.ROW
+    cptr, cidx, vptr, vedge = c.csr()
+    echk = _edge_check(cptr)
+
+    def steps(words, info, cases):
+        for sw in words:
+            D, G, start, gen = sw & 63, (sw >> 6) & 7, (sw >> 9) & 4095, (sw >> 21) & 1
+            assert not gen or G == 1
+            # every fast step must have a (G, D) case in the kernel's switch
+            assert gen or (G, D) in cases, (G, D)
+            for gi in range(G):
+                yield D, gen, int(info[start + gi])
+
+    slot_of = {}
+    seen = set()
+    ccases = {(1, d) for d in range(1, 17)} | {(2, d) for d in range(1, 9)}
+    vcases = ({(1, d) for d in range(1, 33)} | {(2, d) for d in range(1, 9)}
+              | {(4, d) for d in range(1, 5)})
+    for D, gen, info in steps(k["c_steps"], k["cg_info"], ccases):
+        dc, b0, gm = info & 511, (info >> 9) & 2047, (info >> 20) & 1
+        assert dc == D or (gen and D == min(dc, 63))
+        assert (gen or not gm) and (gen or dc <= 16)
+        g = int(np.searchsorted(q["cg_ptr"], b0, side="right") - 1)
+        assert g not in seen
+        seen.add(g)
+        for j in range(Z):
+            chk = int(c.check_index(g, j))
+            vs = []
+            for kk in range(dc):
+                b = b0 + kk
+                if k["be_miss"][b] == j + 1:
+                    continue
+                s, toff = k["be_off"][2 * b], k["be_off"][2 * b + 1]
+                v = toff // R * Z + (j + s) % Z
+                vs.append(v)
+                assert (chk, v) not in slot_of
+                slot_of[(chk, v)] = b * R + (j + s) % Z
+            assert sorted(vs) == c.checks[chk]
+    assert len(seen) == k["ncg"] and len(slot_of) == c.nedges
+    seen = set()
+    for D, gen, info in steps(k["v_steps"], k["vg_info"], vcases):
+        dv, n0, e0, t = info & 63, (info >> 6) & 63, (info >> 12) & 2047, (info >> 24) & 255
+        assert dv == D and t not in seen
+        seen.add(t)
+        rows = k["ve_off"][2 * e0:2 * (e0 + dv):2]
+        us = k["ve_off"][2 * e0 + 1:2 * (e0 + dv):2]
+        vm = k["ve_miss"][e0:e0 + dv]
+        assert gen or not vm.any()
+        for p in range(0, Z, 7):
+            wr = (p + us) >= Z
+            order = []
+            for lo, hi in ((0, n0), (n0, dv)):
+                idx = np.arange(lo, hi)
+                w = int(wr[lo:hi].sum())
+                assert (wr[lo:hi][:hi - lo - w] == 0).all() and wr[lo:hi][hi - lo - w:].all()
+                for kk in list(idx[hi - lo - w:]) + list(idx[:hi - lo - w]):
+                    if vm[kk] != p + 1:
+                        order.append(int(rows[kk]) + p)
+            v = t * Z + p
+            want = [slot_of[(int(echk[e]), v)] for e in vedge[vptr[v]:vptr[v + 1]]]
+            assert order == want, (t, p)
+    assert len(seen) == k["nvg"]
