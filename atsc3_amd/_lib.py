@@ -29,6 +29,11 @@ _SIGS = {
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i32,
                             c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_vp, c_vp],
+    "atsc3_minsum_qc_ctl_words": [c_i32],
+    "atsc3_minsum_qc_phased_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp,
+                                   c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32,
+                                   c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp,
+                                   c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
@@ -68,7 +73,7 @@ def lib():
 
 def call(name, *args):
     rc = getattr(lib(), name)(*args)
-    if rc != 0 and name != "atsc3_demap_ws_doubles":
+    if rc != 0 and name not in ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words"):
         raise FecError(name, rc)
     return rc
 

@@ -41,274 +41,9 @@
 //  * The syndrome of iteration `it` comes from the same tot loads as the
 //    check update; when it reads zero the speculative message writes are
 //    never consumed -- tot is untouched until the variable phase.
-#include <hip/hip_fp16.h>
-
-#include "fec_common.h"
-
-#define QC_ROW 384
+#include "ldpc_qc_common.h"
 
 namespace atsc3 {
-
-struct QcArgs {
-    const float *llr;
-    int n, ncg, nvg, nbe;
-    const int *cg_info;
-    const int2 *be_off;
-    const int *be_miss;
-    const int *vg_info;
-    const int2 *ve_off;
-    const int *ve_miss;
-    const int *c_steps, *v_steps;
-    int nsc, nsv;
-    int iters;
-    float alpha;
-    const float *msg_scale;
-    float *tot_ws;
-    void *msg_ws;
-    uint8_t *bits, *conv;
-    int *its, *bad;
-};
-
-// Message storage codecs.  FMT 0 is exact FP32.
-template <int FMT>
-struct Msg;
-template <>
-struct Msg<0> {
-    using T = float;
-    __device__ static float ld(const T *p, float) { return *p; }
-    __device__ static void st(T *p, float v, float) { *p = v; }
-};
-template <>
-struct Msg<1> {
-    using T = __half;
-    __device__ static float ld(const T *p, float) { return __half2float(*p); }
-    __device__ static void st(T *p, float v, float) {
-        v = fminf(fmaxf(v, -65504.0f), 65504.0f);
-        *p = __float2half_rn(v);
-    }
-};
-template <>
-struct Msg<2> {
-    using T = int8_t;
-    __device__ static float ld(const T *p, float step) { return (float)(*p) * step; }
-    __device__ static void st(T *p, float v, float inv_step) {
-        float q = rintf(v * inv_step);
-        q = fminf(fmaxf(q, -127.0f), 127.0f);
-        *p = (int8_t)q;
-    }
-};
-
-// INT8 step when the caller gives none: clip level = 8 * mean|llr| of the
-// block, 127 steps.  Deterministic fixed-order reduction.
-__device__ float int8_step_of_block(const float *lam, int n, float *fslots) {
-    float acc = 0.0f;
-    for (int v = threadIdx.x; v < n; v += ATSC3_WG) acc += fabsf(lam[v]);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) fslots[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    float t = 0.0f;
-#pragma unroll
-    for (int w = 0; w < ATSC3_WG / 64; ++w) t += fslots[w];
-    __syncthreads();
-    const float mean = t / (float)n;
-    return fmaxf(8.0f * mean / 127.0f, 1e-30f);
-}
-
-__device__ __forceinline__ unsigned wrap360(unsigned p) { return min(p, p - 360u); }
-
-__device__ __forceinline__ float clamp_mag(float x) { return fminf(fabsf(x), INFINITY); }
-
-// G checks (g_0..g_{G-1}, j), all of degree D: every load of the G checks
-// is issued before the first store, then each check folds and emits.
-template <int FMT, int D, int G>
-__device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__restrict__ tot,
-                                                typename Msg<FMT>::T *__restrict__ msg,
-                                                unsigned j, int start, bool first, bool update,
-                                                float step, float inv_step) {
-    using M = Msg<FMT>;
-    float tv[G][D], x[G][D];
-    int b0[G];
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-        b0[gi] = (a.cg_info[start + gi] >> 9) & 2047;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int2 e = a.be_off[b0[gi] + k];
-            const unsigned p = wrap360(j + (unsigned)e.x);
-            tv[gi][k] = (tot + e.y)[p];
-            const float mo =
-                first ? 0.0f : M::ld((msg + (size_t)(b0[gi] + k) * QC_ROW) + p, step);
-            x[gi][k] = tv[gi][k] - mo;
-        }
-    }
-    unsigned syn_total = 0u;
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-        unsigned syn = 0u, par = 0u;
-        float m1 = INFINITY, m2 = INFINITY;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            syn ^= (unsigned)(tv[gi][k] < 0.0f);
-            par ^= (unsigned)(x[gi][k] < 0.0f);
-            const float mag = clamp_mag(x[gi][k]);
-            const float n2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
-            m1 = fminf(m1, mag);
-            m2 = n2;
-        }
-        syn_total += syn;
-        if (update) {
-            const float a1 = m1 * a.alpha, a2 = m2 * a.alpha;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const unsigned p = wrap360(j + (unsigned)a.be_off[b0[gi] + k].x);
-                float val = clamp_mag(x[gi][k]) == m1 ? a2 : a1;
-                if (par ^ (unsigned)(x[gi][k] < 0.0f)) val = -val;
-                M::st((msg + (size_t)(b0[gi] + k) * QC_ROW) + p, val, inv_step);
-            }
-        }
-    }
-    return syn_total;
-}
-
-// Any degree, possibly with the missing corner entry: two passes, the
-// second recomputes x from the same operands (identical FP32 value).
-template <int FMT>
-__device__ unsigned check_generic(const QcArgs &a, const float *__restrict__ tot,
-                                  typename Msg<FMT>::T *__restrict__ msg, unsigned j, int b0,
-                                  int dc, bool first, bool update, float step,
-                                  float inv_step) {
-    using M = Msg<FMT>;
-    unsigned syn = 0u, par = 0u;
-    float m1 = INFINITY, m2 = INFINITY;
-    for (int k = 0; k < dc; ++k) {
-        if (a.be_miss[b0 + k] == (int)j + 1) continue;
-        const int2 e = a.be_off[b0 + k];
-        const unsigned p = wrap360(j + (unsigned)e.x);
-        const float tv = (tot + e.y)[p];
-        const float mo = first ? 0.0f : M::ld((msg + (size_t)(b0 + k) * QC_ROW) + p, step);
-        const float xv = tv - mo;
-        syn ^= (unsigned)(tv < 0.0f);
-        par ^= (unsigned)(xv < 0.0f);
-        const float mag = clamp_mag(xv);
-        const float n2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
-        m1 = fminf(m1, mag);
-        m2 = n2;
-    }
-    if (update) {
-        const float a1 = m1 * a.alpha, a2 = m2 * a.alpha;
-        for (int k = 0; k < dc; ++k) {
-            if (a.be_miss[b0 + k] == (int)j + 1) continue;
-            const int2 e = a.be_off[b0 + k];
-            const unsigned p = wrap360(j + (unsigned)e.x);
-            const float tv = (tot + e.y)[p];
-            const float mo =
-                first ? 0.0f : M::ld((msg + (size_t)(b0 + k) * QC_ROW) + p, step);
-            const float xv = tv - mo;
-            float val = clamp_mag(xv) == m1 ? a2 : a1;
-            if (par ^ (unsigned)(xv < 0.0f)) val = -val;
-            M::st((msg + (size_t)(b0 + k) * QC_ROW) + p, val, inv_step);
-        }
-    }
-    return syn;
-}
-
-// G variable groups of degree D at position p: all loads first, then the
-// ordered sums.
-template <int FMT, int D, int G, bool TA>
-__device__ __forceinline__ void var_multi(const QcArgs &a, const float *__restrict__ lam,
-                                          float *__restrict__ tot,
-                                          const typename Msg<FMT>::T *__restrict__ msg,
-                                          unsigned p, int start, float step) {
-    using M = Msg<FMT>;
-    float val[G][D], lv[G];
-    int w0[G], w1[G], n0[G], tt[G];
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-        const int info = a.vg_info[start + gi];
-        const int e0 = (info >> 12) & 2047;
-        n0[gi] = (info >> 6) & 63;
-        tt[gi] = (info >> 24) & 255;
-        lv[gi] = lam[tt[gi] * ATSC3_Z + p];
-        w0[gi] = 0;
-        w1[gi] = 0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int2 e = a.ve_off[e0 + k];
-            val[gi][k] = M::ld((msg + e.x) + p, step);
-            const int wr = (int)(p + (unsigned)e.y >= 360u);
-            if (TA && k < n0[gi])
-                w0[gi] += wr;
-            else
-                w1[gi] += wr;
-        }
-    }
-#pragma unroll
-    for (int gi = 0; gi < G; ++gi) {
-        float s = 0.0f;
-        if (TA) {
-            const int t0 = n0[gi] - w0[gi];
-#pragma unroll
-            for (int k = 0; k < D; ++k)
-                if (k < n0[gi] && k >= t0) s += val[gi][k];
-#pragma unroll
-            for (int k = 0; k < D; ++k)
-                if (k < n0[gi] && k < t0) s += val[gi][k];
-        }
-        const int t1 = D - w1[gi];
-#pragma unroll
-        for (int k = 0; k < D; ++k)
-            if ((!TA || k >= n0[gi]) && k >= t1) s += val[gi][k];
-#pragma unroll
-        for (int k = 0; k < D; ++k)
-            if ((!TA || k >= n0[gi]) && k < t1) s += val[gi][k];
-        tot[tt[gi] * QC_ROW + p] = lv[gi] + s;
-    }
-}
-
-// Variable group holding the missing corner entry (or degree > 32): runtime
-// degree up to 32, per-entry validity.
-template <int FMT, int VMAX>
-__device__ void var_generic(const QcArgs &a, const float *__restrict__ lam_t,
-                            float *__restrict__ tot_t,
-                            const typename Msg<FMT>::T *__restrict__ msg, unsigned p, int e0,
-                            int dv, int n0, float step) {
-    using M = Msg<FMT>;
-    float val[VMAX];
-    unsigned wrapped = 0u, valid = 0u;
-#pragma unroll
-    for (int k = 0; k < VMAX; ++k) {
-        val[k] = 0.0f;
-        if (k < dv) {
-            const int2 e = a.ve_off[e0 + k];
-            val[k] = M::ld((msg + e.x) + p, step);
-            wrapped |= (unsigned)(p + (unsigned)e.y >= 360u) << k;
-            valid |= (unsigned)(a.ve_miss[e0 + k] != (int)p + 1) << k;
-        }
-    }
-    float s = 0.0f;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-#pragma unroll
-        for (int w = 1; w >= 0; --w) {
-#pragma unroll
-            for (int k = 0; k < VMAX; ++k) {
-                const bool in_r = r == 0 ? k < n0 : k >= n0;
-                if (((valid >> k) & 1u) && in_r && ((wrapped >> k) & 1u) == (unsigned)w)
-                    s += val[k];
-            }
-        }
-    }
-    tot_t[p] = lam_t[p] + s;
-}
-
-#define QC_CASES4(X) X(1) X(2) X(3) X(4)
-#define QC_CASES8(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
-#define QC_CASES16(X) \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
-#define QC_CASES(X) \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
-    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) \
-    X(31) X(32)
 
 template <int FMT, bool TA, int CMAX, int VMAX>
 __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
@@ -339,7 +74,7 @@ __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
         if (tid < ATSC3_Z) {
             const unsigned j = tid;
             for (int si = 0; si < a.nsc; ++si) {
-                const int sw = a.c_steps[si];
+                const int sw = __builtin_amdgcn_readfirstlane(a.c_steps[si]);
                 const int D = sw & 63;
                 const int G = (sw >> 6) & 7;
                 const int start = (sw >> 9) & 4095;
@@ -387,7 +122,7 @@ __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
         if (tid < ATSC3_Z) {
             const unsigned p = tid;
             for (int si = 0; si < a.nsv; ++si) {
-                const int sw = a.v_steps[si];
+                const int sw = __builtin_amdgcn_readfirstlane(a.v_steps[si]);
                 const int D = sw & 63;
                 const int G = (sw >> 6) & 7;
                 const int start = (sw >> 9) & 4095;
@@ -478,19 +213,21 @@ extern "C" int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n
     if (max_vdeg < 1 || max_cdeg < 1) return ATSC3_E_ARG;
     if (max_vdeg > 32) return ATSC3_E_DEGREE;
     if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
+    using CI = const QC_CONST int *;
+    
     atsc3::QcArgs a{llr,
                     n,
                     ncg,
                     nvg,
                     nbe,
-                    cg_info,
-                    reinterpret_cast<const int2 *>(be_off),
-                    be_miss,
-                    vg_info,
-                    reinterpret_cast<const int2 *>(ve_off),
-                    ve_miss,
-                    c_steps,
-                    v_steps,
+                    (CI)cg_info,
+                    (CI)be_off,
+                    (CI)be_miss,
+                    (CI)vg_info,
+                    (CI)ve_off,
+                    (CI)ve_miss,
+                    (CI)c_steps,
+                    (CI)v_steps,
                     nsc,
                     nsv,
                     iters,

@@ -28,6 +28,9 @@ from . import _lib
 from .codes import LdpcCode, get_code, RATES
 
 MSG_FORMATS = {"float32": 0, "fp32": 0, "float16": 1, "fp16": 1, "int8": 2}
+# batches at least this large run on the per-iteration phase kernels; smaller
+# ones on the fused one-workgroup-per-codeword kernel (lower latency)
+PHASE_MIN_BLOCKS = 256
 MSG_BYTES = {0: 4, 1: 2, 2: 1}
 _DEV_TABLES = {}
 _RESOLVED = {}
@@ -128,8 +131,10 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     """Decode device-resident LLRs (nblk, n) float32; no host sync.
 
     Returns device tensors (bits u8 (nblk,n), conv u8, its i32, bad i32).
-    engine: "qc" (360-lifted kernel), "csr" (generic kernel; FP32 only) or
-    "auto" (qc when the code has a QC form).
+    engine: "qc" (fused 360-lifted kernel, one workgroup per codeword),
+    "phase" (the same decode as per-iteration phase kernels, for large
+    batches), "csr" (generic kernel; FP32 only) or "auto" (phase or qc by
+    batch size when the code has a QC form, else csr).
     """
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_dev needs a CUDA (HIP) tensor")
@@ -148,7 +153,9 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     bad = torch.empty(nblk, dtype=torch.int32, device=dev)
     if nblk == 0:
         return bits, conv, its, bad
-    use_qc = engine == "qc" or (engine == "auto" and tabs.qc is not None)
+    if engine == "auto":
+        engine = "csr" if tabs.qc is None else ("phase" if nblk >= PHASE_MIN_BLOCKS else "qc")
+    use_qc = engine in ("qc", "phase")
     if use_qc and tabs.qc is None:
         raise ValueError("this code has no QC form with variable degree <= 32; "
                          "use engine='csr'")
@@ -164,14 +171,20 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
             if msg_scale.numel() != nblk:
                 raise ValueError("msg_scale needs one value per block")
             scale_ptr = msg_scale.data_ptr()
-        _lib.call("atsc3_minsum_qc_dev", llr.data_ptr(), nblk, n, q["ncg"], q["nvg"],
-                  q["nbe"], q["type_a"], q["cg_info"].data_ptr(), q["be_off"].data_ptr(),
-                  q["be_miss"].data_ptr(), q["vg_info"].data_ptr(), q["ve_off"].data_ptr(),
-                  q["ve_miss"].data_ptr(), q["c_steps"].data_ptr(), len(q["c_steps"]),
-                  q["v_steps"].data_ptr(), len(q["v_steps"]), q["max_cdeg"], q["max_vdeg"],
-                  int(iters),
-                  float(np.float32(alpha)), fmt, scale_ptr, tot.data_ptr(), msg.data_ptr(),
-                  bits.data_ptr(), conv.data_ptr(), its.data_ptr(), bad.data_ptr(), stream)
+        tables = (q["ncg"], q["nvg"], q["nbe"], q["type_a"], q["cg_info"].data_ptr(),
+                  q["be_off"].data_ptr(), q["be_miss"].data_ptr(), q["vg_info"].data_ptr(),
+                  q["ve_off"].data_ptr(), q["ve_miss"].data_ptr(), q["c_steps"].data_ptr(),
+                  len(q["c_steps"]), q["v_steps"].data_ptr(), len(q["v_steps"]),
+                  q["max_cdeg"], q["max_vdeg"], int(iters), float(np.float32(alpha)), fmt,
+                  scale_ptr, tot.data_ptr(), msg.data_ptr())
+        outs = (bits.data_ptr(), conv.data_ptr(), its.data_ptr(), bad.data_ptr(), stream)
+        if engine == "phase":
+            ctl = torch.empty(_lib.lib().atsc3_minsum_qc_ctl_words(nblk), dtype=torch.int32,
+                              device=dev)
+            _lib.call("atsc3_minsum_qc_phased_dev", llr.data_ptr(), nblk, n, *tables,
+                      ctl.data_ptr(), *outs)
+        else:
+            _lib.call("atsc3_minsum_qc_dev", llr.data_ptr(), nblk, n, *tables, *outs)
     else:
         if fmt != 0:
             raise ValueError("the CSR engine stores FP32 messages only")

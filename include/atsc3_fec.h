@@ -95,6 +95,32 @@ int32_t atsc3_minsum_qc_dev(const float *llr, int32_t nblk, int32_t n,
                             int32_t *its, int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Same decode, same results, as per-iteration phase kernels: the throughput
+ * engine for large batches (hundreds of blocks and more).  Enqueues about
+ * 3 * (iters + 1) kernels on `stream`; retired blocks drop out of the work
+ * lists, and once all have retired the remaining launches exit at once.
+ * Extra workspace: ctl_ws of atsc3_minsum_qc_ctl_words(nblk) int32 words.
+ * Other arguments as atsc3_minsum_qc_dev.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_minsum_qc_ctl_words(int32_t nblk);
+int32_t atsc3_minsum_qc_phased_dev(const float *llr, int32_t nblk, int32_t n,
+                                   int32_t ncg, int32_t nvg, int32_t nbe,
+                                   int32_t type_a, const int32_t *cg_info,
+                                   const int32_t *be_off,
+                                   const int32_t *be_miss,
+                                   const int32_t *vg_info,
+                                   const int32_t *ve_off,
+                                   const int32_t *ve_miss,
+                                   const int32_t *c_steps, int32_t nsc,
+                                   const int32_t *v_steps, int32_t nsv,
+                                   int32_t max_cdeg, int32_t max_vdeg,
+                                   int32_t iters, float alpha, int32_t msg_fmt,
+                                   const float *msg_scale, float *tot_ws,
+                                   void *msg_ws, int32_t *ctl_ws, uint8_t *bits,
+                                   uint8_t *conv, int32_t *its, int32_t *bad,
+                                   void *stream);
+
+/* ------------------------------------------------------------------------
  * Max-log NUC demapper with fused bit de-interleave.
  * Replaces the upstream demap calls lab/m6_bicm.py:213-225 (demap_llr),
  * lab/m9_fast.py:756 (FrameDecoder.demap_batch), lab/m44_ldm.py:2454

@@ -25,7 +25,7 @@ def _gpu_decode(llr, code, iters, alpha, device, engine="qc", msg="float32"):
 
 
 @pytest.mark.parametrize("name", F.fixture_names())
-@pytest.mark.parametrize("engine", ["qc", "csr"])
+@pytest.mark.parametrize("engine", ["qc", "phase", "csr"])
 def test_fixture_parity(device, name, engine):
     code, llr, iters, alpha, want = F.load(name)
     print(f"[gpu] fixture {name} engine {engine} nblk {len(llr)}", flush=True)
@@ -55,9 +55,10 @@ def test_live_parity_vs_oracle(device, rate, n, snr, alpha, iters, nblk):
     llr = synth.llr_bpsk_like(cw, snr, 901 + nblk)
     print(f"[gpu] live {n} {rate} snr {snr} alpha {alpha} iters {iters} nblk {nblk}", flush=True)
     want = OL.minsum_f32(llr, code, iters, alpha)
-    got = _gpu_decode(llr, code, iters, alpha, device)
-    for x, y, nm in zip(got, want, ("bits", "conv", "its", "bad")):
-        assert np.array_equal(x, y), nm
+    for engine in ("qc", "phase"):
+        got = _gpu_decode(llr, code, iters, alpha, device, engine)
+        for x, y, nm in zip(got, want, ("bits", "conv", "its", "bad")):
+            assert np.array_equal(x, y), (engine, nm)
 
 
 def test_empty_batch(device):
@@ -71,22 +72,26 @@ def test_empty_batch(device):
 
 
 def test_determinism_and_large_batch_invariants(device):
-    """Full-size property check: 64800 r10/15, 160 blocks -- repeated runs
+    """Full-size property check: 64800 r10/15, 320 blocks over five SNRs
+    (mixed convergence) -- repeated phased runs and the fused kernel
     bit-identical; conv <=> zero syndrome of the returned bits; bad equals
     the syndrome weight of non-converged blocks; a 12-block sample equals the
     oracle."""
     code = get_code("10/15", 64800)
-    cw = synth.codewords("10/15", 64800, 160, 4242)
-    llr = synth.llr_bpsk_like(cw, 2.25, 4243)
-    a = _gpu_decode(llr, code, 50, 0.75, device)
-    b = _gpu_decode(llr, code, 50, 0.75, device)
-    for x, y in zip(a, b):
-        assert np.array_equal(x, y)
+    cw = synth.codewords("10/15", 64800, 64, 4242)
+    llr = np.concatenate([synth.llr_bpsk_like(cw, snr, 4243 + k)
+                          for k, snr in enumerate((2.25, 2.5, 2.8, 3.2, 4.0))])
+    a = _gpu_decode(llr, code, 50, 0.75, device, "phase")
+    b = _gpu_decode(llr, code, 50, 0.75, device, "phase")
+    c = _gpu_decode(llr, code, 50, 0.75, device, "qc")
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
     bits, conv, its, bad = a
     syn = code.syndrome_count(bits)
     assert np.array_equal(syn == 0, conv)
     assert np.array_equal(bad[~conv], syn[~conv])
-    idx = np.linspace(0, 159, 12).astype(int)
+    assert 0 < conv.sum() < len(conv)
+    idx = np.linspace(0, len(llr) - 1, 12).astype(int)
     want = OL.minsum_f32(llr[idx], code, 50, 0.75)
     for x, y in zip((bits[idx], conv[idx], its[idx], bad[idx]), want):
         assert np.array_equal(x, y)
@@ -118,6 +123,19 @@ def test_ladder_matches_oracle(device):
     want = OL.ladder(llr, code)
     for x, y in zip(got, want):
         assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+@pytest.mark.parametrize("msg", ["float16", "int8"])
+def test_low_precision_engines_agree(device, msg):
+    """The fused and phased engines run the same arithmetic in every message
+    format."""
+    code = get_code("4/15", 64800)
+    cw = synth.codewords("4/15", 64800, 8, 91)
+    llr = synth.llr_bpsk_like(cw, -1.6, 92)
+    a = _gpu_decode(llr, code, 30, 0.75, device, "qc", msg)
+    b = _gpu_decode(llr, code, 30, 0.75, device, "phase", msg)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("msg", ["float16", "int8"])
