@@ -73,6 +73,10 @@ __global__ __launch_bounds__(ATSC3_WG) void phase_check(QcArgs a, PhaseCtl c, in
     if (nact == 0) return;
     const int *act = active_list(c, sel);
     const unsigned tid = threadIdx.x;
+    // Contiguous block-major chunk per workgroup.  (An XCD-affine round-robin
+    // order raises the L2 hit rate of the tot gathers from ~44% to ~62% but
+    // measured slower: per-item counter flushes and less sequential message
+    // streaming cost more than the saved HBM reads.)
     const long long items = (long long)nact * a.nsc;
     const long long chunk = (items + gridDim.x - 1) / gridDim.x;
     const long long i0 = (long long)blockIdx.x * chunk;

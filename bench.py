@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--fer-blocks", type=int, default=30000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP16 / INT8 message-storage comparison")
+    ap.add_argument("--form-fer-windows", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (reference "
@@ -176,10 +179,33 @@ def main():
     value = nblk / (ms_per_step / 1e3)
     achieved = model_bytes(its, N, E, s_msg) / (decode_ms / 1e3) / 1e9
     key = f"live_frame_{args.msg}_{args.frames}f_es{op:g}"
-    traffic = load_traffic(key)
+    traffic = load_traffic(key)          # measured HBM bytes per decode call
     log(f"[bench] Es/N0 {op:g} dB: {ms_per_step:.3f} ms/step (demap {demap_ms:.3f} ms, "
         f"decode {decode_ms:.3f} ms), mean its {its.mean():.2f}, "
         f"conv {conv.mean():.4f}, errors {step_errors}")
+
+    # -- message-storage forms at the same operating point ----------------------
+    forms = {}
+    if not args.no_forms:
+        for fmt in ("float32", "float16", "int8"):
+            for _ in range(1):
+                run_window(z, w, n0, fmt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fo = run_window(z, w, n0, fmt)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            errs, blocks = errors(idx, fo), nblk
+            for k in range(1, args.form_fer_windows):
+                z2, w2, idx2, _ = stream.batch(nblk, op, start=(7 + k) * nblk)
+                errs += errors(idx2, run_window(z2, w2, n0, fmt))
+                blocks += nblk
+            forms[fmt] = {"blocks_per_s": round(nblk / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                          "mean_its": round(float(fo[2].float().mean().item()), 3),
+                          "fer_blocks": blocks, "fer_errors": errs}
+            log(f"[bench] form {fmt}: {nblk / (ms / 1e3):.0f} blocks/s, "
+                f"{errs}/{blocks} block errors")
 
     # -- CPU baseline (same blocks, bounded sample) ----------------------------
     cpu = None
@@ -229,7 +255,8 @@ def main():
                    "blocks_per_step": nblk, "mean_its": round(float(its.mean()), 3),
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
-                   "fer_sweep": {f"{k:g}": v for k, v in fer.items()}},
+                   "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
+                   "forms": forms},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
