@@ -34,6 +34,9 @@ _SIGS = {
                                    c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32,
                                    c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp,
                                    c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_ldm_ws_doubles": [c_i32, c_i32],
+    "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
+                             c_f32, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
@@ -71,9 +74,12 @@ def lib():
         return _lib
 
 
+SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles")
+
+
 def call(name, *args):
     rc = getattr(lib(), name)(*args)
-    if rc != 0 and name not in ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words"):
+    if rc != 0 and name not in SIZE_QUERIES:
         raise FecError(name, rc)
     return rc
 

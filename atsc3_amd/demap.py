@@ -83,8 +83,8 @@ def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=Non
 
 @functools.lru_cache(maxsize=None)
 def _points_dev(key, device):
-    mod, rate = key
-    pts = bicm.points(mod, rate)
+    mod, rate, scale = key
+    pts = bicm.points(mod, rate) * scale
     nb = _check_quadrant_rule(pts)
     t = torch.from_numpy(np.stack([pts.real, pts.imag], 1).astype(np.float32)).to(device)
     return t.contiguous(), nb
@@ -96,12 +96,20 @@ def _identity_dev(N, device):
 
 
 class Demapper:
-    """Fused demap + de-interleave for one (N, modulation, rate)."""
+    """Fused demap + de-interleave for one (N, modulation, rate).
 
-    def __init__(self, n, mod, rate, device="cuda"):
+    point_scale a > 0 demaps against the alphabet a*p: with sigma2 scaled by
+    a^2 this is the same LLR as demapping y / a against p (how the LDM core
+    layer is demapped with the enhanced layer folded into the noise,
+    lab/m15_ldm_llr.py:84-85), without a pass over the cells.
+    """
+
+    def __init__(self, n, mod, rate, device="cuda", point_scale=1.0):
         self.n, self.mod, self.rate = int(n), mod, rate
         self.device = torch.device(device)
-        self.points, self.nb = _points_dev((mod, rate), self.device)
+        if not point_scale > 0:
+            raise ValueError("point_scale must be positive")
+        self.points, self.nb = _points_dev((mod, rate, float(point_scale)), self.device)
         self.ncell = bicm.cells_per_block(self.n, mod)
         lq = bicm.bicm_map(self.n, mod, rate)
         self.lam_of_q = torch.from_numpy(lq.astype(np.int32)).to(self.device)
@@ -120,6 +128,7 @@ def demap_batch(cells2d, pts, sigma2=None, csi=None, device="cuda"):
     dev = torch.device(device)
     p = torch.from_numpy(np.stack([pts.real, pts.imag], 1).astype(np.float32)).to(dev)
     z = torch.from_numpy(np.ascontiguousarray(cells2d, np.complex64)).to(dev)
+    z = z.reshape(-1, z.shape[-1])
     ident = _identity_dev(z.shape[1] * nb, dev)
     llr, s2 = demap_q_dev(z, p, nb, ident, csi, sigma2)
     return llr.cpu().numpy(), s2.cpu().numpy()

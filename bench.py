@@ -21,6 +21,11 @@ Printed: ONE JSON line on stdout (progress on stderr), including
                duration (HIP events on the launch stream), vs 8 TB/s
   cpu_baseline demap + decode on the host cores in the upstream's
                multi-process topology on a bounded sample of the same blocks
+  config.ldm   the two-layer LDM frame (1,216,939 cells per 242 ms frame,
+               core QPSK 4/15 + enhanced 64-NUC 10/15 at -5 dB injection,
+               AWGN): core demap + decode, cancellation, enhanced demap +
+               decode of 50 frames per step, as a real-time factor;
+               ``--workload ldm`` makes it the headline line instead
 """
 from __future__ import annotations
 
@@ -42,6 +47,9 @@ FRAME_CELLS = 207_900
 FRAME_SEC = 0.242
 RATE, N, MOD = "10/15", 64800, "256QAM"
 SWEEP = (17.0, 18.0, 19.0, 20.0, 21.0, 22.0, 23.0)
+LDM_FRAME_CELLS = 1_216_939
+LDM_ESN0 = 18.0
+LDM_SWEEP = (18.0, 19.0, 20.0, 21.0)
 
 
 def log(*a):
@@ -66,6 +74,69 @@ def load_traffic(workload_key):
         return None
 
 
+def ldm_bench(args, dev, msg_formats=("float32", "float16")):
+    """Two-layer LDM chain over ``args.ldm_frames`` frames per step.
+
+    Core: QPSK 4/15 (32400 cells per block), enhanced: 64-NUC 10/15 (10800
+    cells per block), three enhanced blocks per core block; total power 1,
+    injection -5 dB, complex AWGN.  One step = the whole chain on one stream.
+    """
+    import torch
+    from atsc3_amd import ldpc
+    from atsc3_amd.ldm import LdmPipeline, LdmStream
+    from atsc3_amd import synth
+
+    res = {}
+    for fmt in msg_formats:
+        pipe = LdmPipeline(device=dev, msg_dtype=fmt)
+        if "stream" not in res:
+            res["stream"] = LdmStream(pipe, 64, args.seed + 17)
+        stream = res["stream"]
+        ncore = args.ldm_frames * LDM_FRAME_CELLS // pipe.ncell_c
+        nenh = ncore * pipe.ratio
+
+        def errs(out, ic, ie):
+            ec = int((out["core"][0] != stream.cw_c_dev.index_select(0, ic)).any(1).sum().item())
+            ee = int((out["enh"][0] != stream.cw_e_dev.index_select(0, ie)).any(1).sum().item())
+            return ec, ee
+
+        sweep = {}
+        for es in LDM_SWEEP:
+            y, ic, ie = stream.batch(ncore, synth.n0_of(es), start=int(es) * 7919)
+            out = pipe.run(y, synth.n0_of(es))
+            ec, ee = errs(out, ic, ie)
+            sweep[f"{es:g}"] = {"core_errors": ec, "core_blocks": ncore, "enh_errors": ee,
+                                "enh_blocks": nenh,
+                                "enh_mean_its": round(float(out["enh"][2].float().mean().item()), 2)}
+            log(f"[ldm] {fmt} Es/N0 {es:g} dB: core {ec}/{ncore}, enhanced {ee}/{nenh} "
+                f"block errors, enhanced mean its {sweep[f'{es:g}']['enh_mean_its']}")
+        n0 = synth.n0_of(args.ldm_esn0)
+        y, ic, ie = stream.batch(ncore, n0, start=10 ** 5)
+        for _ in range(args.warmup):
+            out = pipe.run(y, n0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = pipe.run(y, n0)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        ec, ee = errs(out, ic, ie)
+        rt = args.ldm_frames * FRAME_SEC / (ms / 1e3)
+        res[fmt] = {"ms_per_step": round(ms, 3), "realtime_factor": round(rt, 2),
+                    "blocks_per_s": round((ncore + nenh) / (ms / 1e3), 1),
+                    "core_blocks": ncore, "enh_blocks": nenh, "esn0_db": args.ldm_esn0,
+                    "core_errors": ec, "enh_errors": ee,
+                    "core_mean_its": round(float(out["core"][2].float().mean().item()), 2),
+                    "enh_mean_its": round(float(out["enh"][2].float().mean().item()), 2),
+                    "fer_sweep": sweep}
+        log(f"[ldm] {fmt}: {ms:.2f} ms per {args.ldm_frames} frames = {rt:.1f}x real time "
+            f"(core {ec}/{ncore}, enhanced {ee}/{nenh} block errors at {args.ldm_esn0:g} dB)")
+        del pipe, y, out
+        torch.cuda.empty_cache()
+    res.pop("stream", None)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +152,11 @@ def main():
                     help="skip the FP16 / INT8 message-storage comparison")
     ap.add_argument("--form-fer-windows", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", default="live", choices=["live", "ldm"])
+    ap.add_argument("--ldm-frames", type=int, default=50)
+    ap.add_argument("--ldm-esn0", type=float, default=LDM_ESN0)
+    ap.add_argument("--no-ldm", action="store_true",
+                    help="skip the LDM summary in the live-frame line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -103,6 +179,19 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.workload == "ldm":
+        res = ldm_bench(args, dev, (args.msg,))[args.msg]
+        print(json.dumps({
+            "metric": "LDM frame real-time factor (core + enhanced demap, decode, SIC)",
+            "value": res["realtime_factor"], "unit": "x real time", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": (f"LDM: {args.ldm_frames} frames x {LDM_FRAME_CELLS} cells, "
+                                    f"core QPSK 4/15 + enhanced 64-NUC 10/15 N=64800 at -5 dB "
+                                    f"injection, AWGN"), "messages": args.msg, **res},
+            "roofline": None, "cpu_baseline": None}), flush=True)
+        return
     code = get_code(RATE, N)
     E = code.nedges
     ncell = N // 8
@@ -240,6 +329,12 @@ def main():
         log(f"[bench] CPU: {ns / secs:.1f} blocks/s on {procs} cores ({which}); "
             f"1 core {2 / t_one:.2f} blocks/s; GPU/CPU bit agreement {agree:.4f}")
 
+    ldm = None
+    if not args.no_ldm:
+        del z, w, llr
+        torch.cuda.empty_cache()
+        ldm = ldm_bench(args, dev)
+
     info_gbps = value * code.K / 1e9
     rt_factor = value / (FRAME_CELLS / ncell / FRAME_SEC)
     line = {
@@ -256,7 +351,7 @@ def main():
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
                    "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
-                   "forms": forms},
+                   "forms": forms, "ldm": ldm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

@@ -146,6 +146,27 @@ int32_t atsc3_demap_dev(const float *cells, const float *csi,
                         const int32_t *lam_of_q, float *llr_out,
                         double *sigma2_out, double *ws, void *stream);
 
+/* ------------------------------------------------------------------------
+ * LDM successive interference cancellation.
+ * Replaces the core-layer cancellation of lab/e69_enh.py:86-121: for each
+ * block with conv[b] = 1 the symbols s are rebuilt from the decoded codeword
+ * cw (q = cw[lam_of_q], label MSB first, s = points[label]), the gain
+ * g = vdot(s, y) / vdot(s, s) is estimated in FP64 and resid = y - g s;
+ * blocks with conv[b] = 0 pass through.  Every output cell is multiplied by
+ * out_scale.
+ *   cells, resid_out [nblk*ncell] complex64; cw [nblk*n] u8; conv [nblk] u8
+ *   lam_of_q [n] int32, points [2^nbits] complex64, n = ncell * nbits
+ *   gain_out [2*nblk] f64 (re, im) or NULL
+ *   ws       f64 scratch of atsc3_ldm_ws_doubles(nblk, ncell) elements
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_ldm_ws_doubles(int32_t nblk, int32_t ncell);
+int32_t atsc3_ldm_cancel_dev(const float *cells, const uint8_t *cw,
+                             const uint8_t *conv, int32_t nblk, int32_t ncell,
+                             int32_t n, const int32_t *lam_of_q,
+                             const float *points, int32_t nbits,
+                             float out_scale, float *resid_out,
+                             double *gain_out, double *ws, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
