@@ -34,6 +34,12 @@ _SIGS = {
                                    c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32,
                                    c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp, c_vp,
                                    c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_batch_ctl_words": [c_i32, c_i32],
+    "atsc3_minsum_batch_max_grid": [c_i32],
+    "atsc3_minsum_batch_status_word": [],
+    "atsc3_minsum_batch_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
+                               c_f32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
+                               c_vp, c_vp, c_vp, c_vp],
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],
@@ -74,7 +80,9 @@ def lib():
         return _lib
 
 
-SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles")
+SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
+                "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
+                "atsc3_minsum_batch_status_word")
 
 
 def call(name, *args):

@@ -1,0 +1,207 @@
+"""Batched min-sum over a queue of codewords of mixed A/322 codes.
+
+One persistent kernel (atsc3_minsum_batch_dev) decodes every codeword of a
+frame's uneven mix -- 64800- and 16200-bit blocks at any rates -- with a
+bounded number of decode slots; converged codewords retire and free their
+slot for the next one in the device queue.  Per codeword the results are
+those of ldpc_minsum_f32 (lab/ldpc_kernel.c:65-146) in FP32 mode.
+
+It stands where the upstream splits a frame into per-code sub-batches and
+decodes each with its own call (lab/m9_fast.py:843-925 ldpc_batch over
+ThreadPoolExecutor chunks; lab/m44_ldm.py:2497-2576).
+
+``BatchPlan`` owns every device buffer of one batch shape (codes per block,
+slots, grid), so a decode enqueues kernels only and can be captured in a
+HIP graph (``capture``) and replayed per frame.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .codes import LdpcCode
+from .ldpc import MSG_BYTES, MSG_FORMATS, device_tables
+
+ROW = LdpcCode.ROW
+
+
+class _CodeTab(ctypes.Structure):
+    """struct atsc3_code_tab (include/atsc3_fec.h)."""
+    _fields_ = ([(f, ctypes.c_int32) for f in
+                 ("n", "ncg", "nvg", "nbe", "nsc", "nsv", "type_a", "max_vdeg")]
+                + [(f, ctypes.c_void_p) for f in
+                   ("cg_info", "be_off", "be_miss", "vg_info", "ve_off", "ve_miss",
+                    "c_steps", "v_steps")])
+
+
+assert ctypes.sizeof(_CodeTab) == 96
+
+
+class CodeBank:
+    """Device tables of several codes plus their descriptor array."""
+
+    def __init__(self, codes, device):
+        self.device = torch.device(device)
+        self.codes = list(codes)
+        self.tabs = [device_tables(c, self.device) for c in self.codes]
+        arr = (_CodeTab * len(self.codes))()
+        for i, t in enumerate(self.tabs):
+            q = t.qc
+            if q is None:
+                raise ValueError(f"code {i} has no QC form with variable degree <= 32")
+            if q["nvg"] > 255 or q["nbe"] >= 2048:
+                raise ValueError(f"code {i} exceeds the kernel's table limits")
+            d = arr[i]
+            d.n, d.ncg, d.nvg, d.nbe = t.n, q["ncg"], q["nvg"], q["nbe"]
+            d.nsc, d.nsv = len(q["c_steps"]), len(q["v_steps"])
+            d.type_a, d.max_vdeg = int(q["type_a"]), int(q["max_vdeg"])
+            for f in ("cg_info", "be_off", "be_miss", "vg_info", "ve_off", "ve_miss",
+                      "c_steps", "v_steps"):
+                setattr(d, f, q[f].data_ptr())
+        raw = np.frombuffer(bytes(arr), np.uint8).copy()
+        self.desc = torch.from_numpy(raw).to(self.device)
+        self.max_nvg = max(t.qc["nvg"] for t in self.tabs)
+        self.max_nbe = max(t.qc["nbe"] for t in self.tabs)
+
+    def __len__(self):
+        return len(self.codes)
+
+
+def max_grid(msg_dtype="float32"):
+    """Workgroups of the persistent kernel that fit on the current device."""
+    g = _lib.lib().atsc3_minsum_batch_max_grid(MSG_FORMATS[msg_dtype])
+    if g <= 0:
+        raise _lib.FecError("atsc3_minsum_batch_max_grid", g)
+    return g
+
+
+class BatchPlan:
+    """Buffers and launch parameters for decoding one batch composition.
+
+    block_codes: one LdpcCode per codeword, in queue order.  The LLRs of
+    codeword b occupy ``llr[off[b] : off[b] + n_b]`` of the flat ``llr``
+    buffer; its bits the same range of ``bits``.
+    """
+
+    def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75,
+                 msg_dtype="float32", slots=None, grid=None, msg_scale=None):
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        uniq, idx = [], []
+        for c in block_codes:
+            for i, u in enumerate(uniq):
+                if u is c:
+                    idx.append(i)
+                    break
+            else:
+                uniq.append(c)
+                idx.append(len(uniq) - 1)
+        self.bank = CodeBank(uniq, self.device)
+        self.nblk = len(idx)
+        self.fmt = MSG_FORMATS[msg_dtype]
+        self.msg_dtype = msg_dtype
+        self.iters, self.alpha = int(iters), float(np.float32(alpha))
+        ns = np.array([uniq[i].n for i in idx], np.int64)
+        self.n_of = ns
+        self.off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+        self.total = int(self.off[-1])
+        self.code_idx = np.array(idx, np.int32)
+        cap = max_grid(msg_dtype)
+        self.grid = cap if grid is None else int(grid)
+        if not 1 <= self.grid <= cap:
+            raise ValueError(f"grid {self.grid} outside 1..{cap} (co-resident capacity)")
+        self.slots = max(1, min(self.nblk, 1024 if slots is None else int(slots)))
+        dev = self.device
+        self.blk_code = torch.from_numpy(self.code_idx).to(dev)
+        self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
+        self.llr = torch.zeros(self.total, dtype=torch.float32, device=dev)
+        self.bits = torch.empty(self.total, dtype=torch.uint8, device=dev)
+        self.conv = torch.empty(self.nblk, dtype=torch.uint8, device=dev)
+        self.its = torch.empty(self.nblk, dtype=torch.int32, device=dev)
+        self.bad = torch.empty(self.nblk, dtype=torch.int32, device=dev)
+        self.tot = torch.empty(self.slots * self.bank.max_nvg * ROW, dtype=torch.float32,
+                               device=dev)
+        self.msg = torch.empty(self.slots * self.bank.max_nbe * ROW * MSG_BYTES[self.fmt],
+                               dtype=torch.uint8, device=dev)
+        nctl = _lib.lib().atsc3_minsum_batch_ctl_words(max(self.nblk, 0), self.slots)
+        self.ctl = torch.zeros(nctl, dtype=torch.int32, device=dev)
+        self.status_word = _lib.lib().atsc3_minsum_batch_status_word()
+        self.msg_scale = None
+        if msg_scale is not None:
+            self.msg_scale = torch.as_tensor(msg_scale, dtype=torch.float32,
+                                             device=dev).contiguous()
+            if self.msg_scale.numel() != self.nblk:
+                raise ValueError("msg_scale needs one value per codeword")
+
+    def llr_view(self, b):
+        return self.llr[int(self.off[b]):int(self.off[b + 1])]
+
+    def run(self, stream=None):
+        """Enqueue the decode of ``self.llr`` on the current stream."""
+        if self.nblk == 0:
+            return
+        st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        _lib.call("atsc3_minsum_batch_dev", self.llr.data_ptr(), self.nblk,
+                  self.blk_off.data_ptr(), self.blk_code.data_ptr(), self.bank.desc.data_ptr(),
+                  len(self.bank), self.bank.max_nvg, self.bank.max_nbe, self.iters, self.alpha,
+                  self.fmt, None if self.msg_scale is None else self.msg_scale.data_ptr(),
+                  self.slots, self.grid, self.tot.data_ptr(), self.msg.data_ptr(),
+                  self.ctl.data_ptr(), self.bits.data_ptr(), self.conv.data_ptr(),
+                  self.its.data_ptr(), self.bad.data_ptr(), st)
+
+    def check(self):
+        """Synchronize and raise if the kernel aborted (a grid-barrier wait
+        timed out, e.g. when the workgroups could not all be resident)."""
+        if self.nblk and int(self.ctl[self.status_word].item()) != 0:
+            raise RuntimeError("batched min-sum aborted: grid barrier wait timed out")
+
+    def rounds(self):
+        return int(self.ctl[9].item())
+
+    def capture(self, pre=None):
+        """A torch.cuda.CUDAGraph replaying ``pre()`` (e.g. the demaps that fill
+        ``self.llr``) followed by the decode."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            if pre is not None:
+                pre()
+            self.run()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g, stream=s):
+            if pre is not None:
+                pre()
+            self.run()
+        return g
+
+    def block_results(self, b):
+        """(bits u8 [n_b], conv, its, bad) of codeword b as device tensors."""
+        lo, hi = int(self.off[b]), int(self.off[b + 1])
+        return self.bits[lo:hi], self.conv[b], self.its[b], self.bad[b]
+
+
+def decode_mixed(llrs, codes, iters=50, alpha=0.75, msg_dtype="float32", slots=None,
+                 grid=None):
+    """Decode a list of per-codeword LLR vectors (device tensors) with their
+    codes in one batched launch.  Returns a list of device bit vectors and
+    device tensors (conv u8, its i32, bad i32) over the list."""
+    if len(llrs) != len(codes):
+        raise ValueError("one code per LLR vector")
+    if not llrs:
+        raise ValueError("empty batch")
+    dev = llrs[0].device
+    plan = BatchPlan(codes, dev, iters, alpha, msg_dtype, slots, grid)
+    for b, l in enumerate(llrs):
+        if l.numel() != plan.n_of[b]:
+            raise ValueError(f"codeword {b}: {l.numel()} LLRs for a length-{plan.n_of[b]} code")
+        plan.llr_view(b).copy_(l.reshape(-1))
+    plan.run()
+    plan.check()
+    bits = [plan.bits[int(plan.off[b]):int(plan.off[b + 1])] for b in range(plan.nblk)]
+    return bits, plan.conv, plan.its, plan.bad

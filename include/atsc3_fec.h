@@ -121,6 +121,47 @@ int32_t atsc3_minsum_qc_phased_dev(const float *llr, int32_t nblk, int32_t n,
                                    void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched min-sum over a queue of codewords of MIXED A/322 codes in one
+ * persistent launch (plus one init launch).  Same per-codeword semantics and
+ * results as ldpc_minsum_f32 (lab/ldpc_kernel.c:65-146) in FP32 mode; it
+ * replaces the upstream's per-code sub-batched calls
+ * (lab/m9_fast.py:843-925 FrameDecoder.ldpc_batch, lab/m44_ldm.py:2497-2576
+ * ldpc_batch) with one call for a frame's whole mix.
+ *
+ *   codes     device array of ncode atsc3_code_tab (tables of each code, as
+ *             built by the Python layer from the A/322 parity tables)
+ *   blk_code  [nblk] int32 index into codes (device)
+ *   blk_off   [nblk] int64 offset of block b's LLRs in llr and of its bits
+ *             in bits (device; the rows may have different lengths)
+ *   slots     codewords decoded concurrently; the rest wait in the queue
+ *   grid      workgroups of the persistent kernel, 1 ..
+ *             atsc3_minsum_batch_max_grid(msg_fmt) (all must be resident)
+ *   tot_ws    [slots * max_nvg * 384] f32
+ *   msg_ws    slots * max_nbe * 384 elements of the message format
+ *   ctl_ws    atsc3_minsum_batch_ctl_words(nblk, slots) int32; after the
+ *             call ctl_ws[atsc3_minsum_batch_status_word()] != 0 means a
+ *             workgroup timed out waiting at a grid barrier (outputs invalid)
+ * ------------------------------------------------------------------------ */
+typedef struct atsc3_code_tab {
+    int32_t n, ncg, nvg, nbe, nsc, nsv, type_a, max_vdeg;
+    const int32_t *cg_info, *be_off, *be_miss, *vg_info, *ve_off, *ve_miss,
+        *c_steps, *v_steps;
+} atsc3_code_tab;
+
+int32_t atsc3_minsum_batch_ctl_words(int32_t nblk, int32_t slots);
+int32_t atsc3_minsum_batch_max_grid(int32_t msg_fmt);
+int32_t atsc3_minsum_batch_status_word(void);
+int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
+                               const int64_t *blk_off, const int32_t *blk_code,
+                               const void *codes, int32_t ncode,
+                               int32_t max_nvg, int32_t max_nbe, int32_t iters,
+                               float alpha, int32_t msg_fmt,
+                               const float *msg_scale, int32_t slots,
+                               int32_t grid, float *tot_ws, void *msg_ws,
+                               int32_t *ctl_ws, uint8_t *bits, uint8_t *conv,
+                               int32_t *its, int32_t *bad, void *stream);
+
+/* ------------------------------------------------------------------------
  * Max-log NUC demapper with fused bit de-interleave.
  * Replaces the upstream demap calls lab/m6_bicm.py:213-225 (demap_llr),
  * lab/m9_fast.py:756 (FrameDecoder.demap_batch), lab/m44_ldm.py:2454
