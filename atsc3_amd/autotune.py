@@ -1,0 +1,106 @@
+"""Autotuner for the batched decoder's launch shape.
+
+Two knobs set how the persistent kernel maps onto the device:
+  slots  codewords decoded concurrently ("blocks per wave"): more slots give
+         every phase more parallel work and fewer rounds; fewer keep the
+         working set (tot rows + messages of the active codewords) small
+         enough to stay in the last-level cache;
+  grid   workgroups of the persistent kernel (at most the co-resident
+         capacity): more hide latency, fewer make the grid barriers cheaper.
+
+``tune`` times every candidate on the caller's real workload (a factory that
+builds the pipeline for a (slots, grid) pair; each candidate is captured in a
+HIP graph and replayed), keeps the fastest, and caches the choice in a JSON
+file keyed by device name and workload signature (``ATSC3_TUNE_CACHE``, else
+~/.cache/atsc3_amd/batch_tune.json).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import torch
+
+from .batch import max_grid
+
+
+def cache_path():
+    return os.environ.get("ATSC3_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "atsc3_amd",
+                                       "batch_tune.json"))
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d):
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def candidates(nblk, msg_dtype):
+    cap = max_grid(msg_dtype)
+    grids = sorted({g for g in (cap, cap // 2, cap // 4, 128) if 1 <= g <= cap}, reverse=True)
+    slots = sorted({max(1, s) for s in (nblk, (nblk + 1) // 2, (nblk + 3) // 4)}, reverse=True)
+    return [(s, g) for s in slots for g in grids]
+
+
+def time_graph(g, reps):
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g.replay()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        g.replay()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True, log=None):
+    """build(slots, grid) -> object with .capture() and .plan; prepare(obj)
+    loads the tuning input.  Returns (slots, grid, table of ms per candidate)."""
+    dev = torch.cuda.get_device_name()
+    key = f"{dev}|{signature}|{msg_dtype}"
+    cache = _load() if use_cache else {}
+    if key in cache:
+        c = cache[key]
+        return c["slots"], c["grid"], c.get("table", {})
+    table = {}
+    best = None
+    for slots, grid in candidates(nblk, msg_dtype):
+        obj = build(slots, grid)
+        if prepare is not None:
+            prepare(obj)
+        g = obj.capture()
+        ms = time_graph(g, reps)
+        obj.plan.check()
+        table[f"{slots}x{grid}"] = round(ms, 4)
+        if log:
+            log(f"[tune] {signature} {msg_dtype}: slots {slots} grid {grid}: {ms:.3f} ms")
+        if best is None or ms < best[0]:
+            best = (ms, slots, grid)
+        del g, obj
+        torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    _, slots, grid = best
+    if use_cache:
+        cache = _load()
+        cache[key] = {"slots": slots, "grid": grid, "table": table,
+                      "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+        _store(cache)
+    return slots, grid, table

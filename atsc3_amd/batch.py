@@ -205,3 +205,51 @@ def decode_mixed(llrs, codes, iters=50, alpha=0.75, msg_dtype="float32", slots=N
     plan.check()
     bits = [plan.bits[int(plan.off[b]):int(plan.off[b + 1])] for b in range(plan.nblk)]
     return bits, plan.conv, plan.its, plan.bad
+
+
+_SINGLE = {}
+
+
+def decode_batch_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", msg_scale=None,
+                     slots=None, grid=None):
+    """The batched engine on a (nblk, n) batch of ONE code (device tensors in
+    and out, no host sync): same results as ldpc.decode_dev."""
+    dev = llr.device
+    nblk, n = llr.shape
+    key = (id(code), nblk, str(dev))
+    hit = _SINGLE.get(key)
+    if hit is None or hit[0] is not code:
+        bank = CodeBank([code], dev)
+        blk_code = torch.zeros(nblk, dtype=torch.int32, device=dev)
+        blk_off = torch.arange(nblk, dtype=torch.int64, device=dev) * n
+        hit = (code, bank, blk_code, blk_off)
+        if len(_SINGLE) > 64:
+            _SINGLE.clear()
+        _SINGLE[key] = hit
+    _, bank, blk_code, blk_off = hit
+    fmt = MSG_FORMATS[msg_dtype]
+    bits = torch.empty((nblk, n), dtype=torch.uint8, device=dev)
+    conv = torch.empty(nblk, dtype=torch.uint8, device=dev)
+    its = torch.empty(nblk, dtype=torch.int32, device=dev)
+    bad = torch.empty(nblk, dtype=torch.int32, device=dev)
+    if nblk == 0:
+        return bits, conv, its, bad
+    cap = max_grid(msg_dtype)
+    grid = cap if grid is None else min(int(grid), cap)
+    slots = nblk if slots is None else max(1, min(int(slots), nblk))
+    tot = torch.empty(slots * bank.max_nvg * ROW, dtype=torch.float32, device=dev)
+    msg = torch.empty(slots * bank.max_nbe * ROW * MSG_BYTES[fmt], dtype=torch.uint8,
+                      device=dev)
+    ctl = torch.empty(_lib.lib().atsc3_minsum_batch_ctl_words(nblk, slots), dtype=torch.int32,
+                      device=dev)
+    sp = None
+    if msg_scale is not None:
+        msg_scale = torch.as_tensor(msg_scale, dtype=torch.float32, device=dev).contiguous()
+        sp = msg_scale.data_ptr()
+    llr = llr.contiguous()
+    _lib.call("atsc3_minsum_batch_dev", llr.data_ptr(), nblk, blk_off.data_ptr(),
+              blk_code.data_ptr(), bank.desc.data_ptr(), 1, bank.max_nvg, bank.max_nbe,
+              int(iters), float(np.float32(alpha)), fmt, sp, slots, grid, tot.data_ptr(),
+              msg.data_ptr(), ctl.data_ptr(), bits.data_ptr(), conv.data_ptr(), its.data_ptr(),
+              bad.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
+    return bits, conv, its, bad

@@ -252,25 +252,42 @@ __device__ void compact(const BatchArgs &b) {
     }
 }
 
+// Wait until this wave's vector memory operations (stores and atomics
+// included) have completed at the L2 (gfx9 counts stores in vmcnt):
+// s_waitcnt vmcnt(0), expcnt and lgkmcnt left unconstrained.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Grid barrier number `epoch` (1, 2, ...); the last workgroup to arrive runs
 // the compaction first when asked.  Returns false when the abort word is set.
+//
+// Visibility across XCDs (each has its own L2): every wave first waits for
+// its own stores to complete at its L2; after the workgroup barrier ONE wave
+// issues the agent-scope release (an L2 write-back, which covers the whole
+// workgroup's stores since they are all in that L2) before arriving, and
+// after the release of the barrier ONE wave issues the agent-scope acquire
+// (L2/L1 invalidation, the workgroup's waves share the CU's L1) before the
+// workgroup proceeds.  One write-back and one invalidation per workgroup
+// instead of one per wave: the barrier cost scales with their number.
 __device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact, int *s_flag) {
     unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wait_vmem();
     __syncthreads();
     if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned old =
             __hip_atomic_fetch_add(h + H_ARRIVE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_flag[0] = old + 1u == epoch * gridDim.x;
+        if (s_flag[0]) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     if (s_flag[0]) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (do_compact) compact(b);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        wait_vmem();
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(h + H_GEN, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     } else if (threadIdx.x == 0) {
         unsigned spins = 0;
         while (__hip_atomic_load(h + H_GEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
@@ -281,9 +298,8 @@ __device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact
             }
             __builtin_amdgcn_s_sleep(8);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (threadIdx.x == 0)
         s_flag[1] = __hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();

@@ -133,8 +133,9 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     Returns device tensors (bits u8 (nblk,n), conv u8, its i32, bad i32).
     engine: "qc" (fused 360-lifted kernel, one workgroup per codeword),
     "phase" (the same decode as per-iteration phase kernels, for large
-    batches), "csr" (generic kernel; FP32 only) or "auto" (phase or qc by
-    batch size when the code has a QC form, else csr).
+    batches), "batch" (the persistent batched engine, atsc3_amd.batch),
+    "csr" (generic kernel; FP32 only) or "auto" (the QC engines by batch size
+    and message format when the code has a QC form, else csr).
     """
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_dev needs a CUDA (HIP) tensor")
@@ -154,7 +155,17 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     if nblk == 0:
         return bits, conv, its, bad
     if engine == "auto":
-        engine = "csr" if tabs.qc is None else ("phase" if nblk >= PHASE_MIN_BLOCKS else "qc")
+        if tabs.qc is None:
+            engine = "csr"
+        elif nblk < PHASE_MIN_BLOCKS:
+            engine = "qc"
+        else:
+            engine = "phase" if fmt == 0 else "batch"
+    if engine == "batch":
+        if tabs.qc is None:
+            raise ValueError("this code has no QC form with variable degree <= 32")
+        from .batch import decode_batch_dev
+        return decode_batch_dev(llr, code, iters, alpha, msg_dtype, msg_scale)
     use_qc = engine in ("qc", "phase")
     if use_qc and tabs.qc is None:
         raise ValueError("this code has no QC form with variable degree <= 32; "

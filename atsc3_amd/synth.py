@@ -111,3 +111,35 @@ class FadingStream:
         z = x + nz / h
         w = (h.real * h.real + h.imag * h.imag).contiguous()
         return z.contiguous(), w, idx, n0
+
+
+class AwgnStream:
+    """Device-side generator of blocks over complex AWGN (unit channel):
+    ``ncw`` seeded codewords, block i carries codeword (start + i) % ncw,
+    noise from a seeded torch generator.  ``esn0_db`` is a scalar or one
+    value per block.  Returns cells (nblk, ncell) complex64, the codeword
+    indices, and N0 per block (float64 device tensor)."""
+
+    def __init__(self, rate, n, mod, ncw, seed, device):
+        import torch
+        self.rate, self.n, self.mod = rate, n, mod
+        self.device = torch.device(device)
+        self.cw = codewords(rate, n, ncw, seed)
+        x = modulate(self.cw, n, mod, rate).astype(np.complex64)
+        self.x = torch.from_numpy(x).to(self.device)
+        self.cw_dev = torch.from_numpy(self.cw).to(self.device)
+        self.ncw = ncw
+        self.gen = torch.Generator(device=self.device)
+        self.gen.manual_seed(seed + 2718)
+
+    def batch(self, nblk, esn0_db, start=0):
+        import torch
+        idx = (torch.arange(nblk, device=self.device) + start) % self.ncw
+        x = self.x.index_select(0, idx)
+        es = torch.as_tensor(esn0_db, dtype=torch.float64, device=self.device).reshape(-1)
+        n0 = torch.pow(10.0, -es / 10.0).expand(nblk).contiguous()
+        g = lambda: torch.randn(x.shape, generator=self.gen, device=self.device,
+                                dtype=torch.float32)
+        amp = torch.sqrt(n0 / 2.0).to(torch.float32)[:, None]
+        z = x + torch.complex(g() * amp, g() * amp)
+        return z.contiguous(), idx, n0

@@ -47,6 +47,7 @@ FRAME_CELLS = 207_900
 FRAME_SEC = 0.242
 RATE, N, MOD = "10/15", 64800, "256QAM"
 SWEEP = (17.0, 18.0, 19.0, 20.0, 21.0, 22.0, 23.0)
+FORM_SWEEP = (18.0, 18.25, 18.5, 18.75, 19.0, 19.25, 19.5, 19.75, 20.0)
 LDM_FRAME_CELLS = 1_216_939
 LDM_ESN0 = 18.0
 LDM_SWEEP = (18.0, 19.0, 20.0, 21.0)
@@ -72,6 +73,24 @@ def load_traffic(workload_key):
         return d.get(workload_key)
     except (OSError, ValueError):
         return None
+
+
+def fer_gap_db(ref, other, target):
+    """Es/N0 difference (dB) at which two FER curves cross ``target``
+    (log-linear interpolation between sweep points; None if a curve does not
+    cross it inside the sweep)."""
+    def cross(curve):
+        pts = [(c[0], c[1] / c[2]) for c in curve]
+        for (e0, f0), (e1, f1) in zip(pts, pts[1:]):
+            if f0 >= target > f1:
+                if f1 <= 0:
+                    return e1
+                import math
+                a = (math.log(f0) - math.log(target)) / (math.log(f0) - math.log(f1))
+                return e0 + a * (e1 - e0)
+        return None
+    a, b = cross(ref), cross(other)
+    return None if a is None or b is None else round(b - a, 3)
 
 
 def ldm_bench(args, dev, msg_formats=("float32", "float16")):
@@ -137,6 +156,86 @@ def ldm_bench(args, dev, msg_formats=("float32", "float16")):
     return res
 
 
+def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
+    """Uneven frames: 74 blocks per frame mixing 64800-bit rates 6/8/10/12/15
+    and 16200-bit rates 4/9/15 on QPSK..256-NUC, per-block AWGN Es/N0 drawn
+    around each (code, constellation)'s threshold (iterations 3..50).  Per
+    message format: autotuned (slots, grid), FER over fresh frames, and the
+    time of one HIP-graph replay (all demaps + the persistent decode)."""
+    import torch
+    from atsc3_amd import autotune
+    from atsc3_amd.mixed import MixedPipeline, MixedSource, block_esn0, frame_mix
+
+    specs = frame_mix(74, args.seed)
+    counts = {}
+    for n, r, m in specs:
+        k = f"{n} {r} {m}"
+        counts[k] = counts.get(k, 0) + 1
+    res = {"blocks_per_frame": len(specs), "mix": counts, "frames_per_launch": args.mixed_frames}
+    F = args.mixed_frames
+    nblk = len(specs) * F
+
+    def build(fmt):
+        return lambda slots, grid: MixedPipeline(specs, F, dev, 50, 0.75, fmt, slots, grid)
+
+    def prepare(p):
+        src = MixedSource(p, 8, args.seed + 5)
+        cells, s2, _ = src.frame(block_esn0(p.specs, args.seed + 9))
+        p.load(cells, s2)
+
+    ref_bits = None
+    for fmt in formats:
+        slots, grid, table = autotune.tune(f"mixed{len(specs)}x{F}", nblk, fmt, build(fmt),
+                                           prepare, use_cache=False, log=log)
+        pipe = build(fmt)(slots, grid)
+        src = MixedSource(pipe, 8, args.seed + 5)
+        errs, blocks, its, disagree = 0, 0, [], 0
+        nfr = max(1, args.mixed_fer_frames // F)
+        for w in range(nfr):
+            cells, s2, idx = src.frame(block_esn0(pipe.specs, args.seed + 1000 + w), start=3 * w)
+            pipe.load(cells, s2)
+            pipe.run()
+            errs += int(src.errors(idx).sum().item())
+            blocks += nblk
+            its.append(pipe.plan.its.clone())
+            if w == 0:
+                if fmt == "float32":
+                    ref_bits = pipe.plan.bits.clone()
+                elif ref_bits is not None:
+                    disagree = float((pipe.plan.bits != ref_bits).float().mean().item())
+        pipe.plan.check()
+        its = torch.cat(its).float()
+        prepare(pipe)
+        g = pipe.capture()
+        ms_graph = autotune.time_graph(g, max(args.steps, 5))
+        pipe.plan.check()
+        # the same work without the graph (launch overheads included)
+        pipe.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(args.steps, 5)):
+            pipe.run()
+        torch.cuda.synchronize()
+        ms_eager = (time.perf_counter() - t0) * 1e3 / max(args.steps, 5)
+        kbits = sum(__import__("atsc3_amd.codes", fromlist=["get_code"]).get_code(r, n).K
+                    for n, r, m in pipe.specs)
+        res[fmt] = {"slots": slots, "grid": grid, "tune_ms": table,
+                    "ms_per_launch_graph": round(ms_graph, 4),
+                    "ms_per_launch_eager": round(ms_eager, 4),
+                    "blocks_per_s": round(nblk / (ms_graph / 1e3), 1),
+                    "info_gbit_s": round(kbits / (ms_graph / 1e3) / 1e9, 3),
+                    "fer_blocks": blocks, "fer_errors": errs,
+                    "its_min": int(its.min().item()), "its_max": int(its.max().item()),
+                    "its_mean": round(float(its.mean().item()), 2),
+                    "bit_disagreement_vs_fp32": disagree, "rounds": pipe.plan.rounds()}
+        log(f"[mixed] {fmt}: slots {slots} grid {grid}: graph {ms_graph:.3f} ms "
+            f"(eager {ms_eager:.3f}) per {nblk} blocks = {nblk / (ms_graph / 1e3):.0f} blocks/s; "
+            f"{errs}/{blocks} block errors, its {its.min().item():.0f}..{its.max().item():.0f}")
+        del pipe, g
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +251,12 @@ def main():
                     help="skip the FP16 / INT8 message-storage comparison")
     ap.add_argument("--form-fer-windows", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="live", choices=["live", "ldm"])
+    ap.add_argument("--workload", default="live", choices=["live", "ldm", "mixed"])
+    ap.add_argument("--mixed-frames", type=int, default=1,
+                    help="frames of the uneven workload per batched launch")
+    ap.add_argument("--mixed-fer-frames", type=int, default=200)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the uneven-frame summary in the live-frame line")
     ap.add_argument("--ldm-frames", type=int, default=50)
     ap.add_argument("--ldm-esn0", type=float, default=LDM_ESN0)
     ap.add_argument("--no-ldm", action="store_true",
@@ -192,6 +296,21 @@ def main():
                                     f"injection, AWGN"), "messages": args.msg, **res},
             "roofline": None, "cpu_baseline": None}), flush=True)
         return
+    if args.workload == "mixed":
+        res = mixed_bench(args, dev, (args.msg,))
+        r = res[args.msg]
+        print(json.dumps({
+            "metric": "FEC blocks/s, uneven frames (mixed codes and constellations)",
+            "value": r["blocks_per_s"], "unit": "FEC blocks/s", "n_gpus": 1,
+            "steps": max(args.steps, 5), "warmup": 1, "ms_per_step": r["ms_per_launch_graph"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": (f"uneven frames: {res['blocks_per_frame']} blocks per frame x "
+                                    f"{args.mixed_frames} frames per launch, 64800 r6/8/10/12 + "
+                                    f"16200 r4/9 on QPSK..256-NUC, per-block AWGN"),
+                       "messages": args.msg, **res},
+            "roofline": None, "cpu_baseline": None}), flush=True)
+        return
     code = get_code(RATE, N)
     E = code.nedges
     ncell = N // 8
@@ -203,9 +322,9 @@ def main():
     dm = Demapper(N, MOD, RATE, dev)
     log(f"[bench] codewords ready in {time.time() - t0:.1f} s")
 
-    def run_window(z, w, n0, msg):
+    def run_window(z, w, n0, msg, engine="auto"):
         llr, _ = dm.demap_dev(z, csi=w, sigma2=n0)
-        return ldpc.decode_dev(llr, code, 50, 0.75, msg)
+        return ldpc.decode_dev(llr, code, 50, 0.75, msg, engine)
 
     def errors(idx, out):
         bits = out[0]
@@ -277,24 +396,40 @@ def main():
     forms = {}
     if not args.no_forms:
         for fmt in ("float32", "float16", "int8"):
-            for _ in range(1):
-                run_window(z, w, n0, fmt)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                fo = run_window(z, w, n0, fmt)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            eng_ms = {}
+            for eng in ("phase", "batch"):
+                run_window(z, w, n0, fmt, eng)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    fo = run_window(z, w, n0, fmt, eng)
+                torch.cuda.synchronize()
+                eng_ms[eng] = (time.perf_counter() - t0) * 1e3 / args.steps
+            best = min(eng_ms, key=eng_ms.get)
+            ms = eng_ms[best]
             errs, blocks = errors(idx, fo), nblk
             for k in range(1, args.form_fer_windows):
                 z2, w2, idx2, _ = stream.batch(nblk, op, start=(7 + k) * nblk)
                 errs += errors(idx2, run_window(z2, w2, n0, fmt))
                 blocks += nblk
+            # FER curve below the operating point (same blocks for every form)
+            curve = []
+            for es in FORM_SWEEP:
+                z3, w3, idx3, n03 = stream.batch(nblk, es, start=int(es * 4) * 7919)
+                o3 = run_window(z3, w3, n03, fmt)
+                curve.append([es, errors(idx3, o3), nblk,
+                              round(float(o3[2].float().mean().item()), 3)])
             forms[fmt] = {"blocks_per_s": round(nblk / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                          "engine": best, "ms_by_engine": {k: round(v, 3) for k, v in eng_ms.items()},
                           "mean_its": round(float(fo[2].float().mean().item()), 3),
-                          "fer_blocks": blocks, "fer_errors": errs}
-            log(f"[bench] form {fmt}: {nblk / (ms / 1e3):.0f} blocks/s, "
-                f"{errs}/{blocks} block errors")
+                          "fer_blocks": blocks, "fer_errors": errs, "fer_curve": curve}
+            log(f"[bench] form {fmt}: {nblk / (ms / 1e3):.0f} blocks/s ({best}; "
+                f"{', '.join(f'{k} {v:.2f} ms' for k, v in eng_ms.items())}), "
+                f"{errs}/{blocks} block errors; curve {[(c[0], c[1]) for c in curve]}")
+        if "float32" in forms:
+            for fmt in ("float16", "int8"):
+                forms[fmt]["gap_db_at_fer_1e-2"] = fer_gap_db(forms["float32"]["fer_curve"],
+                                                              forms[fmt]["fer_curve"], 1e-2)
 
     # -- CPU baseline (same blocks, bounded sample) ----------------------------
     cpu = None
@@ -329,9 +464,13 @@ def main():
         log(f"[bench] CPU: {ns / secs:.1f} blocks/s on {procs} cores ({which}); "
             f"1 core {2 / t_one:.2f} blocks/s; GPU/CPU bit agreement {agree:.4f}")
 
+    mixed = None
+    if not args.no_mixed:
+        del z, w, llr
+        torch.cuda.empty_cache()
+        mixed = mixed_bench(args, dev)
     ldm = None
     if not args.no_ldm:
-        del z, w, llr
         torch.cuda.empty_cache()
         ldm = ldm_bench(args, dev)
 
@@ -351,7 +490,7 @@ def main():
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
                    "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
-                   "forms": forms, "ldm": ldm},
+                   "forms": forms, "mixed": mixed, "ldm": ldm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
