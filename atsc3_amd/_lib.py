@@ -40,6 +40,29 @@ _SIGS = {
     "atsc3_minsum_batch_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
                                c_f32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp, c_vp, c_vp, c_vp],
+    "atsc3_unrolled_check_fwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
+                                     c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_unrolled_check_fwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
+                                     c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_unrolled_var_fwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp],
+    "atsc3_unrolled_var_fwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp],
+    "atsc3_unrolled_check_bwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                     c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp],
+    "atsc3_unrolled_check_bwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                     c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp],
+    "atsc3_unrolled_var_bwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
+                                   c_vp],
+    "atsc3_unrolled_var_bwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
+                                   c_vp],
+    "atsc3_soft_demap_fwd_f32": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32,
+                                 c_vp, c_vp],
+    "atsc3_soft_demap_fwd_f64": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32,
+                                 c_vp, c_vp],
+    "atsc3_soft_demap_bwd_grid": [c_i32, c_i32],
+    "atsc3_soft_demap_bwd_f32": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32,
+                                 c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_soft_demap_bwd_f64": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32,
+                                 c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],
@@ -82,7 +105,7 @@ def lib():
 
 SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
-                "atsc3_minsum_batch_status_word")
+                "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid")
 
 
 def call(name, *args):

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libatsc3fec.so")
 SOURCES = ["ldpc_qc.hip", "ldpc_qc_phase.hip", "ldpc_batch.hip", "ldpc_csr.hip", "demap.hip",
-           "ldm.hip"]
+           "ldm.hip", "unrolled.hip"]
 ARCH = os.environ.get("ATSC3_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
