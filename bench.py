@@ -236,6 +236,33 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     return res
 
 
+def learn_bench(args, dev, steps, eval_blocks, ebn0_points=(1.0, 1.5, 2.0, 2.5, 3.0)):
+    """Differentiation: fit per-iteration, per-check-degree normalization
+    factors of the 20-iteration unrolled min-sum (16200 r6/15, QPSK, AWGN,
+    Eb/N0 ~ U[1, 3] dB, batch 512, BCE on the output LLRs), then FER of the
+    learned table vs the plain one on fresh blocks."""
+    import torch
+    from atsc3_amd.learn import Trainer
+
+    tr = Trainer(n=16200, rate="6/15", mod="QPSK", iters=20, batch=512, ebn0=(1.0, 3.0),
+                 lr=args.learn_lr, seed=args.seed, device=dev, ncw=1024)
+    tr.step()                       # warm-up (first launches)
+    torch.cuda.synchronize()
+    secs = tr.train(steps, log=log)
+    per_pt = max(1, eval_blocks // len(ebn0_points))
+    fer = {}
+    for eb in ebn0_points:
+        f, e = tr.evaluate(eb, per_pt)
+        fer[f"{eb:g}"] = {"blocks": per_pt, **{k: int(v) for k, v in e.items()}}
+        log(f"[learn] Eb/N0 {eb:g} dB: " + ", ".join(f"{k} {v}/{per_pt}" for k, v in e.items()))
+    a = tr.alphas.detach().cpu().numpy()
+    return {"steps": steps + 1, "batch": 512, "iters": 20, "classes": tr.graph.classes,
+            "ms_per_step": round(secs * 1e3 / steps, 3),
+            "loss_first": round(tr.history[0], 5), "loss_last": round(float(np.mean(tr.history[-10:])), 5),
+            "alphas": [[round(float(x), 4) for x in row] for row in a],
+            "fer_block_errors": fer}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,7 +278,12 @@ def main():
                     help="skip the FP16 / INT8 message-storage comparison")
     ap.add_argument("--form-fer-windows", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="live", choices=["live", "ldm", "mixed"])
+    ap.add_argument("--workload", default="live", choices=["live", "ldm", "mixed", "learn"])
+    ap.add_argument("--learn-steps", type=int, default=300)
+    ap.add_argument("--learn-eval", type=int, default=100000)
+    ap.add_argument("--learn-lr", type=float, default=0.01)
+    ap.add_argument("--no-learn", action="store_true",
+                    help="skip the short alpha-fitting summary in the live-frame line")
     ap.add_argument("--mixed-frames", type=int, default=1,
                     help="frames of the uneven workload per batched launch")
     ap.add_argument("--mixed-fer-frames", type=int, default=200)
@@ -294,6 +326,22 @@ def main():
             "config": {"workload": (f"LDM: {args.ldm_frames} frames x {LDM_FRAME_CELLS} cells, "
                                     f"core QPSK 4/15 + enhanced 64-NUC 10/15 N=64800 at -5 dB "
                                     f"injection, AWGN"), "messages": args.msg, **res},
+            "roofline": None, "cpu_baseline": None}), flush=True)
+        return
+    if args.workload == "learn":
+        res = learn_bench(args, dev, args.learn_steps, args.learn_eval)
+        tot = {k: sum(v[k] for v in res["fer_block_errors"].values())
+               for k in ("learned", "plain_unrolled", "plain_minsum", "learned_fallback")}
+        nb = sum(v["blocks"] for v in res["fer_block_errors"].values())
+        print(json.dumps({
+            "metric": "training step time, unrolled min-sum + soft demapper (forward + backward)",
+            "value": res["ms_per_step"], "unit": "ms/step", "n_gpus": 1, "steps": res["steps"],
+            "warmup": 1, "ms_per_step": res["ms_per_step"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": ("differentiation: 20-iteration unrolled normalized min-sum, "
+                                    "16200 r6/15 QPSK AWGN Eb/N0 U[1,3] dB, batch 512, "
+                                    "per-iteration per-degree factors"),
+                       "fer_total": {k: v / nb for k, v in tot.items()}, "eval_blocks": nb, **res},
             "roofline": None, "cpu_baseline": None}), flush=True)
         return
     if args.workload == "mixed":
@@ -469,6 +517,9 @@ def main():
         del z, w, llr
         torch.cuda.empty_cache()
         mixed = mixed_bench(args, dev)
+    learn = None
+    if not args.no_learn:
+        learn = learn_bench(args, dev, 100, 10000, (1.5, 2.5))
     ldm = None
     if not args.no_ldm:
         torch.cuda.empty_cache()
@@ -490,7 +541,7 @@ def main():
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
                    "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
-                   "forms": forms, "mixed": mixed, "ldm": ldm},
+                   "forms": forms, "mixed": mixed, "ldm": ldm, "learn": learn},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
