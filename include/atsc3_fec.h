@@ -208,6 +208,103 @@ int32_t atsc3_ldm_cancel_dev(const float *cells, const uint8_t *cw,
                              float out_scale, float *resid_out,
                              double *gain_out, double *ws, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Differentiable decoding (training; no upstream counterpart for the
+ * backward -- the forward update is that of lab/ldpc_kernel.c:82-143 without
+ * the syndrome stop, on the CSR tables of lab/m3_ldpc.py:459-482).
+ *
+ * Unrolled normalized min-sum, one iteration per call pair:
+ *   check_fwd: m_out[b][e] = alpha_t[ccls[c]] * s_e * mu_e from
+ *              x = tot[b][cidx[e]] - m_in[b][e] (m_in NULL = zero messages);
+ *              optional state (st_mn [nblk*nc*2], st_idx, st_sgn [nblk*nc])
+ *              records min1/min2, argmins | parity << 16, sign bits of x
+ *   var_fwd:   tot[b][v] = lam[b][v] + sum over v's edges of m[b][e]
+ *   check_bwd: from g_tot = dL/dtot_{t+1}, g_carry = G_x of iteration t+1
+ *              (NULL at the last iteration; must not alias g_x) and the
+ *              recorded state: g_x [nblk*E] (subgradient of the min1/min2
+ *              selection) and g_alpha_part [nblk*nc] (dL/dalpha per check)
+ *   var_bwd:   g_tot[b][v] = sum over v's edges of g_x; g_lam += g_tot_next
+ * Check degree <= 32.  _f32 and _f64 variants.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_unrolled_check_fwd_f32(const float *tot, const float *m_in,
+                                     float *m_out, const int32_t *cptr,
+                                     const int32_t *cidx, const int32_t *ccls,
+                                     const float *alpha_t, int32_t nblk,
+                                     int32_t n, int32_t nc, int32_t E,
+                                     float *st_mn, int32_t *st_idx,
+                                     uint32_t *st_sgn, void *stream);
+int32_t atsc3_unrolled_check_fwd_f64(const double *tot, const double *m_in,
+                                     double *m_out, const int32_t *cptr,
+                                     const int32_t *cidx, const int32_t *ccls,
+                                     const double *alpha_t, int32_t nblk,
+                                     int32_t n, int32_t nc, int32_t E,
+                                     double *st_mn, int32_t *st_idx,
+                                     uint32_t *st_sgn, void *stream);
+int32_t atsc3_unrolled_var_fwd_f32(const float *lam, const float *m,
+                                   const int32_t *vptr, const int32_t *vedge,
+                                   float *tot, int32_t nblk, int32_t n,
+                                   int32_t E, void *stream);
+int32_t atsc3_unrolled_var_fwd_f64(const double *lam, const double *m,
+                                   const int32_t *vptr, const int32_t *vedge,
+                                   double *tot, int32_t nblk, int32_t n,
+                                   int32_t E, void *stream);
+int32_t atsc3_unrolled_check_bwd_f32(const float *g_tot, const float *g_carry,
+                                     const int32_t *cptr, const int32_t *cidx,
+                                     const int32_t *ccls, const float *alpha_t,
+                                     const float *st_mn, const int32_t *st_idx,
+                                     const uint32_t *st_sgn, int32_t nblk,
+                                     int32_t n, int32_t nc, int32_t E,
+                                     float *g_x, float *g_alpha_part,
+                                     void *stream);
+int32_t atsc3_unrolled_check_bwd_f64(const double *g_tot, const double *g_carry,
+                                     const int32_t *cptr, const int32_t *cidx,
+                                     const int32_t *ccls, const double *alpha_t,
+                                     const double *st_mn, const int32_t *st_idx,
+                                     const uint32_t *st_sgn, int32_t nblk,
+                                     int32_t n, int32_t nc, int32_t E,
+                                     double *g_x, double *g_alpha_part,
+                                     void *stream);
+int32_t atsc3_unrolled_var_bwd_f32(const float *g_x, const float *g_tot_next,
+                                   const int32_t *vptr, const int32_t *vedge,
+                                   float *g_tot, float *g_lam, int32_t nblk,
+                                   int32_t n, int32_t E, void *stream);
+int32_t atsc3_unrolled_var_bwd_f64(const double *g_x, const double *g_tot_next,
+                                   const int32_t *vptr, const int32_t *vedge,
+                                   double *g_tot, double *g_lam, int32_t nblk,
+                                   int32_t n, int32_t E, void *stream);
+
+/* Exhaustive max-log soft demapper for any constellation of M = 2^nb points
+ * (same LLR as lab/m6_bicm.py:213-225), codeword-order output through
+ * lam_of_q, and its backward: g_z [nblk*ncell*2], per-workgroup point
+ * partials g_pts_part [grid*M*2], sigma2 partials g_s2_part [nblk*grid]
+ * (zeroed by the caller), grid = atsc3_soft_demap_bwd_grid(nblk, ncell).
+ * z is the (re, im) view of the cells; csi may be NULL.                     */
+int32_t atsc3_soft_demap_fwd_f32(const float *z, const float *csi,
+                                 const float *sigma2, const float *pts,
+                                 int32_t M, int32_t nb,
+                                 const int32_t *lam_of_q, int32_t nblk,
+                                 int32_t ncell, float *out, void *stream);
+int32_t atsc3_soft_demap_fwd_f64(const double *z, const double *csi,
+                                 const double *sigma2, const double *pts,
+                                 int32_t M, int32_t nb,
+                                 const int32_t *lam_of_q, int32_t nblk,
+                                 int32_t ncell, double *out, void *stream);
+int32_t atsc3_soft_demap_bwd_grid(int32_t nblk, int32_t ncell);
+int32_t atsc3_soft_demap_bwd_f32(const float *z, const float *csi,
+                                 const float *sigma2, const float *pts,
+                                 int32_t M, int32_t nb,
+                                 const int32_t *lam_of_q, int32_t nblk,
+                                 int32_t ncell, const float *g_llr, float *g_z,
+                                 float *g_pts_part, float *g_s2_part,
+                                 void *stream);
+int32_t atsc3_soft_demap_bwd_f64(const double *z, const double *csi,
+                                 const double *sigma2, const double *pts,
+                                 int32_t M, int32_t nb,
+                                 const int32_t *lam_of_q, int32_t nblk,
+                                 int32_t ncell, const double *g_llr,
+                                 double *g_z, double *g_pts_part,
+                                 double *g_s2_part, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
