@@ -40,15 +40,24 @@ struct QcArgs {
 // Message storage codecs.  FMT 0 is exact FP32.
 template <int FMT>
 struct Msg;
+// ld = cvt(raw(p)): the engines issue every raw load of a step before the
+// first conversion (a conversion right after its load would make the wave
+// wait for that load before issuing the next one).
 template <>
 struct Msg<0> {
     using T = float;
+    using R = float;
+    __device__ static R raw(const T *p) { return *p; }
+    __device__ static float cvt(R r, float) { return r; }
     __device__ static float ld(const T *p, float) { return *p; }
     __device__ static void st(T *p, float v, float) { *p = v; }
 };
 template <>
 struct Msg<1> {
     using T = __half;
+    using R = unsigned short;
+    __device__ static R raw(const T *p) { return *reinterpret_cast<const unsigned short *>(p); }
+    __device__ static float cvt(R r, float) { return __half2float(__ushort_as_half(r)); }
     __device__ static float ld(const T *p, float) { return __half2float(*p); }
     __device__ static void st(T *p, float v, float) {
         v = fminf(fmaxf(v, -65504.0f), 65504.0f);
@@ -58,6 +67,9 @@ struct Msg<1> {
 template <>
 struct Msg<2> {
     using T = int8_t;
+    using R = int;
+    __device__ static R raw(const T *p) { return (int)*p; }
+    __device__ static float cvt(R r, float step) { return (float)r * step; }
     __device__ static float ld(const T *p, float step) { return (float)(*p) * step; }
     __device__ static void st(T *p, float v, float inv_step) {
         float q = rintf(v * inv_step);
@@ -102,32 +114,44 @@ __device__ __forceinline__ T &at(T *base, unsigned idx) {
 __device__ __forceinline__ float clamp_mag(float x) { return fminf(fabsf(x), INFINITY); }
 
 // G checks (g_0..g_{G-1}, j), all of degree D: every load of the G checks
-// is issued before the first store, then each check folds and emits.
-template <int FMT, int D, int G>
-__device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__restrict__ tot,
-                                                typename Msg<FMT>::T *__restrict__ msg,
-                                                unsigned j, int start, bool first, bool update,
-                                                float step, float inv_step) {
+// is issued before the first conversion or store, then each check folds and
+// emits.  FIRST (iteration 0: all messages zero) is a template parameter so
+// the load sequence is straight-line code in both cases.
+template <int FMT, int D, int G, bool FIRST>
+__device__ __forceinline__ unsigned check_multi_impl(const QcArgs &a,
+                                                     const float *__restrict__ tot,
+                                                     typename Msg<FMT>::T *__restrict__ msg,
+                                                     unsigned j, int start, bool update,
+                                                     float step, float inv_step) {
     using M = Msg<FMT>;
-    float x[G][D];
+    float tv[G][D];
+    typename M::R mr[G][D];
     int b0[G];
-    unsigned syn[G];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
         b0[gi] = __builtin_amdgcn_readfirstlane((a.cg_info[start + gi] >> 9) & 2047);
-        syn[gi] = 0u;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const int ex = a.be_off[2 * (b0[gi] + k)], ey = a.be_off[2 * (b0[gi] + k) + 1];
             const unsigned p = wrap360(j + (unsigned)ex);
-            const float tv = at(tot + ey, p);
-            const float mo =
-                first ? 0.0f : M::ld(&at(msg + (size_t)(b0[gi] + k) * QC_ROW, p), step);
-            syn[gi] ^= (unsigned)(tv < 0.0f);
-            x[gi][k] = tv - mo;
+            tv[gi][k] = at(tot + ey, p);
+            if (!FIRST) mr[gi][k] = M::raw(&at(msg + (size_t)(b0[gi] + k) * QC_ROW, p));
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
+    float x[G][D];
     unsigned syn_total = 0u;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        unsigned syn = 0u;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float mo = FIRST ? 0.0f : M::cvt(mr[gi][k], step);
+            syn ^= (unsigned)(tv[gi][k] < 0.0f);
+            x[gi][k] = tv[gi][k] - mo;
+        }
+        syn_total += syn;
+    }
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
         unsigned par = 0u;
@@ -140,7 +164,6 @@ __device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__
             m1 = fminf(m1, mag);
             m2 = n2;
         }
-        syn_total += syn[gi];
         if (update) {
             const float a1 = m1 * a.alpha, a2 = m2 * a.alpha;
 #pragma unroll
@@ -153,6 +176,17 @@ __device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__
         }
     }
     return syn_total;
+}
+
+template <int FMT, int D, int G>
+__device__ __forceinline__ unsigned check_multi(const QcArgs &a, const float *__restrict__ tot,
+                                                typename Msg<FMT>::T *__restrict__ msg,
+                                                unsigned j, int start, bool first, bool update,
+                                                float step, float inv_step) {
+    return first ? check_multi_impl<FMT, D, G, true>(a, tot, msg, j, start, update, step,
+                                                     inv_step)
+                 : check_multi_impl<FMT, D, G, false>(a, tot, msg, j, start, update, step,
+                                                      inv_step);
 }
 
 // Any degree, possibly with the missing corner entry: two passes, the
