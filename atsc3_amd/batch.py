@@ -65,6 +65,7 @@ class CodeBank:
         self.desc = torch.from_numpy(raw).to(self.device)
         self.max_nvg = max(t.qc["nvg"] for t in self.tabs)
         self.max_nbe = max(t.qc["nbe"] for t in self.tabs)
+        self.max_vdeg = max(int(t.qc["max_vdeg"]) for t in self.tabs)
 
     def __len__(self):
         return len(self.codes)
@@ -147,7 +148,8 @@ class BatchPlan:
         st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         _lib.call("atsc3_minsum_batch_dev", self.llr.data_ptr(), self.nblk,
                   self.blk_off.data_ptr(), self.blk_code.data_ptr(), self.bank.desc.data_ptr(),
-                  len(self.bank), self.bank.max_nvg, self.bank.max_nbe, self.iters, self.alpha,
+                  len(self.bank), self.bank.max_nvg, self.bank.max_nbe, self.bank.max_vdeg,
+                  self.iters, self.alpha,
                   self.fmt, None if self.msg_scale is None else self.msg_scale.data_ptr(),
                   self.slots, self.grid, self.tot.data_ptr(), self.msg.data_ptr(),
                   self.ctl.data_ptr(), self.bits.data_ptr(), self.conv.data_ptr(),
@@ -249,7 +251,7 @@ def decode_batch_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", msg_s
     llr = llr.contiguous()
     _lib.call("atsc3_minsum_batch_dev", llr.data_ptr(), nblk, blk_off.data_ptr(),
               blk_code.data_ptr(), bank.desc.data_ptr(), 1, bank.max_nvg, bank.max_nbe,
-              int(iters), float(np.float32(alpha)), fmt, sp, slots, grid, tot.data_ptr(),
+              bank.max_vdeg, int(iters), float(np.float32(alpha)), fmt, sp, slots, grid, tot.data_ptr(),
               msg.data_ptr(), ctl.data_ptr(), bits.data_ptr(), conv.data_ptr(), its.data_ptr(),
               bad.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
     return bits, conv, its, bad

@@ -19,28 +19,48 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libatsc3fec.so")
-SOURCES = ["ldpc_qc.hip", "ldpc_qc_phase.hip", "ldpc_batch.hip", "ldpc_csr.hip", "demap.hip",
-           "ldm.hip", "unrolled.hip"]
+SOURCES = ["ldpc_qc.hip", "ldpc_qc_phase.hip", "ldpc_batch.hip", "ldpc_batch_f32.hip",
+           "ldpc_batch_f16.hip", "ldpc_batch_i8.hip", "ldpc_csr.hip", "demap.hip", "ldm.hip",
+           "unrolled.hip"]
 ARCH = os.environ.get("ATSC3_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
+OBJDIR = os.path.join(CSRC, "_obj")
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compile every source to an object (in parallel), then link the .so."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJDIR, exist_ok=True)
+    jobs = jobs or max(1, min(len(SOURCES), len(os.sched_getaffinity(0)), 16))
+    procs, objs = [], []
+    pending = list(SOURCES)
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            s = pending.pop(0)
+            obj = os.path.join(OBJDIR, s.replace(".hip", ".o"))
+            objs.append(obj)
+            cmd = [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, s), "-o", obj]
+            if verbose:
+                print(" ".join(cmd))
+            procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                              stderr=subprocess.PIPE, text=True)))
+        s, p = procs.pop(0)
+        _, err = p.communicate()
+        if p.returncode != 0:
+            for _, q in procs:
+                q.kill()
+            raise RuntimeError(f"hipcc failed on {s} ({p.returncode}):\n{err[-4000:]}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *srcs]
-    if verbose:
-        print(" ".join(cmd))
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-4000:]}")
+        raise RuntimeError(f"link failed ({r.returncode}):\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
     return LIB
 

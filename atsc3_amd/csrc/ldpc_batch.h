@@ -1,0 +1,65 @@
+// Shared definitions of the batched persistent min-sum engine (control
+// layout and kernel arguments); see ldpc_batch.hip for the schedule.
+#pragma once
+#include "ldpc_qc_common.h"
+
+namespace atsc3 {
+
+// Mirrors struct atsc3_code_tab of the public header (same field order and
+// sizes); pointers are device addresses of the code's kernel tables.
+struct CodeTab {
+    int n, ncg, nvg, nbe, nsc, nsv, type_a, max_vdeg;
+    const QC_CONST int *cg_info, *be_off, *be_miss, *vg_info, *ve_off, *ve_miss, *c_steps,
+        *v_steps;
+};
+static_assert(sizeof(CodeTab) == 96, "code table descriptor layout");
+
+enum { H_ARRIVE = 0, H_GEN, H_ABORT, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS,
+       H_WORDS = 16 };
+enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2 };
+
+__host__ __device__ inline int round4(int x) { return (x + 3) & ~3; }
+
+struct CtlLayout {
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, words;
+    __host__ __device__ CtlLayout(int nblk, int slots) {
+        var_list = H_WORDS;
+        var_pre = var_list + 4 * slots;
+        chk_list = var_pre + round4(slots + 1);
+        chk_pre = chk_list + 4 * slots;
+        unsat = chk_pre + round4(slots + 1);
+        free_list = unsat + round4(slots);
+        pend_list = free_list + round4(slots);
+        steps = pend_list + round4(slots);
+        words = steps + round4(nblk);
+    }
+};
+
+struct BatchArgs {
+    const float *llr;
+    const QC_CONST long long *blk_off;
+    const QC_CONST int *blk_code;
+    const QC_CONST CodeTab *codes;
+    int nblk, slots, iters, max_rounds;
+    float alpha;
+    const float *msg_scale;
+    float *tot_ws;
+    void *msg_ws;
+    long long tot_stride, msg_stride;   // elements per slot
+    int *ctl;
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps;
+    uint8_t *bits, *conv;
+    int *its, *bad;
+};
+
+
+// Per-format launchers and capacities, one translation unit per message
+// format (ldpc_batch_f32/_f16/_i8.hip) so the variants build in parallel.
+int32_t batch_launch_f32(const BatchArgs &b, int grid, hipStream_t st, bool v16);
+int32_t batch_launch_f16(const BatchArgs &b, int grid, hipStream_t st, bool v16);
+int32_t batch_launch_i8(const BatchArgs &b, int grid, hipStream_t st, bool v16);
+int batch_capacity_f32();
+int batch_capacity_f16();
+int batch_capacity_i8();
+
+}  // namespace atsc3

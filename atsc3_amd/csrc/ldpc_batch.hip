@@ -31,523 +31,7 @@
 // wait is bounded: a workgroup that waits beyond the limit sets the abort
 // word and every workgroup leaves at its next barrier, so the kernel always
 // ends (the host reads the abort word to report the failure).
-#include "ldpc_qc_common.h"
-
-#define BATCH_SPIN_LIMIT (1u << 22)
-
-namespace atsc3 {
-
-// Mirrors struct atsc3_code_tab of the public header (same field order and
-// sizes); pointers are device addresses of the code's kernel tables.
-struct CodeTab {
-    int n, ncg, nvg, nbe, nsc, nsv, type_a, max_vdeg;
-    const QC_CONST int *cg_info, *be_off, *be_miss, *vg_info, *ve_off, *ve_miss, *c_steps,
-        *v_steps;
-};
-static_assert(sizeof(CodeTab) == 96, "code table descriptor layout");
-
-enum { H_ARRIVE = 0, H_GEN, H_ABORT, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS,
-       H_WORDS = 16 };
-enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2 };
-
-__host__ __device__ inline int round4(int x) { return (x + 3) & ~3; }
-
-struct CtlLayout {
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, words;
-    __host__ __device__ CtlLayout(int nblk, int slots) {
-        var_list = H_WORDS;
-        var_pre = var_list + 4 * slots;
-        chk_list = var_pre + round4(slots + 1);
-        chk_pre = chk_list + 4 * slots;
-        unsat = chk_pre + round4(slots + 1);
-        free_list = unsat + round4(slots);
-        pend_list = free_list + round4(slots);
-        steps = pend_list + round4(slots);
-        words = steps + round4(nblk);
-    }
-};
-
-struct BatchArgs {
-    const float *llr;
-    const QC_CONST long long *blk_off;
-    const QC_CONST int *blk_code;
-    const QC_CONST CodeTab *codes;
-    int nblk, slots, iters, max_rounds;
-    float alpha;
-    const float *msg_scale;
-    float *tot_ws;
-    void *msg_ws;
-    long long tot_stride, msg_stride;   // elements per slot
-    int *ctl;
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps;
-    uint8_t *bits, *conv;
-    int *its, *bad;
-};
-
-// Loads of words the kernel itself writes: relaxed atomic loads are never
-// turned into scalar-cache loads, and after the barrier's acquire fence they
-// read what the other workgroups published.
-__device__ __forceinline__ int ldw(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-
-__device__ __forceinline__ QcArgs qc_of(const QC_CONST CodeTab *ct, float alpha) {
-    QcArgs a{};
-    a.n = ct->n;
-    a.ncg = ct->ncg;
-    a.nvg = ct->nvg;
-    a.nbe = ct->nbe;
-    a.cg_info = ct->cg_info;
-    a.be_off = ct->be_off;
-    a.be_miss = ct->be_miss;
-    a.vg_info = ct->vg_info;
-    a.ve_off = ct->ve_off;
-    a.ve_miss = ct->ve_miss;
-    a.c_steps = ct->c_steps;
-    a.v_steps = ct->v_steps;
-    a.nsc = ct->nsc;
-    a.nsv = ct->nsv;
-    a.alpha = alpha;
-    return a;
-}
-
-// Exclusive workgroup scan of two ints; totals returned to every thread.
-// Uses 2 * (ATSC3_WG / 64) ints of LDS and two barriers.
-__device__ inline void wg_scan2(int x0, int x1, int &e0, int &e1, int &t0, int &t1, int *lds) {
-    int s0 = x0, s1 = x1;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y0 = __shfl_up(s0, off, 64), y1 = __shfl_up(s1, off, 64);
-        if (lane >= off) {
-            s0 += y0;
-            s1 += y1;
-        }
-    }
-    if (lane == 63) {
-        lds[wave] = s0;
-        lds[ATSC3_WG / 64 + wave] = s1;
-    }
-    __syncthreads();
-    int b0 = 0, b1 = 0;
-    t0 = 0;
-    t1 = 0;
-#pragma unroll
-    for (int w = 0; w < ATSC3_WG / 64; ++w) {
-        const int v0 = lds[w], v1 = lds[ATSC3_WG / 64 + w];
-        if (w < wave) {
-            b0 += v0;
-            b1 += v1;
-        }
-        t0 += v0;
-        t1 += v1;
-    }
-    e0 = b0 + s0 - x0;
-    e1 = b1 + s1 - x1;
-    __syncthreads();
-}
-
-__device__ __forceinline__ void put4(int *p, int x, int y, int z, int w) {
-    p[0] = x;
-    p[1] = y;
-    p[2] = z;
-    p[3] = w;
-}
-
-// Next round's lists (run by one whole workgroup).  With n_chk = 0 it only
-// admits codewords (the initial fill).
-__device__ void compact(const BatchArgs &b) {
-    __shared__ int lds[4 * (ATSC3_WG / 64)];
-    int *h = b.ctl;
-    int *var_list = b.ctl + b.var_list, *var_pre = b.ctl + b.var_pre;
-    int *chk_list = b.ctl + b.chk_list, *chk_pre = b.ctl + b.chk_pre;
-    int *unsat = b.ctl + b.unsat, *fl = b.ctl + b.free_list, *pl = b.ctl + b.pend_list;
-    const int tid = threadIdx.x;
-    const int n = ldw(h + H_NCHK), n_pend = ldw(h + H_NPEND), qn = ldw(h + H_QNEXT);
-    int n_free = ldw(h + H_NFREE);
-    // slots retired last round have been read by their RETIRE pass: free them
-    for (int k = tid; k < n_pend; k += ATSC3_WG) fl[n_free + k] = ldw(pl + k);
-    n_free += n_pend;
-    __syncthreads();
-    int n_keep = 0, n_ret = 0, bv = 0, bc = 0;
-    for (int k0 = 0; k0 < n; k0 += ATSC3_WG) {
-        const int k = k0 + tid;
-        const bool valid = k < n;
-        int slot = 0, blk = 0, cz = 0, u = 0, code = 0, bit = 0;
-        bool ret = false;
-        if (valid) {
-            slot = ldw(chk_list + 4 * k);
-            blk = ldw(chk_list + 4 * k + 1);
-            cz = ldw(chk_list + 4 * k + 2);
-            code = cz & 255;
-            bit = cz >> 12;
-            u = ldw(unsat + slot);
-            ret = u == 0 || bit >= b.iters;
-        }
-        const bool keep = valid && !ret;
-        int ek, er, tk, tr, ev, ec, tv, tc;
-        wg_scan2(keep ? 1 : 0, ret ? 1 : 0, ek, er, tk, tr, lds);
-        wg_scan2(valid ? b.codes[code].nsv : 0, keep ? b.codes[code].nsc : 0, ev, ec, tv, tc,
-                 lds);
-        if (valid) {
-            put4(var_list + 4 * k, slot, blk, code | ((ret ? M_RETIRE : M_CONT) << 8) | (bit << 12),
-                 u);
-            var_pre[k] = bv + ev;
-            if (keep) {
-                const int q = n_keep + ek;
-                put4(chk_list + 4 * q, slot, blk, code | ((bit + 1) << 12), 0);
-                chk_pre[q] = bc + ec;
-                unsat[slot] = 0;
-            } else {
-                pl[n_ret + er] = slot;
-            }
-        }
-        n_keep += tk;
-        n_ret += tr;
-        bv += tv;
-        bc += tc;
-        __syncthreads();
-    }
-    // admit queued codewords into free slots (most recently freed first)
-    const int rem = b.nblk - qn;
-    const int adm = n_free < rem ? n_free : rem;
-    for (int i0 = 0; i0 < adm; i0 += ATSC3_WG) {
-        const int i = i0 + tid;
-        const bool valid = i < adm;
-        int slot = 0, blk = 0, code = 0;
-        if (valid) {
-            slot = ldw(fl + n_free - 1 - i);
-            blk = qn + i;
-            code = b.blk_code[blk];
-        }
-        int ev, ec, tv, tc;
-        wg_scan2(valid ? b.codes[code].nsv : 0, valid ? b.codes[code].nsc : 0, ev, ec, tv, tc,
-                 lds);
-        if (valid) {
-            put4(var_list + 4 * (n + i), slot, blk, code | (M_INIT << 8), 0);
-            var_pre[n + i] = bv + ev;
-            put4(chk_list + 4 * (n_keep + i), slot, blk, code, 0);
-            chk_pre[n_keep + i] = bc + ec;
-            unsat[slot] = 0;
-        }
-        bv += tv;
-        bc += tc;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        var_pre[n + adm] = bv;
-        chk_pre[n_keep + adm] = bc;
-        h[H_NVAR] = n + adm;
-        h[H_NCHK] = n_keep + adm;
-        h[H_NFREE] = n_free - adm;
-        h[H_NPEND] = n_ret;
-        h[H_QNEXT] = qn + adm;
-        h[H_DONE] = (n_keep + adm == 0 && qn + adm == b.nblk) ? 1 : 0;
-        const int rounds = ldw(h + H_ROUNDS) + 1;
-        h[H_ROUNDS] = rounds;
-        // every codeword leaves its slot within iters + 3 rounds of admission,
-        // so this bound is never reached by a correct schedule: it only
-        // guarantees that the kernel ends
-        if (rounds > b.max_rounds) h[H_ABORT] = 1;
-    }
-}
-
-// Wait until this wave's vector memory operations (stores and atomics
-// included) have completed at the L2 (gfx9 counts stores in vmcnt):
-// s_waitcnt vmcnt(0), expcnt and lgkmcnt left unconstrained.
-__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
-// Grid barrier number `epoch` (1, 2, ...); the last workgroup to arrive runs
-// the compaction first when asked.  Returns false when the abort word is set.
-//
-// Visibility across XCDs (each has its own L2): every wave first waits for
-// its own stores to complete at its L2; after the workgroup barrier ONE wave
-// issues the agent-scope release (an L2 write-back, which covers the whole
-// workgroup's stores since they are all in that L2) before arriving, and
-// after the release of the barrier ONE wave issues the agent-scope acquire
-// (L2/L1 invalidation, the workgroup's waves share the CU's L1) before the
-// workgroup proceeds.  One write-back and one invalidation per workgroup
-// instead of one per wave: the barrier cost scales with their number.
-__device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact, int *s_flag) {
-    unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
-    wait_vmem();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned old =
-            __hip_atomic_fetch_add(h + H_ARRIVE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_flag[0] = old + 1u == epoch * gridDim.x;
-        if (s_flag[0]) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-    if (s_flag[0]) {
-        if (do_compact) compact(b);
-        wait_vmem();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(h + H_GEN, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    } else if (threadIdx.x == 0) {
-        unsigned spins = 0;
-        while (__hip_atomic_load(h + H_GEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-            if (__hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            if (++spins >= BATCH_SPIN_LIMIT) {
-                __hip_atomic_store(h + H_ABORT, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(8);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    if (threadIdx.x == 0)
-        s_flag[1] = __hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    return s_flag[1] == 0;
-}
-
-// This workgroup's contiguous item range of a list; k0 = the entry holding
-// item i0 (found by the whole workgroup).  Returns false when empty.
-__device__ bool my_range(const int *pre, int n, int &i0, int &i1, int &k0, int *s_k) {
-    const int total = uni(ldw(pre + n));
-    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    i0 = (int)blockIdx.x * chunk;
-    i1 = i0 + chunk < total ? i0 + chunk : total;
-    if (i0 >= i1) return false;
-    if (threadIdx.x == 0) s_k[0] = 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += ATSC3_WG)
-        if (ldw(pre + k) <= i0 && i0 < ldw(pre + k + 1)) s_k[0] = k;
-    __syncthreads();
-    k0 = uni(s_k[0]);
-    __syncthreads();
-    return true;
-}
-
-__device__ __forceinline__ void flush_unsat_slot(int *unsat, int slot, unsigned cnt) {
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(reinterpret_cast<unsigned *>(unsat + slot), cnt);
-}
-
-template <int FMT, int CMAX>
-__device__ void check_phase(const BatchArgs &b, int *s_k) {
-    using T = typename Msg<FMT>::T;
-    const int n = uni(ldw(b.ctl + H_NCHK));
-    const int *list = b.ctl + b.chk_list, *pre = b.ctl + b.chk_pre;
-    int i0, i1, k;
-    if (!my_range(pre, n, i0, i1, k, s_k)) return;
-    const unsigned j = threadIdx.x;
-    for (int i = i0; i < i1; ++k) {
-        const int slot = uni(ldw(list + 4 * k)), blk = uni(ldw(list + 4 * k + 1));
-        const int cz = uni(ldw(list + 4 * k + 2));
-        const int p0 = uni(ldw(pre + k)), p1 = uni(ldw(pre + k + 1));
-        const int end = p1 < i1 ? p1 : i1;
-        const QC_CONST CodeTab *ct = b.codes + (cz & 255);
-        const QcArgs a = qc_of(ct, b.alpha);
-        const int bit = cz >> 12;
-        const bool first = bit == 0, update = bit < b.iters;
-        const float *tot = b.tot_ws + (size_t)slot * b.tot_stride;
-        T *msg = reinterpret_cast<T *>(b.msg_ws) + (size_t)slot * b.msg_stride;
-        const float step =
-            FMT == 2 ? __builtin_bit_cast(float, uni(ldw(b.ctl + b.steps + blk))) : 1.0f;
-        const float inv_step = FMT == 2 ? 1.0f / step : 1.0f;
-        unsigned cnt = 0u;
-        for (; i < end; ++i) {
-            if (j >= ATSC3_Z) continue;
-            const int sw = uni(a.c_steps[i - p0]);
-            const int D = sw & 63;
-            const int G = (sw >> 6) & 7;
-            const int start = (sw >> 9) & 4095;
-            if ((sw >> 21) & 1) {
-                const int info = a.cg_info[start];
-                cnt += check_generic<FMT>(a, tot, msg, j, (info >> 9) & 2047, info & 511, first,
-                                          update, step, inv_step);
-                continue;
-            }
-            switch (G * 64 + D) {
-#define BT_C1(D)                                                                        \
-    case 64 + D:                                                                        \
-        if constexpr (D <= CMAX)                                                        \
-            cnt += check_multi<FMT, D, 1>(a, tot, msg, j, start, first, update, step,   \
-                                          inv_step);                                    \
-        break;
-#define BT_C2(D)                                                                          \
-    case 128 + D:                                                                         \
-        cnt += check_multi<FMT, D, 2>(a, tot, msg, j, start, first, update, step, inv_step); \
-        break;
-                QC_CASES16(BT_C1)
-                QC_CASES8(BT_C2)
-#undef BT_C1
-#undef BT_C2
-                default: break;
-            }
-        }
-        flush_unsat_slot(b.ctl + b.unsat, slot, cnt);
-    }
-}
-
-template <int FMT, bool TA, int VMAX>
-__device__ __forceinline__ void var_step(const QcArgs &a, const float *lam, float *tot,
-                                         const typename Msg<FMT>::T *msg, unsigned p, int sw,
-                                         float step) {
-    const int D = sw & 63;
-    const int G = (sw >> 6) & 7;
-    const int start = (sw >> 9) & 4095;
-    if ((sw >> 21) & 1) {
-        const int info = a.vg_info[start];
-        const int t = (info >> 24) & 255;
-        var_generic<FMT, VMAX>(a, lam + t * ATSC3_Z, tot + t * QC_ROW, msg, p,
-                               (info >> 12) & 2047, info & 63, (info >> 6) & 63, step);
-        return;
-    }
-    switch (G * 64 + D) {
-#define BT_V1(D)                                                                       \
-    case 64 + D:                                                                       \
-        if constexpr (D <= VMAX) var_multi<FMT, D, 1, TA>(a, lam, tot, msg, p, start, step); \
-        break;
-#define BT_V2(D)                                                   \
-    case 128 + D:                                                  \
-        var_multi<FMT, D, 2, TA>(a, lam, tot, msg, p, start, step); \
-        break;
-#define BT_V4(D)                                                   \
-    case 256 + D:                                                  \
-        var_multi<FMT, D, 4, TA>(a, lam, tot, msg, p, start, step); \
-        break;
-        QC_CASES(BT_V1)
-        QC_CASES8(BT_V2)
-        QC_CASES4(BT_V4)
-#undef BT_V1
-#undef BT_V2
-#undef BT_V4
-        default: break;
-    }
-}
-
-template <int FMT, int VMAX>
-__device__ void var_phase(const BatchArgs &b, int *s_k) {
-    using T = typename Msg<FMT>::T;
-    const int n = uni(ldw(b.ctl + H_NVAR));
-    const int *list = b.ctl + b.var_list, *pre = b.ctl + b.var_pre;
-    int i0, i1, k;
-    if (!my_range(pre, n, i0, i1, k, s_k)) return;
-    const unsigned p = threadIdx.x;
-    for (int i = i0; i < i1; ++k) {
-        const int slot = uni(ldw(list + 4 * k)), blk = uni(ldw(list + 4 * k + 1));
-        const int cz = uni(ldw(list + 4 * k + 2)), u = uni(ldw(list + 4 * k + 3));
-        const int p0 = uni(ldw(pre + k)), p1 = uni(ldw(pre + k + 1));
-        const int end = p1 < i1 ? p1 : i1;
-        const QC_CONST CodeTab *ct = b.codes + (cz & 255);
-        const QcArgs a = qc_of(ct, b.alpha);
-        const int mode = (cz >> 8) & 3, bit = cz >> 12;
-        const long long off = b.blk_off[blk];
-        const float *lam = b.llr + off;
-        float *tot = b.tot_ws + (size_t)slot * b.tot_stride;
-        if (mode == M_RETIRE) {
-            if (i == p0 && p == 0) {
-                b.conv[blk] = (uint8_t)(u == 0);
-                b.its[blk] = bit;
-                b.bad[blk] = u;
-            }
-            uint8_t *bout = b.bits + off;
-            for (; i < end; ++i) {
-                const int sw = uni(a.v_steps[i - p0]);
-                const int G = (sw >> 6) & 7, start = (sw >> 9) & 4095;
-                if (p < ATSC3_Z)
-                    for (int gi = 0; gi < G; ++gi) {
-                        const int t = (a.vg_info[start + gi] >> 24) & 255;
-                        bout[t * ATSC3_Z + p] = (uint8_t)(tot[t * QC_ROW + p] < 0.0f);
-                    }
-            }
-            continue;
-        }
-        if (mode == M_INIT) {
-            for (; i < end; ++i) {
-                const int sw = uni(a.v_steps[i - p0]);
-                const int G = (sw >> 6) & 7, start = (sw >> 9) & 4095;
-                if (p < ATSC3_Z)
-                    for (int gi = 0; gi < G; ++gi) {
-                        const int t = (a.vg_info[start + gi] >> 24) & 255;
-                        tot[t * QC_ROW + p] = lam[t * ATSC3_Z + p];
-                    }
-            }
-            continue;
-        }
-        const T *msg = reinterpret_cast<const T *>(b.msg_ws) + (size_t)slot * b.msg_stride;
-        const float step =
-            FMT == 2 ? __builtin_bit_cast(float, uni(ldw(b.ctl + b.steps + blk))) : 1.0f;
-        if (ct->type_a) {
-            for (; i < end; ++i)
-                if (p < ATSC3_Z) var_step<FMT, true, VMAX>(a, lam, tot, msg, p, uni(a.v_steps[i - p0]), step);
-        } else {
-            for (; i < end; ++i)
-                if (p < ATSC3_Z) var_step<FMT, false, VMAX>(a, lam, tot, msg, p, uni(a.v_steps[i - p0]), step);
-        }
-    }
-}
-
-template <int FMT>
-__global__ __launch_bounds__(ATSC3_WG) void batch_kernel(BatchArgs b) {
-    __shared__ int s_k[1];
-    __shared__ int s_flag[2];
-    unsigned epoch = 0;
-    for (;;) {
-        var_phase<FMT, 32>(b, s_k);
-        if (!grid_barrier(b, ++epoch, false, s_flag)) return;
-        if (uni(ldw(b.ctl + H_DONE))) return;
-        check_phase<FMT, 16>(b, s_k);
-        if (!grid_barrier(b, ++epoch, true, s_flag)) return;
-    }
-}
-
-// Control words reset, every slot free, first admissions.
-__global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
-    int *h = b.ctl;
-    for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) h[k] = 0;
-    for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG) b.ctl[b.free_list + k] = b.slots - 1 - k;
-    if (threadIdx.x == 0) h[H_NFREE] = b.slots;
-    __syncthreads();
-    compact(b);
-}
-
-// INT8 message step per codeword (given, or 8 * mean|llr| / 127).
-__global__ __launch_bounds__(ATSC3_WG) void batch_steps(BatchArgs b) {
-    __shared__ float fslots[ATSC3_WG / 64];
-    const int blk = blockIdx.x;
-    const QC_CONST CodeTab *ct = b.codes + b.blk_code[blk];
-    const float s = b.msg_scale ? b.msg_scale[blk]
-                                : int8_step_of_block(b.llr + b.blk_off[blk], ct->n, fslots);
-    if (threadIdx.x == 0) b.ctl[b.steps + blk] = __builtin_bit_cast(int, s);
-}
-
-template <int FMT>
-static int capacity_of() {
-    static int cap[64][3];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-    int &c = cap[dev][FMT];
-    if (c == 0) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<FMT>, ATSC3_WG,
-                                                         0) != hipSuccess)
-            return 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-            hipSuccess)
-            return 0;
-        c = per_cu * cus;
-    }
-    return c;
-}
-
-template <int FMT>
-static int32_t run_batch(const BatchArgs &b, int grid, hipStream_t st) {
-    if (FMT == 2)
-        hipLaunchKernelGGL(batch_steps, dim3(b.nblk), dim3(ATSC3_WG), 0, st, b);
-    hipLaunchKernelGGL(batch_init, dim3(1), dim3(ATSC3_WG), 0, st, b);
-    hipLaunchKernelGGL((batch_kernel<FMT>), dim3(grid), dim3(ATSC3_WG), 0, st, b);
-    return launch_status();
-}
-
-}  // namespace atsc3
+#include "ldpc_batch.h"
 
 extern "C" int32_t atsc3_minsum_batch_ctl_words(int32_t nblk, int32_t slots) {
     if (nblk < 0 || slots < 1) return ATSC3_E_ARG;
@@ -556,9 +40,9 @@ extern "C" int32_t atsc3_minsum_batch_ctl_words(int32_t nblk, int32_t slots) {
 
 extern "C" int32_t atsc3_minsum_batch_max_grid(int32_t msg_fmt) {
     switch (msg_fmt) {
-        case 0: return atsc3::capacity_of<0>();
-        case 1: return atsc3::capacity_of<1>();
-        case 2: return atsc3::capacity_of<2>();
+        case 0: return atsc3::batch_capacity_f32();
+        case 1: return atsc3::batch_capacity_f16();
+        case 2: return atsc3::batch_capacity_i8();
         default: return ATSC3_E_ARG;
     }
 }
@@ -567,7 +51,8 @@ extern "C" int32_t atsc3_minsum_batch_status_word(void) { return atsc3::H_ABORT;
 
 extern "C" int32_t atsc3_minsum_batch_dev(
     const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
-    const void *codes, int32_t ncode, int32_t max_nvg, int32_t max_nbe, int32_t iters,
+    const void *codes, int32_t ncode, int32_t max_nvg, int32_t max_nbe, int32_t max_vdeg,
+    int32_t iters,
     float alpha, int32_t msg_fmt, const float *msg_scale, int32_t slots, int32_t grid,
     float *tot_ws, void *msg_ws, int32_t *ctl_ws, uint8_t *bits, uint8_t *conv, int32_t *its,
     int32_t *bad, void *stream) {
@@ -577,7 +62,7 @@ extern "C" int32_t atsc3_minsum_batch_dev(
         !conv || !its || !bad)
         return ATSC3_E_ARG;
     if (ncode < 1 || ncode > 256 || max_nvg < 1 || max_nvg > 255 || max_nbe < 1 ||
-        max_nbe >= 2048 || slots < 1)
+        max_nbe >= 2048 || slots < 1 || max_vdeg < 1 || max_vdeg > 32)
         return ATSC3_E_ARG;
     if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
     const int cap = atsc3_minsum_batch_max_grid(msg_fmt);
@@ -615,9 +100,10 @@ extern "C" int32_t atsc3_minsum_batch_dev(
     b.its = its;
     b.bad = bad;
     hipStream_t st = atsc3::as_stream(stream);
+    const bool v16 = max_vdeg <= 16;
     switch (msg_fmt) {
-        case 0: return atsc3::run_batch<0>(b, grid, st);
-        case 1: return atsc3::run_batch<1>(b, grid, st);
-        default: return atsc3::run_batch<2>(b, grid, st);
+        case 0: return atsc3::batch_launch_f32(b, grid, st, v16);
+        case 1: return atsc3::batch_launch_f16(b, grid, st, v16);
+        default: return atsc3::batch_launch_i8(b, grid, st, v16);
     }
 }

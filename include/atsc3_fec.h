@@ -133,6 +133,10 @@ int32_t atsc3_minsum_qc_phased_dev(const float *llr, int32_t nblk, int32_t n,
  *   blk_code  [nblk] int32 index into codes (device)
  *   blk_off   [nblk] int64 offset of block b's LLRs in llr and of its bits
  *             in bits (device; the rows may have different lengths)
+ *   max_nvg, max_nbe, max_vdeg  maxima over the codes (variable groups,
+ *             base edges, variable degree <= 32)
+ *   msg_fmt   0 FP32 messages (bit-exact with the reference), 1 FP16, 2 INT8
+ *             (the storage codecs of atsc3_minsum_qc_dev)
  *   slots     codewords decoded concurrently; the rest wait in the queue
  *   grid      workgroups of the persistent kernel, 1 ..
  *             atsc3_minsum_batch_max_grid(msg_fmt) (all must be resident)
@@ -154,7 +158,8 @@ int32_t atsc3_minsum_batch_status_word(void);
 int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
                                const int64_t *blk_off, const int32_t *blk_code,
                                const void *codes, int32_t ncode,
-                               int32_t max_nvg, int32_t max_nbe, int32_t iters,
+                               int32_t max_nvg, int32_t max_nbe,
+                               int32_t max_vdeg, int32_t iters,
                                float alpha, int32_t msg_fmt,
                                const float *msg_scale, int32_t slots,
                                int32_t grid, float *tot_ws, void *msg_ws,

@@ -2,8 +2,8 @@
 bit-exact with the reference min-sum in FP32 -- the golden fixtures of every
 code decoded together in mixed batches, live mixed frames against the
 oracle, tiny slot/grid counts that force queueing and slot recycling -- and
-identical to the per-code phase engine in FP16 / INT8 storage.  HIP-graph
-replays are checked against eager runs."""
+identical to the per-code engines in FP16 / INT8 storage.  HIP-graph replays
+are checked against eager runs."""
 import numpy as np
 import pytest
 
@@ -94,11 +94,11 @@ def test_live_mixed_vs_oracle(device, nblk, seed, slots, grid):
 @pytest.mark.parametrize("msg", ["float16", "int8"])
 def test_low_precision_matches_phase_engine(device, msg):
     """FP16 / INT8 storage: same arithmetic as the per-code engines, so the
-    batched results equal decode_dev per code group."""
+    batched results equal decode_dev per code group (fused engine)."""
     import torch
     from atsc3_amd import ldpc
     codes, llrs = _mixed_llrs(36, 11)
-    print(f"[gpu] batch {msg} vs phase engine", flush=True)
+    print(f"[gpu] batch {msg} vs per-code engine", flush=True)
     got, _ = _batch(codes, llrs, device, 50, 0.75, msg=msg, slots=7)
     for c in {id(c): c for c in codes}.values():
         idx = [b for b in range(len(codes)) if codes[b] is c]
