@@ -63,6 +63,8 @@ _SIGS = {
                                  c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_soft_demap_bwd_f64": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32,
                                  c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_baseband_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                           c_vp],
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],

@@ -214,6 +214,25 @@ int32_t atsc3_ldm_cancel_dev(const float *cells, const uint8_t *cw,
                              double *gain_out, double *ws, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Baseband Packet step after the decode: BCH syndrome and descrambling.
+ * Replaces lab/m6_bicm.py:357-364 (PlpChain.baseband_packet) and its batched
+ * form lab/m9_fast.py:942-949 (bch_syndrome_batch of lab/m9_accel.py:150-236
+ * + descramble of lab/m4_scrambler.py:148-165).
+ *   bits     [nblk*n] u8 decoded codewords (0/1), first kldpc = BCH codeword
+ *   mask     u32 [ceil(kldpc/32)][mouter] syndrome matrix (bit 31-t of
+ *            [w][j] = coefficient j of x^(kldpc-1-(32w+t)) mod g)
+ *   seq      [(kldpc-mouter)/8] scrambling bytes (A/322 5.2.3, MSB first)
+ *   syn_out  [nblk][8] u32: syndrome, coefficient j at limb j/32 bit j%32
+ *   ok_out   [nblk] u8: syndrome == 0
+ *   bytes_out[nblk][(kldpc-mouter)/8] descrambled payload bytes
+ * kldpc <= 58368; mouter <= 256.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_baseband_dev(const uint8_t *bits, int32_t nblk, int32_t n,
+                           int32_t kldpc, int32_t mouter, const uint32_t *mask,
+                           const uint8_t *seq, uint32_t *syn_out,
+                           uint8_t *ok_out, uint8_t *bytes_out, void *stream);
+
+/* ------------------------------------------------------------------------
  * Differentiable decoding (training; no upstream counterpart for the
  * backward -- the forward update is that of lab/ldpc_kernel.c:82-143 without
  * the syndrome stop, on the CSR tables of lab/m3_ldpc.py:459-482).
