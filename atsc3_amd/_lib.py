@@ -38,8 +38,8 @@ _SIGS = {
     "atsc3_minsum_batch_max_grid": [c_i32],
     "atsc3_minsum_batch_status_word": [],
     "atsc3_minsum_batch_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
-                               c_i32, c_f32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp,
-                               c_vp, c_vp, c_vp, c_vp, c_vp],
+                               c_i32, c_f32, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp,
+                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_unrolled_check_fwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                                      c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
     "atsc3_unrolled_check_fwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,

@@ -26,6 +26,7 @@ from .codes import LdpcCode
 from .ldpc import MSG_BYTES, MSG_FORMATS, device_tables
 
 ROW = LdpcCode.ROW
+MAX_RUNGS = 4            # alpha ladder rungs run inside the kernel
 
 
 class _CodeTab(ctypes.Structure):
@@ -88,7 +89,7 @@ class BatchPlan:
     """
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75,
-                 msg_dtype="float32", slots=None, grid=None, msg_scale=None):
+                 msg_dtype="float32", slots=None, grid=None, msg_scale=None, ladder=None):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -106,6 +107,7 @@ class BatchPlan:
         self.fmt = MSG_FORMATS[msg_dtype]
         self.msg_dtype = msg_dtype
         self.iters, self.alpha = int(iters), float(np.float32(alpha))
+        self.ladder = _ladder_arg(ladder)
         ns = np.array([uniq[i].n for i in idx], np.int64)
         self.n_of = ns
         self.off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
@@ -149,7 +151,7 @@ class BatchPlan:
         _lib.call("atsc3_minsum_batch_dev", self.llr.data_ptr(), self.nblk,
                   self.blk_off.data_ptr(), self.blk_code.data_ptr(), self.bank.desc.data_ptr(),
                   len(self.bank), self.bank.max_nvg, self.bank.max_nbe, self.bank.max_vdeg,
-                  self.iters, self.alpha,
+                  self.iters, self.alpha, *self.ladder,
                   self.fmt, None if self.msg_scale is None else self.msg_scale.data_ptr(),
                   self.slots, self.grid, self.tot.data_ptr(), self.msg.data_ptr(),
                   self.ctl.data_ptr(), self.bits.data_ptr(), self.conv.data_ptr(),
@@ -212,10 +214,21 @@ def decode_mixed(llrs, codes, iters=50, alpha=0.75, msg_dtype="float32", slots=N
 _SINGLE = {}
 
 
+def _ladder_arg(ladder):
+    """(host float array or None, nrungs) for the C call."""
+    if ladder is None:
+        return (None, 0)
+    a = [float(np.float32(x)) for x in ladder]
+    if not 1 <= len(a) <= MAX_RUNGS:
+        raise ValueError(f"ladder needs 1..{MAX_RUNGS} alphas")
+    return ((ctypes.c_float * len(a))(*a), len(a))
+
+
 def decode_batch_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", msg_scale=None,
-                     slots=None, grid=None):
+                     slots=None, grid=None, ladder=None):
     """The batched engine on a (nblk, n) batch of ONE code (device tensors in
-    and out, no host sync): same results as ldpc.decode_dev."""
+    and out, no host sync): same results as ldpc.decode_dev, or with
+    ``ladder`` (1..4 alphas) those of ldpc.ladder_dev."""
     dev = llr.device
     nblk, n = llr.shape
     key = (id(code), nblk, str(dev))
@@ -251,7 +264,8 @@ def decode_batch_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", msg_s
     llr = llr.contiguous()
     _lib.call("atsc3_minsum_batch_dev", llr.data_ptr(), nblk, blk_off.data_ptr(),
               blk_code.data_ptr(), bank.desc.data_ptr(), 1, bank.max_nvg, bank.max_nbe,
-              bank.max_vdeg, int(iters), float(np.float32(alpha)), fmt, sp, slots, grid, tot.data_ptr(),
+              bank.max_vdeg, int(iters), float(np.float32(alpha)), *_ladder_arg(ladder), fmt, sp,
+              slots, grid, tot.data_ptr(),
               msg.data_ptr(), ctl.data_ptr(), bits.data_ptr(), conv.data_ptr(), its.data_ptr(),
               bad.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
     return bits, conv, its, bad

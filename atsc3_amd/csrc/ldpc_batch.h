@@ -16,12 +16,15 @@ static_assert(sizeof(CodeTab) == 96, "code table descriptor layout");
 
 enum { H_ARRIVE = 0, H_GEN, H_ABORT, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS,
        H_WORDS = 16 };
-enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2 };
+// Var-list entry modes.  RETIRE_INIT: the codeword failed at its ladder rung
+// and starts the next one in the same slot (result kept first if taken).
+enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2, M_RETIRE_INIT = 3 };
+#define BATCH_MAX_RUNGS 4
 
 __host__ __device__ inline int round4(int x) { return (x + 3) & ~3; }
 
 struct CtlLayout {
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, words;
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best, words;
     __host__ __device__ CtlLayout(int nblk, int slots) {
         var_list = H_WORDS;
         var_pre = var_list + 4 * slots;
@@ -31,7 +34,8 @@ struct CtlLayout {
         free_list = unsat + round4(slots);
         pend_list = free_list + round4(slots);
         steps = pend_list + round4(slots);
-        words = steps + round4(nblk);
+        best = steps + round4(nblk);
+        words = best + round4(nblk);
     }
 };
 
@@ -42,12 +46,14 @@ struct BatchArgs {
     const QC_CONST CodeTab *codes;
     int nblk, slots, iters, max_rounds;
     float alpha;
+    float ladder[BATCH_MAX_RUNGS];   // normalization per ladder rung
+    int nrungs;
     const float *msg_scale;
     float *tot_ws;
     void *msg_ws;
     long long tot_stride, msg_stride;   // elements per slot
     int *ctl;
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps;
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best;
     uint8_t *bits, *conv;
     int *its, *bad;
 };

@@ -52,8 +52,8 @@ extern "C" int32_t atsc3_minsum_batch_status_word(void) { return atsc3::H_ABORT;
 extern "C" int32_t atsc3_minsum_batch_dev(
     const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
     const void *codes, int32_t ncode, int32_t max_nvg, int32_t max_nbe, int32_t max_vdeg,
-    int32_t iters,
-    float alpha, int32_t msg_fmt, const float *msg_scale, int32_t slots, int32_t grid,
+    int32_t iters, float alpha, const float *ladder, int32_t nrungs, int32_t msg_fmt,
+    const float *msg_scale, int32_t slots, int32_t grid,
     float *tot_ws, void *msg_ws, int32_t *ctl_ws, uint8_t *bits, uint8_t *conv, int32_t *its,
     int32_t *bad, void *stream) {
     if (nblk < 0 || iters < 0 || iters >= (1 << 19)) return ATSC3_E_ARG;
@@ -65,6 +65,7 @@ extern "C" int32_t atsc3_minsum_batch_dev(
         max_nbe >= 2048 || slots < 1 || max_vdeg < 1 || max_vdeg > 32)
         return ATSC3_E_ARG;
     if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
+    if (ladder && (nrungs < 1 || nrungs > BATCH_MAX_RUNGS)) return ATSC3_E_ARG;
     const int cap = atsc3_minsum_batch_max_grid(msg_fmt);
     if (cap <= 0) return ATSC3_E_HIP;
     if (grid < 1 || grid > cap) return ATSC3_E_SIZE;
@@ -77,10 +78,14 @@ extern "C" int32_t atsc3_minsum_batch_dev(
     b.nblk = nblk;
     b.slots = slots;
     b.iters = iters;
-    b.max_rounds = (int)(((long long)nblk + 1) * (iters + 3) + 8 < (1LL << 30)
-                             ? ((long long)nblk + 1) * (iters + 3) + 8
-                             : (1LL << 30));
-    b.alpha = alpha;
+    b.nrungs = ladder ? nrungs : 1;
+    for (int r = 0; r < BATCH_MAX_RUNGS; ++r) b.ladder[r] = ladder && r < nrungs ? ladder[r] : alpha;
+    {
+        const long long per = (long long)b.nrungs * (iters + 2) + 1;
+        const long long mr = ((long long)nblk + 1) * per + 8;
+        b.max_rounds = (int)(mr < (1LL << 30) ? mr : (1LL << 30));
+    }
+    b.alpha = b.ladder[0];
     b.msg_scale = msg_scale;
     b.tot_ws = tot_ws;
     b.msg_ws = msg_ws;
@@ -95,6 +100,7 @@ extern "C" int32_t atsc3_minsum_batch_dev(
     b.free_list = L.free_list;
     b.pend_list = L.pend_list;
     b.steps = L.steps;
+    b.best = L.best;
     b.bits = bits;
     b.conv = conv;
     b.its = its;

@@ -94,32 +94,45 @@ __device__ void compact(const BatchArgs &b) {
     n_free += n_pend;
     __syncthreads();
     int n_keep = 0, n_ret = 0, bv = 0, bc = 0;
+    int *best = b.ctl + b.best;
     for (int k0 = 0; k0 < n; k0 += ATSC3_WG) {
         const int k = k0 + tid;
         const bool valid = k < n;
-        int slot = 0, blk = 0, cz = 0, u = 0, code = 0, bit = 0;
-        bool ret = false;
+        int slot = 0, blk = 0, cz = 0, u = 0, code = 0, bit = 0, rung = 0;
+        bool ret = false, more = false, take = false;
         if (valid) {
             slot = ldw(chk_list + 4 * k);
             blk = ldw(chk_list + 4 * k + 1);
             cz = ldw(chk_list + 4 * k + 2);
             code = cz & 255;
+            rung = (cz >> 10) & 3;
             bit = cz >> 12;
             u = ldw(unsat + slot);
             ret = u == 0 || bit >= b.iters;
+            if (ret) {
+                // a rung's result is taken when it converged or leaves fewer
+                // unsatisfied checks than every earlier rung; a failed rung
+                // hands the codeword to the next one in the same slot
+                take = u == 0 || u < ldw(best + blk);
+                if (take) best[blk] = u;
+                more = u != 0 && rung + 1 < b.nrungs;
+            }
         }
-        const bool keep = valid && !ret;
+        const bool keep = valid && (!ret || more);
+        const bool fin = valid && ret && !more;
         int ek, er, tk, tr, ev, ec, tv, tc;
-        wg_scan2(keep ? 1 : 0, ret ? 1 : 0, ek, er, tk, tr, lds);
+        wg_scan2(keep ? 1 : 0, fin ? 1 : 0, ek, er, tk, tr, lds);
         wg_scan2(valid ? b.codes[code].nsv : 0, keep ? b.codes[code].nsc : 0, ev, ec, tv, tc,
                  lds);
         if (valid) {
-            put4(var_list + 4 * k, slot, blk, code | ((ret ? M_RETIRE : M_CONT) << 8) | (bit << 12),
-                 u);
+            const int mode = !ret ? M_CONT : (more ? M_RETIRE_INIT : M_RETIRE);
+            put4(var_list + 4 * k, slot, blk,
+                 code | (mode << 8) | (rung << 10) | (bit << 12), u | ((take ? 1 : 0) << 30));
             var_pre[k] = bv + ev;
             if (keep) {
                 const int q = n_keep + ek;
-                put4(chk_list + 4 * q, slot, blk, code | ((bit + 1) << 12), 0);
+                const int nz = ret ? code | ((rung + 1) << 10) : code | (rung << 10) | ((bit + 1) << 12);
+                put4(chk_list + 4 * q, slot, blk, nz, 0);
                 chk_pre[q] = bc + ec;
                 unsat[slot] = 0;
             } else {
@@ -269,7 +282,7 @@ __device__ __forceinline__ void check_phase(const BatchArgs &b, int *s_k) {
         const int p0 = uni(ldw(pre + k)), p1 = uni(ldw(pre + k + 1));
         const int end = p1 < i1 ? p1 : i1;
         const QC_CONST CodeTab *ct = b.codes + (cz & 255);
-        const QcArgs a = qc_of(ct, b.alpha);
+        const QcArgs a = qc_of(ct, b.ladder[(cz >> 10) & 3]);
         const int bit = cz >> 12;
         const bool first = bit == 0, update = bit < b.iters;
         const float *tot = b.tot_ws + (size_t)slot * b.tot_stride;
@@ -368,11 +381,16 @@ __device__ __forceinline__ void var_phase(const BatchArgs &b, int *s_k) {
         const long long off = b.blk_off[blk];
         const float *lam = b.llr + off;
         float *tot = b.tot_ws + (size_t)slot * b.tot_stride;
-        if (mode == M_RETIRE) {
-            if (i == p0 && p == 0) {
-                b.conv[blk] = (uint8_t)(u == 0);
+        if (mode == M_RETIRE || mode == M_RETIRE_INIT) {
+            // the rung's result, when taken: bits, conv, its, bad; a failed
+            // rung with a next one then restarts the slot (tot = lam) -- each
+            // lane reads its position's tot before overwriting it
+            const bool take = (u >> 30) & 1;
+            const int ucnt = u & ((1 << 30) - 1);
+            if (take && i == p0 && p == 0) {
+                b.conv[blk] = (uint8_t)(ucnt == 0);
                 b.its[blk] = bit;
-                b.bad[blk] = u;
+                b.bad[blk] = ucnt;
             }
             uint8_t *bout = b.bits + off;
             for (; i < end; ++i) {
@@ -381,7 +399,8 @@ __device__ __forceinline__ void var_phase(const BatchArgs &b, int *s_k) {
                 if (p < ATSC3_Z)
                     for (int gi = 0; gi < G; ++gi) {
                         const int t = (a.vg_info[start + gi] >> 24) & 255;
-                        bout[t * ATSC3_Z + p] = (uint8_t)(tot[t * QC_ROW + p] < 0.0f);
+                        if (take) bout[t * ATSC3_Z + p] = (uint8_t)(tot[t * QC_ROW + p] < 0.0f);
+                        if (mode == M_RETIRE_INIT) tot[t * QC_ROW + p] = lam[t * ATSC3_Z + p];
                     }
             }
             continue;
@@ -434,6 +453,7 @@ __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
     int *h = b.ctl;
     for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) h[k] = 0;
     for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG) b.ctl[b.free_list + k] = b.slots - 1 - k;
+    for (int k = threadIdx.x; k < b.nblk; k += ATSC3_WG) b.ctl[b.best + k] = 0x7fffffff;
     if (threadIdx.x == 0) h[H_NFREE] = b.slots;
     __syncthreads();
     compact(b);

@@ -269,9 +269,14 @@ def ladder_dev(llr, code, ladder=(1.0, 0.85, 0.75), iters=50, msg_dtype="float32
                engine="auto"):
     """Alpha ladder on device tensors; returns device (bits, conv, its, bad).
 
-    One host sync per rung (the number of rows still failing)."""
+    Engines "auto" and "batch" (up to 4 rungs) run the whole ladder inside
+    the batched persistent kernel: no host sync.  Otherwise one decode per
+    rung and one host sync per rung (the number of rows still failing)."""
     nb = llr.shape[0]
     dev = llr.device
+    if engine in ("auto", "batch") and len(ladder) <= 4:
+        from .batch import decode_batch_dev
+        return decode_batch_dev(llr, code, iters, ladder[0], msg_dtype, ladder=ladder)
     bits = torch.empty((nb, code.n), dtype=torch.uint8, device=dev)
     conv = torch.zeros(nb, dtype=torch.uint8, device=dev)
     its = torch.zeros(nb, dtype=torch.int32, device=dev)

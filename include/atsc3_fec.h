@@ -135,6 +135,12 @@ int32_t atsc3_minsum_qc_phased_dev(const float *llr, int32_t nblk, int32_t n,
  *             in bits (device; the rows may have different lengths)
  *   max_nvg, max_nbe, max_vdeg  maxima over the codes (variable groups,
  *             base edges, variable degree <= 32)
+ *   ladder    NULL: one decode with alpha.  Else host array of nrungs
+ *             (1..4) alphas, the alpha ladder of lab/m9_fast.py:843-891 run
+ *             inside the kernel: a codeword failing rung r restarts from its
+ *             LLRs at rung r+1 in the same slot; a rung's result (bits,
+ *             conv, its, bad) is kept when it converged or has fewer
+ *             unsatisfied checks than every earlier rung
  *   msg_fmt   0 FP32 messages (bit-exact with the reference), 1 FP16, 2 INT8
  *             (the storage codecs of atsc3_minsum_qc_dev)
  *   slots     codewords decoded concurrently; the rest wait in the queue
@@ -160,7 +166,8 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
                                const void *codes, int32_t ncode,
                                int32_t max_nvg, int32_t max_nbe,
                                int32_t max_vdeg, int32_t iters,
-                               float alpha, int32_t msg_fmt,
+                               float alpha, const float *ladder,
+                               int32_t nrungs, int32_t msg_fmt,
                                const float *msg_scale, int32_t slots,
                                int32_t grid, float *tot_ws, void *msg_ws,
                                int32_t *ctl_ws, uint8_t *bits, uint8_t *conv,

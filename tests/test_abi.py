@@ -16,6 +16,33 @@ def declared():
     return sorted(set(re.findall(r"\bint32_t\s+(\w+)\s*\(", txt)))
 
 
+def prototypes():
+    """name -> list of parameter type strings, from the header."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\bint32_t\s+(\w+)\s*\(([^)]*)\)\s*;", txt):
+        params = " ".join(params.split())
+        if params in ("", "void"):
+            out[name] = []
+            continue
+        types = []
+        for p in params.split(","):
+            p = p.strip()
+            m = re.match(r"(.*?)(\w+)$", p)
+            types.append(" ".join(m.group(1).replace("*", " * ").split()))
+        out[name] = types
+    return out
+
+
+def _ctype_of(t):
+    import ctypes
+    if "*" in t:
+        return ctypes.c_void_p
+    return {"int32_t": ctypes.c_int32, "int64_t": ctypes.c_int64, "float": ctypes.c_float,
+            "double": ctypes.c_double}[t.replace("const ", "")]
+
+
 def _lib():
     from atsc3_amd import _lib
     if not os.path.exists(_lib.LIB_PATH):
@@ -39,6 +66,17 @@ def test_library_exports_every_declared_symbol():
 def test_ctypes_layer_binds_the_header():
     _l = _lib()
     assert sorted(_l.exported_symbols()) == declared()
+
+
+def test_ctypes_signatures_match_prototypes():
+    """Every bound signature has the header's parameter count and kinds
+    (a short list would let ctypes pass a 64-bit handle as a C int)."""
+    from atsc3_amd import _lib as L
+    protos = prototypes()
+    assert set(protos) == set(L._SIGS)
+    for name, args in L._SIGS.items():
+        want = [_ctype_of(t) for t in protos[name]]
+        assert list(args) == want, name
 
 
 def test_size_queries():

@@ -36,10 +36,11 @@ def _mixed_llrs(nblk, seed, spread=0.6):
     return codes, llrs
 
 
-def _batch(codes, llrs, device, iters, alpha, msg="float32", slots=None, grid=None):
+def _batch(codes, llrs, device, iters, alpha, msg="float32", slots=None, grid=None,
+           ladder=None):
     import torch
     from atsc3_amd.batch import BatchPlan
-    plan = BatchPlan(codes, device, iters, alpha, msg, slots, grid)
+    plan = BatchPlan(codes, device, iters, alpha, msg, slots, grid, ladder=ladder)
     for b, l in enumerate(llrs):
         plan.llr_view(b).copy_(torch.from_numpy(np.ascontiguousarray(l, np.float32)))
     plan.run()
@@ -147,3 +148,83 @@ def test_grid_beyond_capacity_rejected(device):
     assert cap >= 256
     with pytest.raises(ValueError):
         BatchPlan([get_code("10/15", 64800)], device, grid=cap + 1)
+
+
+# ---------------------------------------------------------------------------
+# alpha ladder inside the kernel (lab/m9_fast.py:843-891)
+# ---------------------------------------------------------------------------
+
+LADDER = (1.0, 0.85, 0.75)
+
+
+def _ladder_case():
+    """24 blocks of the 16200-bit 8/15 code at 1.6 dB: with this ladder some
+    converge at each rung, some never (and some of those take a later rung's
+    fewer unsatisfied checks)."""
+    code = get_code("8/15", 16200)
+    cw = synth.codewords("8/15", 16200, 24, 204)
+    return code, synth.llr_bpsk_like(cw, 1.6, 205)
+
+
+@pytest.mark.parametrize("slots,grid", [(None, None), (3, 16)])
+def test_ladder_in_kernel_vs_oracle(device, slots, grid):
+    code, llr = _ladder_case()
+    want = OL.ladder(llr, code, LADDER, 50)
+    r1 = OL.minsum_f32(llr, code, 50, LADDER[0])
+    r2 = OL.minsum_f32(llr, code, 50, LADDER[1])
+    # the case exercises every branch of the ladder: codewords converging
+    # only at rung 2, only at rung 3, never (some of those keeping a later
+    # rung's result)
+    assert r1[1].sum() < want[1].sum() < len(llr)
+    assert (want[1] & ~r1[1] & r2[1]).any() and (want[1] & ~r1[1] & ~r2[1]).any()
+    assert ((~want[1]) & (want[3] < r1[3])).any()
+    print(f"[gpu] ladder in kernel slots={slots} grid={grid}", flush=True)
+    got, _ = _batch([code] * len(llr), list(llr), device, 50, LADDER[0], slots=slots,
+                    grid=grid, ladder=LADDER)
+    for b, g in enumerate(got):
+        _assert_block(g, (want[0][b], want[1][b], want[2][b], want[3][b]), f"blk {b}")
+
+
+def test_ladder_mixed_codes_vs_oracle(device):
+    """A mixed batch under the ladder: per codeword the oracle ladder of its
+    own code."""
+    code_a, llr_a = _ladder_case()
+    code_b = get_code("6/15", 64800)
+    llr_b = synth.llr_bpsk_like(synth.codewords("6/15", 64800, 3, 31), 0.5, 32)
+    codes = [code_a] * 6 + [code_b] * 3
+    llrs = list(llr_a[:6]) + list(llr_b)
+    wa = OL.ladder(llr_a[:6], code_a, LADDER, 50)
+    wb = OL.ladder(llr_b, code_b, LADDER, 50)
+    print("[gpu] ladder mixed codes", flush=True)
+    got, _ = _batch(codes, llrs, device, 50, LADDER[0], slots=4, ladder=LADDER)
+    for b, g in enumerate(got):
+        w, k = (wa, b) if b < 6 else (wb, b - 6)
+        _assert_block(g, (w[0][k], w[1][k], w[2][k], w[3][k]), f"blk {b}")
+
+
+@pytest.mark.parametrize("msg", ["float16", "int8"])
+def test_ladder_low_precision_matches_host_ladder(device, msg):
+    """FP16 / INT8 storage: the in-kernel ladder equals one decode per rung
+    on the per-code engine with the selection done on the host."""
+    import torch
+    from atsc3_amd import ldpc
+    code, llr = _ladder_case()
+    t = torch.from_numpy(np.ascontiguousarray(llr, np.float32)).to(device)
+    print(f"[gpu] ladder {msg}", flush=True)
+    a = ldpc.ladder_dev(t, code, LADDER, 50, msg, engine="batch")
+    b = ldpc.ladder_dev(t, code, LADDER, 50, msg, engine="qc")
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_single_rung_ladder_is_plain_decode(device):
+    import torch
+    from atsc3_amd import ldpc
+    code, llr = _ladder_case()
+    t = torch.from_numpy(np.ascontiguousarray(llr, np.float32)).to(device)
+    a = ldpc.ladder_dev(t, code, (0.85,), 50, engine="batch")
+    b = ldpc.decode_dev(t, code, 50, 0.85, engine="qc")
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
