@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libatsc3fec.so")
 _lock = threading.Lock()
 _lib = None
 
-c_i32, c_f32, c_vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
+c_i32, c_f32, c_f64, c_vp = ctypes.c_int32, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
 
 # name -> argtypes (all pointers are passed as integers / c_void_p)
 _SIGS = {
@@ -65,6 +65,13 @@ _SIGS = {
                                  c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_baseband_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp],
+    "atsc3_sp_ctl_words": [c_i32],
+    "atsc3_sum_product_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp,
+                              c_vp, c_i32, c_f64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                              c_vp, c_vp],
+    "atsc3_sp_select_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp],
+    "atsc3_sp_merge_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                           c_vp, c_vp],
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],
@@ -107,7 +114,8 @@ def lib():
 
 SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
-                "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid")
+                "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
+                "atsc3_sp_ctl_words")
 
 
 def call(name, *args):

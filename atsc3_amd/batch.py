@@ -140,6 +140,13 @@ class BatchPlan:
             if self.msg_scale.numel() != self.nblk:
                 raise ValueError("msg_scale needs one value per codeword")
 
+    def bank_index(self, code):
+        """Index of ``code`` in this plan's code bank (its blk_code value)."""
+        for i, c in enumerate(self.bank.codes):
+            if c is code:
+                return i
+        raise KeyError("code not in this plan")
+
     def llr_view(self, b):
         return self.llr[int(self.off[b]):int(self.off[b + 1])]
 

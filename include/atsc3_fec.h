@@ -240,6 +240,61 @@ int32_t atsc3_baseband_dev(const uint8_t *bits, int32_t nblk, int32_t n,
                            uint8_t *ok_out, uint8_t *bytes_out, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Exact sum-product rescue of near misses (float64, log-tanh BP).
+ * atsc3_sum_product_dev replaces lab/m16_margin.py:325-390
+ * (sum_product_decode_batch; = lab/e58_weighted.py:134-187 with no
+ * deadline); atsc3_sp_select_dev + atsc3_sp_merge_dev replace the candidate
+ * policy and write-back of lab/m44_ldm.py:2527-2555 and
+ * lab/m9_fast.py:1126-1151, on the device (no host synchronisation).
+ *
+ * atsc3_sum_product_dev decodes call rows r < live (live = *nsel, or rows
+ * when nsel is NULL) of one code; row r reads LLR row s = sel[r] (or r when
+ * sel is NULL) at llr + row_off[s] (or s * n when row_off is NULL).
+ *   cvar     [dmax][nc] int32: variable of slot k of check thread i, n for
+ *            padding (slot order = the check's sorted variable list)
+ *   vptr     [n + 1], vedge [E] int32: per variable its message slots
+ *            k * nc + i, in ascending check index
+ *   iters    iteration limit (the last syndrome test is at iteration iters)
+ *   budget_s > 0: no iteration starts once this much wall-clock time has
+ *            passed since the call's first kernel began (the upstream's
+ *            deadline); rows still failing leave with shed = 1
+ *   tot_ws   rows * n doubles; msg_ws rows * dmax * nc doubles;
+ *   ctl_ws   atsc3_sp_ctl_words(rows) int32
+ *   bits     [rows][n] u8, conv/shed [rows] u8, its/bad [rows] int32;
+ *            rows past live get conv 0, bad -1
+ * dmax <= 128 (ATSC3_E_DEGREE otherwise).
+ *
+ * atsc3_sp_select_dev: rows_out[0 .. *nrows_out) = the rows b with
+ * conv[b] == 0 and bad[b] <= near_max (and blk_code[b] == code_id when
+ * blk_code is not NULL: one code of a mixed batch), by ascending bad then
+ * b, at most max_rows of them.
+ * atsc3_sp_merge_dev: for r < *nsel with sp_conv[r], copy sp_bits row r to
+ * bits row sel[r] (at row_off[sel[r]] or sel[r] * n) and set conv = 1,
+ * its = sp_its[r], bad = 0 there.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_sp_ctl_words(int32_t rows);
+int32_t atsc3_sum_product_dev(const float *llr, const int64_t *row_off,
+                              const int32_t *sel, const int32_t *nsel,
+                              int32_t rows, int32_t n, int32_t nc,
+                              int32_t dmax, const int32_t *cvar,
+                              const int32_t *vptr, const int32_t *vedge,
+                              int32_t iters, double budget_s, double *tot_ws,
+                              double *msg_ws, int32_t *ctl_ws, uint8_t *bits,
+                              uint8_t *conv, int32_t *its, int32_t *bad,
+                              uint8_t *shed, void *stream);
+int32_t atsc3_sp_select_dev(const uint8_t *conv, const int32_t *bad,
+                            const int32_t *blk_code, int32_t code_id,
+                            int32_t nblk, int32_t near_max, int32_t max_rows,
+                            int32_t *rows_out, int32_t *nrows_out,
+                            void *stream);
+int32_t atsc3_sp_merge_dev(const int32_t *sel, const int32_t *nsel,
+                           int32_t rows, int32_t n, const int64_t *row_off,
+                           const uint8_t *sp_bits, const uint8_t *sp_conv,
+                           const int32_t *sp_its, uint8_t *bits,
+                           uint8_t *conv, int32_t *its, int32_t *bad,
+                           void *stream);
+
+/* ------------------------------------------------------------------------
  * Differentiable decoding (training; no upstream counterpart for the
  * backward -- the forward update is that of lab/ldpc_kernel.c:82-143 without
  * the syndrome stop, on the CSR tables of lab/m3_ldpc.py:459-482).

@@ -12,6 +12,9 @@ Contents:
   demap.py                   FP64 max-log demapper (lab/m6_bicm.py:213-225),
                              the sigma^2 estimator, CSI weighting and the LDM
                              Gaussian-folded core demap (lab/m15_ldm_llr.py:77-85).
+  sp_ref.py                  float64 log-tanh sum-product (lab/m16_margin.py:
+                             325-390) and the near-miss rescue policy
+                             (lab/m44_ldm.py:2527-2555).
   Makefile                   builds liboracle_minsum.so and, where the upstream
                              tree is present, _ref/ldpc_kernel.so from the
                              upstream's own source file.
