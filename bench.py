@@ -269,6 +269,73 @@ def learn_bench(args, dev, steps, eval_blocks, ebn0_points=(1.0, 1.5, 2.0, 2.5, 
             "fer_block_errors": fer}
 
 
+def rescue_bench(args, dev, rate="10/15", snr=2.8, nblk=64, max_blocks=8, iters=100):
+    """Near-threshold batch of one 64800-bit code: the alpha ladder inside the
+    batched kernel (against one host-synchronised decode per rung), then the
+    float64 sum-product rescue of the closest near misses (select, decode,
+    write-back on the device), beside the numpy decoder of the upstream's
+    statement on the host."""
+    import torch
+    from atsc3_amd import ldpc, synth
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.rescue import rescue_dev
+    from oracle import sp_ref
+    code = get_code(rate, 64800)
+    cw = synth.codewords(rate, 64800, nblk, args.seed + 5)
+    llr_h = synth.llr_bpsk_like(cw, snr, args.seed + 6)
+    llr = torch.from_numpy(llr_h).to(dev)
+    cur = torch.cuda.current_stream(dev)
+    ladder = (1.0, 0.85, 0.75)
+    ms = {}
+    for eng in ("batch", "qc"):
+        ldpc.ladder_dev(llr, code, ladder, 50, engine=eng)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lad = ldpc.ladder_dev(llr, code, ladder, 50, engine=eng)
+        torch.cuda.synchronize()
+        ms[eng] = (time.perf_counter() - t0) * 1e3 / args.steps
+    bits0, conv0, its0, bad0 = ldpc.ladder_dev(llr, code, ladder, 50, engine="batch")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for step in range(args.warmup + args.steps):
+        bits, conv, its, bad = (x.clone() for x in (bits0, conv0, its0, bad0))
+        e0.record(cur)
+        rows, count, c2, s2 = rescue_dev(llr, bits, conv, its, bad, code, 1500, max_blocks,
+                                         iters, budget_s=None)
+        e1.record(cur)
+        torch.cuda.synchronize()
+        if step >= args.warmup:
+            times.append(e0.elapsed_time(e1))
+    r_ms = float(np.median(times))
+    offered = int(count.item())
+    rescued = int(c2[:offered].sum().item())
+    conv_h = conv.cpu().numpy().astype(bool)
+    wrong = int(sum(not np.array_equal(bits[b].cpu().numpy(), cw[b])
+                    for b in np.flatnonzero(conv_h)))
+    its_h = its.cpu().numpy()
+    sp_its = int(its_h[rows.cpu().numpy()[:offered]].max()) if offered else 0
+    # the upstream's numpy decoder (restated by the oracle) on one block, a
+    # bounded number of iterations
+    k = 3
+    t0 = time.perf_counter()
+    sp_ref.sum_product(llr_h[:1], code.checks, 64800, k)
+    cpu_ms_blk_it = (time.perf_counter() - t0) * 1e3 / (k + 1)
+    out = {"code": f"{rate} N=64800", "snr_db": snr, "blocks": nblk,
+           "ladder_ms": {"in_kernel": round(ms["batch"], 3), "host_per_rung": round(ms["qc"], 3)},
+           "ladder_converged": int(conv0.sum().item()),
+           "rescue_offered": offered, "rescue_converged": rescued, "rescue_iters_max": sp_its,
+           "converged_after": int(conv_h.sum()), "wrong_codewords": wrong,
+           "rescue_ms": round(r_ms, 3), "rescue_iters_limit": iters,
+           "cpu_numpy_ms_per_block_iteration": round(cpu_ms_blk_it, 1),
+           "cpu_sample": f"float64 numpy sum-product, 1 block x {k} iterations, 1 core"}
+    log(f"[bench] rescue: ladder {ms['batch']:.2f} ms in kernel vs {ms['qc']:.2f} ms per-rung; "
+        f"{int(conv0.sum().item())}/{nblk} converged, {offered} offered, {rescued} rescued in "
+        f"{r_ms:.2f} ms (<= {sp_its} its); numpy {cpu_ms_blk_it:.1f} ms per block-iteration")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +365,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the uneven-frame summary in the live-frame line")
     ap.add_argument("--ldm-frames", type=int, default=50)
+    ap.add_argument("--no-rescue", action="store_true",
+                    help="skip the ladder + sum-product rescue summary")
     ap.add_argument("--ldm-esn0", type=float, default=LDM_ESN0)
     ap.add_argument("--no-ldm", action="store_true",
                     help="skip the LDM summary in the live-frame line")
@@ -575,6 +644,10 @@ def main():
     if not args.no_ldm:
         torch.cuda.empty_cache()
         ldm = ldm_bench(args, dev)
+    rescue = None
+    if not args.no_rescue:
+        torch.cuda.empty_cache()
+        rescue = rescue_bench(args, dev)
 
     info_gbps = value * code.K / 1e9
     rt_factor = value / (FRAME_CELLS / ncell / FRAME_SEC)
@@ -594,7 +667,8 @@ def main():
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
                    "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
-                   "forms": forms, "mixed": mixed, "ldm": ldm, "learn": learn},
+                   "forms": forms, "mixed": mixed, "ldm": ldm, "learn": learn,
+                   "rescue": rescue},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
