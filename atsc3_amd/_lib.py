@@ -65,6 +65,8 @@ _SIGS = {
                                  c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_baseband_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp],
+    "atsc3_demap_gather_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp,
+                               c_vp, c_vp, c_vp],
     "atsc3_sp_ctl_words": [c_i32],
     "atsc3_sum_product_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp,
                               c_vp, c_i32, c_f64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

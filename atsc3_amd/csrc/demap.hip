@@ -23,6 +23,18 @@
 // Output position of q-stream bit q = cell*nb + i is lam_of_q[q]
 // (lab/m6_bicm.py:331-335), so the LLRs land in codeword order.
 //
+// Cell gather (atsc3_demap_gather_dev): cell s of block b is read from a
+// pool through an index map instead of cells[b * ncell + s], which fuses the
+// time de-interleaver's FEC-block gather into the demap read:
+//   * TABLE: pool[idx[b * ncell + s]] -- the hybrid time interleaver's
+//     twisted-block map (A/322 7.1.5.4, virtual FEC Blocks included) as the
+//     upstream gathers it (lab/m9_fast.py:1087-1093);
+//   * CTI: output cell i = first + b * ncell + s of the convolutional time
+//     de-interleaver reads received cell q = i + Nrows * ((start_row + i) mod
+//     Nrows) (A/322 9.3.9.1; lab/m10_cti.py:108-146), held in a ring buffer
+//     of `ring` cells at q mod ring (lab/m44_ldm.py:1289-1304).
+// The CSI weight of a cell is read through the same index.
+//
 // sigma2 == NULL: per block sigma2 = max(mean(w * d2min), 1e-9) with d2min
 // = min_p |z - p|^2 (lab/m6_bicm.py:219-220), reduced in FP64 in a fixed
 // order (per-workgroup tree, then sequential over workgroups), then a second
@@ -33,11 +45,33 @@
 
 namespace atsc3 {
 
-template <int NB>
+enum { GATHER_NONE = 0, GATHER_TABLE = 1, GATHER_CTI = 2 };
+
+struct Gather {
+    const int32_t *idx;
+    long long first, ring;
+    int nrows, start_row;
+};
+
+template <int GM>
+__device__ __forceinline__ size_t cell_of(const Gather &g, int blk, int ncell, int s) {
+    if constexpr (GM == GATHER_TABLE) {
+        return (size_t)(uint32_t)g.idx[(size_t)blk * ncell + s];
+    } else if constexpr (GM == GATHER_CTI) {
+        const long long i = g.first + (long long)blk * ncell + s;
+        const long long q = i + (long long)g.nrows * ((g.start_row + i) % g.nrows);
+        return (size_t)(q % g.ring);
+    } else {
+        return (size_t)blk * ncell + s;
+    }
+}
+
+template <int NB, int GM>
 __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
     const float2 *__restrict__ cells, const float *__restrict__ csi,
     const double *__restrict__ sigma2, int ncell, const float2 *__restrict__ base,
-    const int *__restrict__ lam_of_q, float *__restrict__ llr, double *__restrict__ partial) {
+    const int *__restrict__ lam_of_q, float *__restrict__ llr, double *__restrict__ partial,
+    Gather g) {
     constexpr int NW = 1 << (NB - 2);
     __shared__ float2 w[NW];
     __shared__ double red[DEMAP_WG / 64];
@@ -46,11 +80,11 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
     for (int k = tid; k < NW; k += DEMAP_WG) w[k] = base[k];
     __syncthreads();
     const int s = blockIdx.x * DEMAP_WG + tid;
-    const size_t cbase = (size_t)blk * ncell;
     const int N = ncell * NB;
     double d2w = 0.0;
     if (s < ncell) {
-        const float2 z = cells[cbase + s];
+        const size_t ci = cell_of<GM>(g, blk, ncell, s);
+        const float2 z = cells[ci];
         const float ar = fabsf(z.x), ai = fabsf(z.y);
         float dmin = INFINITY, opp_im = INFINITY, opp_re = INFINITY;
         float m0[NB > 2 ? NB - 2 : 1], m1[NB > 2 ? NB - 2 : 1];
@@ -74,7 +108,7 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
                     m0[i - 2] = fminf(m0[i - 2], ac);
             }
         }
-        const float wgt = csi ? csi[cbase + s] : 1.0f;
+        const float wgt = csi ? csi[ci] : 1.0f;
         float diff[NB];
         // y0: imaginary sign; y1: real sign (see header)
         diff[0] = z.y >= 0.0f ? opp_im - dmin : dmin - opp_im;
@@ -131,14 +165,40 @@ __global__ void copy_sigma2_kernel(const double *src, double *dst, int nblk) {
     if (i < nblk) dst[i] = src[i];
 }
 
-template <int NB>
+template <int NB, int GM>
 static void launch_demap(const float *cells, const float *csi, const double *sigma2,
                          int nblk, int ncell, const float *points,
-                         const int32_t *lam_of_q, float *llr, double *ws, hipStream_t st) {
+                         const int32_t *lam_of_q, float *llr, double *ws, const Gather &g,
+                         hipStream_t st) {
     const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
-    hipLaunchKernelGGL((demap_kernel<NB>), dim3(nwg, nblk), dim3(DEMAP_WG), 0, st,
+    hipLaunchKernelGGL((demap_kernel<NB, GM>), dim3(nwg, nblk), dim3(DEMAP_WG), 0, st,
                        reinterpret_cast<const float2 *>(cells), csi, sigma2, ncell,
-                       reinterpret_cast<const float2 *>(points), lam_of_q, llr, ws);
+                       reinterpret_cast<const float2 *>(points), lam_of_q, llr, ws, g);
+}
+
+template <int GM>
+static int32_t demap_any(const float *cells, const float *csi, const double *sigma2,
+                         int nblk, int ncell, const float *points, int nbits,
+                         const int32_t *lam_of_q, float *llr_out, double *sigma2_out,
+                         double *ws, const Gather &g, hipStream_t st) {
+    // `points` holds the M/4 first-quadrant base points (host-checked).
+    switch (nbits) {
+        case 2: launch_demap<2, GM>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, g, st); break;
+        case 4: launch_demap<4, GM>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, g, st); break;
+        case 6: launch_demap<6, GM>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, g, st); break;
+        case 8: launch_demap<8, GM>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, g, st); break;
+        default: return ATSC3_E_ARG;
+    }
+    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    if (!sigma2) {
+        const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
+        hipLaunchKernelGGL(demap_scale_kernel, dim3(nblk), dim3(DEMAP_WG), 0, st, llr_out,
+                           ncell * nbits, ncell, ws, nwg, sigma2_out);
+    } else if (sigma2_out) {
+        hipLaunchKernelGGL(copy_sigma2_kernel, dim3((nblk + 255) / 256), dim3(256), 0, st,
+                           sigma2, sigma2_out, nblk);
+    }
+    return launch_status();
 }
 
 }  // namespace atsc3
@@ -157,23 +217,37 @@ extern "C" int32_t atsc3_demap_dev(const float *cells, const float *csi,
     if (nblk == 0 || ncell == 0) return ATSC3_OK;
     if (!cells || !points || !lam_of_q || !llr_out) return ATSC3_E_ARG;
     if (!sigma2 && !ws) return ATSC3_E_ARG;
+    const atsc3::Gather g{nullptr, 0, 0, 0, 0};
+    return atsc3::demap_any<atsc3::GATHER_NONE>(cells, csi, sigma2, nblk, ncell, points, nbits,
+                                                lam_of_q, llr_out, sigma2_out, ws, g,
+                                                atsc3::as_stream(stream));
+}
+
+extern "C" int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_csi,
+                                          const atsc3_gather *gather, const double *sigma2,
+                                          int32_t nblk, int32_t ncell, const float *points,
+                                          int32_t nbits, const int32_t *lam_of_q,
+                                          float *llr_out, double *sigma2_out, double *ws,
+                                          void *stream) {
+    if (nblk < 0 || ncell < 0 || !gather) return ATSC3_E_ARG;
+    if (nblk == 0 || ncell == 0) return ATSC3_OK;
+    if (!pool || !points || !lam_of_q || !llr_out) return ATSC3_E_ARG;
+    if (!sigma2 && !ws) return ATSC3_E_ARG;
+    atsc3::Gather g{gather->idx, gather->first, gather->ring, gather->nrows, gather->start_row};
     hipStream_t st = atsc3::as_stream(stream);
-    // `points` holds the M/4 first-quadrant base points (host-checked).
-    switch (nbits) {
-        case 2: atsc3::launch_demap<2>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
-        case 4: atsc3::launch_demap<4>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
-        case 6: atsc3::launch_demap<6>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
-        case 8: atsc3::launch_demap<8>(cells, csi, sigma2, nblk, ncell, points, lam_of_q, llr_out, ws, st); break;
-        default: return ATSC3_E_ARG;
+    if (gather->mode == ATSC3_GATHER_TABLE) {
+        if (!g.idx) return ATSC3_E_ARG;
+        return atsc3::demap_any<atsc3::GATHER_TABLE>(pool, pool_csi, sigma2, nblk, ncell, points,
+                                                     nbits, lam_of_q, llr_out, sigma2_out, ws, g,
+                                                     st);
     }
-    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
-    if (!sigma2) {
-        const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
-        hipLaunchKernelGGL(atsc3::demap_scale_kernel, dim3(nblk), dim3(DEMAP_WG), 0, st,
-                           llr_out, ncell * nbits, ncell, ws, nwg, sigma2_out);
-    } else if (sigma2_out) {
-        hipLaunchKernelGGL(atsc3::copy_sigma2_kernel, dim3((nblk + 255) / 256), dim3(256), 0,
-                           st, sigma2, sigma2_out, nblk);
+    if (gather->mode == ATSC3_GATHER_CTI) {
+        if (g.nrows <= 0 || g.ring <= 0 || g.first < 0 || g.start_row < 0 ||
+            g.start_row >= g.nrows)
+            return ATSC3_E_ARG;
+        return atsc3::demap_any<atsc3::GATHER_CTI>(pool, pool_csi, sigma2, nblk, ncell, points,
+                                                   nbits, lam_of_q, llr_out, sigma2_out, ws, g,
+                                                   st);
     }
-    return atsc3::launch_status();
+    return ATSC3_E_ARG;
 }

@@ -17,6 +17,7 @@ on the device.
 """
 from __future__ import annotations
 
+import ctypes
 import functools
 
 import numpy as np
@@ -81,6 +82,52 @@ def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=Non
     return llr, s2out
 
 
+class GatherSpec(ctypes.Structure):
+    """struct atsc3_gather (include/atsc3_fec.h)."""
+    _fields_ = [("mode", ctypes.c_int32), ("nrows", ctypes.c_int32),
+                ("start_row", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("idx", ctypes.c_void_p), ("first", ctypes.c_int64), ("ring", ctypes.c_int64)]
+
+
+GATHER_TABLE, GATHER_CTI = 1, 2
+
+
+def demap_gather_q_dev(pool, pts_dev, nb, lam_of_q_dev, spec, nblk, ncell, csi=None,
+                       sigma2=None, out=None):
+    """Demap ``nblk`` blocks of ``ncell`` cells read from ``pool`` (1-D
+    complex64 device tensor) through ``spec`` (a GatherSpec)."""
+    if not (isinstance(pool, torch.Tensor) and pool.is_cuda and pool.dtype == torch.complex64
+            and pool.dim() == 1 and pool.is_contiguous()):
+        raise TypeError("pool must be a contiguous 1-D complex64 device tensor")
+    dev = pool.device
+    N = ncell * nb
+    llr = out if out is not None else torch.empty((nblk, N), dtype=torch.float32, device=dev)
+    s2out = torch.empty(nblk, dtype=torch.float64, device=dev)
+    if nblk == 0 or ncell == 0:
+        return llr, s2out
+    csi_p = None
+    if csi is not None:
+        if not (csi.is_cuda and csi.dtype == torch.float32 and csi.shape == pool.shape
+                and csi.is_contiguous()):
+            raise ValueError("csi must be a float32 device tensor shaped like the pool")
+        csi_p = csi.data_ptr()
+    s2_p, ws, ws_p = None, None, None
+    if sigma2 is not None:
+        sigma2 = torch.as_tensor(sigma2, dtype=torch.float64, device=dev).reshape(-1)
+        if sigma2.numel() == 1 and nblk > 1:
+            sigma2 = sigma2.expand(nblk)
+        sigma2 = sigma2.contiguous()
+        s2_p = sigma2.data_ptr()
+    else:
+        nws = _lib.lib().atsc3_demap_ws_doubles(nblk, ncell)
+        ws = torch.empty(max(nws, 1), dtype=torch.float64, device=dev)
+        ws_p = ws.data_ptr()
+    _lib.call("atsc3_demap_gather_dev", torch.view_as_real(pool).data_ptr(), csi_p,
+              ctypes.byref(spec), s2_p, nblk, ncell, pts_dev.data_ptr(), nb,
+              lam_of_q_dev.data_ptr(), llr.data_ptr(), s2out.data_ptr(), ws_p, _stream(dev))
+    return llr, s2out
+
+
 @functools.lru_cache(maxsize=None)
 def _points_dev(key, device):
     mod, rate, scale = key
@@ -119,6 +166,13 @@ class Demapper:
         if cells.shape[1] != self.ncell:
             raise ValueError(f"{cells.shape[1]} cells per block, expected {self.ncell}")
         return demap_q_dev(cells, self.points, self.nb, self.lam_of_q, csi, sigma2, out)
+
+    def demap_gather_dev(self, pool, spec, nblk, csi=None, sigma2=None, out=None):
+        """Blocks read from ``pool`` through a GatherSpec (the time
+        de-interleaver's FEC-block gather) -> same as demap_dev on the
+        gathered cells."""
+        return demap_gather_q_dev(pool, self.points, self.nb, self.lam_of_q, spec, nblk,
+                                  self.ncell, csi, sigma2, out)
 
 
 def demap_batch(cells2d, pts, sigma2=None, csi=None, device="cuda"):

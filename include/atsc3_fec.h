@@ -200,6 +200,35 @@ int32_t atsc3_demap_dev(const float *cells, const float *csi,
                         double *sigma2_out, double *ws, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Demap with the time de-interleaver's FEC-block gather fused into the read.
+ * Replaces the FEC-block gathers in front of the demap calls: the hybrid
+ * (twisted block) time de-interleave lab/m6_tbi.py:47-96 as batched in
+ * lab/m9_fast.py:1087-1093 (cells = plp0[hidx]), and the convolutional time
+ * de-interleave lab/m10_cti.py:108-146 as streamed by CtiStream.take_blocks
+ * (lab/m44_ldm.py:1289-1304).  Same outputs as atsc3_demap_dev on the
+ * gathered cells; cell s of block b (and its CSI weight) is read from
+ *   ATSC3_GATHER_TABLE  pool[idx[b * ncell + s]]
+ *   ATSC3_GATHER_CTI    pool[q mod ring], q = i + nrows * ((start_row + i)
+ *                       mod nrows), i = first + b * ncell + s
+ *   pool, pool_csi      complex64 / f32 cells of the frame, TI block or ring
+ * ------------------------------------------------------------------------ */
+#define ATSC3_GATHER_TABLE 1
+#define ATSC3_GATHER_CTI 2
+typedef struct atsc3_gather {
+    int32_t mode, nrows, start_row, reserved;
+    const int32_t *idx;
+    int64_t first, ring;
+} atsc3_gather;
+
+int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_csi,
+                               const atsc3_gather *gather,
+                               const double *sigma2, int32_t nblk,
+                               int32_t ncell, const float *points,
+                               int32_t nbits, const int32_t *lam_of_q,
+                               float *llr_out, double *sigma2_out, double *ws,
+                               void *stream);
+
+/* ------------------------------------------------------------------------
  * LDM successive interference cancellation.
  * Replaces the core-layer cancellation of lab/e69_enh.py:86-121: for each
  * block with conv[b] = 1 the symbols s are rebuilt from the decoded codeword

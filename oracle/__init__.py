@@ -15,6 +15,9 @@ Contents:
   sp_ref.py                  float64 log-tanh sum-product (lab/m16_margin.py:
                              325-390) and the near-miss rescue policy
                              (lab/m44_ldm.py:2527-2555).
+  ti.py                      twisted-block (hybrid) and convolutional time
+                             de-interleavers (lab/m6_tbi.py:47-96,
+                             lab/m10_cti.py:108-152, lab/m44_ldm.py:1187-1304).
   Makefile                   builds liboracle_minsum.so and, where the upstream
                              tree is present, _ref/ldpc_kernel.so from the
                              upstream's own source file.
