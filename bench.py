@@ -547,6 +547,26 @@ def main():
     e1.record(cur)
     torch.cuda.synchronize()
     baseband_ms = e0.elapsed_time(e1) / args.steps
+    # the same window read through the hybrid time de-interleaver's gather
+    # (the window's cells taken as the transmitted order of `frames` TI
+    # blocks): the gather fused into the demap read against the plain demap
+    from atsc3_amd.ti import HtiGather
+    ncols = nblk // args.frames
+    hti = HtiGather(args.frames * ncols * ncell, args.frames, ncell, ncols, 0, dev)
+    pool, pool_w = z.reshape(-1), w.reshape(-1)
+    demap_times = {}
+    for name, fn in (("plain", lambda: dm.demap_dev(z, csi=w, sigma2=n0)),
+                     ("hti_gather", lambda: hti.demap(dm, pool, pool_w, n0))):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(cur)
+        for _ in range(args.steps):
+            fn()
+        e1.record(cur)
+        torch.cuda.synchronize()
+        demap_times[name] = round(e0.elapsed_time(e1) / args.steps, 4)
+    log(f"[bench] demap {demap_times['plain']:.3f} ms, through the HTI gather "
+        f"{demap_times['hti_gather']:.3f} ms ({hti.nblk} blocks)")
     # plain FP32 (the bit-exact decoder) on the same resident window
     run_window(z, w, n0, "float32")
     torch.cuda.synchronize()
@@ -663,6 +683,7 @@ def main():
                    "esn0_db": op, "messages": args.msg, "accumulation": "float32",
                    "plain_fp32": plain, "iters_max": 50, "alpha": 0.75,
                    "baseband_ms": round(baseband_ms, 4),
+                   "demap_ms_plain_vs_hti_gather": demap_times,
                    "blocks_per_step": nblk, "mean_its": round(float(its.mean()), 3),
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
