@@ -67,6 +67,12 @@ _SIGS = {
                            c_vp],
     "atsc3_demap_gather_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp,
                                c_vp, c_vp, c_vp],
+    "atsc3_demap_weighted_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
+                                 c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_cpe_ws_doubles": [c_i32],
+    "atsc3_cpe_dev": [c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_i32,
+                      c_vp, c_vp, c_vp, c_vp],
+    "atsc3_dummy_snr_dev": [c_vp, c_i32, c_vp, c_vp, c_vp],
     "atsc3_sp_ctl_words": [c_i32],
     "atsc3_sum_product_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp,
                               c_vp, c_i32, c_f64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
@@ -117,7 +123,7 @@ def lib():
 SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
-                "atsc3_sp_ctl_words")
+                "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles")
 
 
 def call(name, *args):

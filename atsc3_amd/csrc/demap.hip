@@ -71,7 +71,7 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
     const float2 *__restrict__ cells, const float *__restrict__ csi,
     const double *__restrict__ sigma2, int ncell, const float2 *__restrict__ base,
     const int *__restrict__ lam_of_q, float *__restrict__ llr, double *__restrict__ partial,
-    Gather g) {
+    Gather g, float *__restrict__ d2_out) {
     constexpr int NW = 1 << (NB - 2);
     __shared__ float2 w[NW];
     __shared__ double red[DEMAP_WG / 64];
@@ -125,6 +125,7 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
 #pragma unroll
             for (int i = 0; i < NB; ++i) out[lq[i]] = diff[i] * wgt;
             d2w = (double)wgt * (double)dmin;
+            if (d2_out) d2_out[(size_t)blk * ncell + s] = dmin;
         }
     }
     if (!sigma2) {
@@ -169,12 +170,55 @@ template <int NB, int GM>
 static void launch_demap(const float *cells, const float *csi, const double *sigma2,
                          int nblk, int ncell, const float *points,
                          const int32_t *lam_of_q, float *llr, double *ws, const Gather &g,
-                         hipStream_t st) {
+                         hipStream_t st, float *d2_out = nullptr) {
     const int nwg = (ncell + DEMAP_WG - 1) / DEMAP_WG;
     hipLaunchKernelGGL((demap_kernel<NB, GM>), dim3(nwg, nblk), dim3(DEMAP_WG), 0, st,
                        reinterpret_cast<const float2 *>(cells), csi, sigma2, ncell,
-                       reinterpret_cast<const float2 *>(points), lam_of_q, llr, ws, g);
+                       reinterpret_cast<const float2 *>(points), lam_of_q, llr, ws, g, d2_out);
 }
+
+// s2[p] = max(mean of d2 over the cells of part p, 1e-6) as float; the cells
+// of part p are order[seg[p] .. seg[p+1]).  One workgroup per part.
+__global__ __launch_bounds__(DEMAP_WG) void part_noise(const float *__restrict__ d2,
+                                                     const int32_t *__restrict__ order,
+                                                     const int32_t *__restrict__ seg,
+                                                     float *__restrict__ s2) {
+    __shared__ double red[DEMAP_WG / 64];
+    const int p = blockIdx.x;
+    const int lo = seg[p], hi = seg[p + 1];
+    double s = 0.0;
+    for (int k = lo + (int)threadIdx.x; k < hi; k += DEMAP_WG) s += (double)d2[order[k]];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s = 0.0;
+        for (int w = 0; w < DEMAP_WG / 64; ++w) s += red[w];
+        const int cnt = hi - lo > 1 ? hi - lo : 1;
+        const double v = s / (double)cnt;
+        s2[p] = (float)(v > 1e-6 ? v : 1e-6);
+    }
+}
+
+// LLRs of block cell (b, s) (codeword positions lam_of_q[s * nb + i]) divided
+// by the noise variance of the cell's part.
+__global__ __launch_bounds__(DEMAP_WG) void part_rescale(float *__restrict__ llr, int nblk,
+                                                       int ncell, int nb,
+                                                       const int32_t *__restrict__ lam_of_q,
+                                                       const int32_t *__restrict__ part,
+                                                       const float *__restrict__ s2) {
+    const int b = blockIdx.y;
+    const int s = blockIdx.x * DEMAP_WG + threadIdx.x;
+    if (b >= nblk || s >= ncell) return;
+    const float v = s2[part[(size_t)b * ncell + s]];
+    float *row = llr + (size_t)b * ncell * nb;
+    for (int i = 0; i < nb; ++i) {
+        float &x = row[lam_of_q[s * nb + i]];
+        x = x / v;
+    }
+}
+
+
 
 template <int GM>
 static int32_t demap_any(const float *cells, const float *csi, const double *sigma2,
@@ -250,4 +294,55 @@ extern "C" int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_c
                                                    st);
     }
     return ATSC3_E_ARG;
+}
+
+extern "C" int32_t atsc3_demap_weighted_dev(
+    const float *pool, const float *pool_csi, const atsc3_gather *gather, int32_t nblk,
+    int32_t ncell, const float *points, int32_t nbits, const int32_t *lam_of_q,
+    const int32_t *part, const int32_t *order, const int32_t *seg, int32_t nparts,
+    float *llr_out, float *s2_out, float *d2_ws, double *ws, void *stream) {
+    if (nblk < 0 || ncell < 0 || nparts < 0) return ATSC3_E_ARG;
+    if (nblk == 0 || ncell == 0) return ATSC3_OK;
+    if (!pool || !points || !lam_of_q || !part || !order || !seg || !llr_out || !s2_out ||
+        !d2_ws || !ws || nparts == 0)
+        return ATSC3_E_ARG;
+    atsc3::Gather g{nullptr, 0, 0, 0, 0};
+    int mode = atsc3::GATHER_NONE;
+    if (gather) {
+        g = atsc3::Gather{gather->idx, gather->first, gather->ring, gather->nrows,
+                          gather->start_row};
+        mode = gather->mode;
+        if (mode == ATSC3_GATHER_TABLE && !g.idx) return ATSC3_E_ARG;
+        if (mode == ATSC3_GATHER_CTI && (g.nrows <= 0 || g.ring <= 0 || g.first < 0 ||
+                                         g.start_row < 0 || g.start_row >= g.nrows))
+            return ATSC3_E_ARG;
+        if (mode != ATSC3_GATHER_TABLE && mode != ATSC3_GATHER_CTI) return ATSC3_E_ARG;
+    }
+    hipStream_t st = atsc3::as_stream(stream);
+#define W_LAUNCH(NB, GM)                                                                      \
+    atsc3::launch_demap<NB, GM>(pool, pool_csi, nullptr, nblk, ncell, points, lam_of_q,        \
+                                llr_out, ws, g, st, d2_ws)
+#define W_MODES(NB)                                                                           \
+    if (mode == atsc3::GATHER_TABLE)                                                          \
+        W_LAUNCH(NB, atsc3::GATHER_TABLE);                                                    \
+    else if (mode == atsc3::GATHER_CTI)                                                       \
+        W_LAUNCH(NB, atsc3::GATHER_CTI);                                                      \
+    else                                                                                      \
+        W_LAUNCH(NB, atsc3::GATHER_NONE);
+    switch (nbits) {
+        case 2: W_MODES(2) break;
+        case 4: W_MODES(4) break;
+        case 6: W_MODES(6) break;
+        case 8: W_MODES(8) break;
+        default: return ATSC3_E_ARG;
+    }
+#undef W_MODES
+#undef W_LAUNCH
+    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    hipLaunchKernelGGL(atsc3::part_noise, dim3(nparts), dim3(DEMAP_WG), 0, st, d2_ws, order, seg,
+                       s2_out);
+    hipLaunchKernelGGL(atsc3::part_rescale, dim3((ncell + DEMAP_WG - 1) / DEMAP_WG, nblk),
+                       dim3(DEMAP_WG), 0, st, llr_out, nblk, ncell, nbits, lam_of_q, part,
+                       s2_out);
+    return atsc3::launch_status();
 }

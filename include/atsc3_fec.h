@@ -229,6 +229,56 @@ int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_csi,
                                void *stream);
 
 /* ------------------------------------------------------------------------
+ * Weighted LLRs: per-part noise variance from the demap's own nearest-point
+ * distances.  Replaces the weighted-LLR path of lab/m9_fast.py:1096-1115
+ * (demap_batch with d2min_out, lab/m9_fast.py:756-822, then per pool part
+ * s2 = max(mean d2min, 1e-6) and q /= s2[part]).
+ *   pool/gather  as atsc3_demap_gather_dev; gather NULL: cells [nblk][ncell]
+ *   part         [nblk*ncell] part id of every block cell (0 .. nparts-1)
+ *   order, seg   the block cells of part p are order[seg[p] .. seg[p+1])
+ *   llr_out      [nblk][ncell*nbits] codeword order (lam_of_q), each cell's
+ *                raw max-log differences divided by its part's s2
+ *   s2_out       [nparts] float; d2_ws [nblk*ncell] float;
+ *   ws           atsc3_demap_ws_doubles(nblk, ncell) doubles
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_demap_weighted_dev(const float *pool, const float *pool_csi,
+                                 const atsc3_gather *gather, int32_t nblk,
+                                 int32_t ncell, const float *points,
+                                 int32_t nbits, const int32_t *lam_of_q,
+                                 const int32_t *part, const int32_t *order,
+                                 const int32_t *seg, int32_t nparts,
+                                 float *llr_out, float *s2_out, float *d2_ws,
+                                 double *ws, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Common-phase-error correction and the dummy-cell SNR (cells in front of
+ * the demapper).  atsc3_cpe_dev replaces lab/m6_cells.py:176-209
+ * cpe_correct (lab/m9_fast.py:619-634, lab/m9_gpu.py:283-299): pool
+ * normalised by its RMS, then per symbol `iters` rounds of
+ * c = sum(conj(h) z) / max(sum |h|^2, 1e-12), z /= c, with h the nearest
+ * point of the cell's region alphabet (region 0: pts0, 1: pts1) or its
+ * known dummy value (region 2).
+ *   pool [npool] complex64; cells/seg: symbol s = pool cells
+ *   cells[seg[s] .. seg[s+1]); region [npool] u8; dummy [npool] complex64
+ *   (may be NULL without region-2 cells); iters <= 8; alphabets pts0/pts1
+ *   complex128 (re, im pairs), at most 4096 points; out [npool] complex64 (cells of no symbol are not written);
+ *   gains [nsym] complex128 (the product of a symbol's rounds);
+ *   ws atsc3_cpe_ws_doubles(npool) doubles.
+ * atsc3_dummy_snr_dev replaces lab/m16_margin.py:303-318 frame_snr_db:
+ * cells [n] complex64 (the dummies), bits [n] scrambling bits (sign
+ * s = 1 - 2 bit); out[0] = 10 log10(a^2 / noise) with a = sum(re d s) /
+ * sum(s^2), noise = mean |d - a s|^2; out[1] = 1 if defined, else 0.
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_cpe_ws_doubles(int32_t npool);
+int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t *cells,
+                      const int32_t *seg, int32_t nsym, const uint8_t *region,
+                      const double *pts0, int32_t m0, const double *pts1,
+                      int32_t m1, const float *dummy, int32_t iters,
+                      float *out, double *gains, double *ws, void *stream);
+int32_t atsc3_dummy_snr_dev(const float *cells, int32_t n, const uint8_t *bits,
+                            double *out, void *stream);
+
+/* ------------------------------------------------------------------------
  * LDM successive interference cancellation.
  * Replaces the core-layer cancellation of lab/e69_enh.py:86-121: for each
  * block with conv[b] = 1 the symbols s are rebuilt from the decoded codeword

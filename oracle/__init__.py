@@ -18,6 +18,10 @@ Contents:
   ti.py                      twisted-block (hybrid) and convolutional time
                              de-interleavers (lab/m6_tbi.py:47-96,
                              lab/m10_cti.py:108-152, lab/m44_ldm.py:1187-1304).
+  cpe.py                     per-symbol common-phase-error correction
+                             (lab/m6_cells.py:176-209), dummy-cell SNR
+                             (lab/m16_margin.py:303-318), per-cell noise
+                             weights (lab/m9_fast.py:1096-1115).
   Makefile                   builds liboracle_minsum.so and, where the upstream
                              tree is present, _ref/ldpc_kernel.so from the
                              upstream's own source file.
