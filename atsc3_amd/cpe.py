@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from .baseband import scrambler_sequence
-from .demap import GATHER_TABLE, GatherSpec
+from .demap import GATHER_TABLE, GatherSpec, _check_quadrant_rule
 
 
 def _stream(dev):
@@ -61,8 +61,17 @@ class Cpe:
         self.cells = torch.from_numpy(order.astype(np.int32)).to(dev)
         self.seg = torch.from_numpy(seg.astype(np.int32)).to(dev)
         self.region = torch.from_numpy(region.astype(np.uint8)).to(dev)
-        self.alph = [_pts(a, dev) for a in alphabets]
-        self.msizes = [len(a) for a in alphabets]
+        # quadrant-rule alphabets are searched through their base points
+        self.alph, self.msizes = [], []
+        for a in alphabets:
+            a = np.asarray(a, np.complex128)
+            try:
+                _check_quadrant_rule(a)
+                self.alph.append(_pts(a[: len(a) // 4], dev))
+                self.msizes.append(-(len(a) // 4))
+            except ValueError:
+                self.alph.append(_pts(a, dev))
+                self.msizes.append(len(a))
         self.dummy = None
         if dummy_values is not None:
             self.dummy = torch.from_numpy(np.asarray(dummy_values, np.complex64)).to(dev)

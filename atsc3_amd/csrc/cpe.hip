@@ -99,17 +99,36 @@ __global__ __launch_bounds__(CPE_WG) void cpe_symbols(
                 if (m == 0) continue;              // no alphabet for the region
                 double best = INFINITY;
                 int arg = 0;
-                for (int j = 0; j < m; ++j) {
-                    const double2 w = pts[j];
-                    const double dr = z.re - w.x, di = z.im - w.y;
-                    const double d2 = dr * dr + di * di;
-                    if (d2 < best) {
-                        best = d2;
-                        arg = j;
+                if (m > 0) {
+                    for (int j = 0; j < m; ++j) {
+                        const double2 w = pts[j];
+                        const double dr = z.re - w.x, di = z.im - w.y;
+                        const double d2 = dr * dr + di * di;
+                        if (d2 < best) {
+                            best = d2;
+                            arg = j;
+                        }
                     }
+                    hr = pts[arg].x;
+                    hi_ = pts[arg].y;
+                } else {
+                    // quadrant-rule alphabet given by its -m first-quadrant
+                    // base points: the nearest point lies in z's quadrant
+                    // (|z - p| >= |z~ - w| for the mirror images), and there
+                    // (|zr| - wr)^2 is bit-identical to (zr - (+-wr))^2
+                    const double ar = fabs(z.re), ai = fabs(z.im);
+                    for (int j = 0; j < -m; ++j) {
+                        const double2 w = pts[j];
+                        const double dr = ar - w.x, di = ai - w.y;
+                        const double d2 = dr * dr + di * di;
+                        if (d2 < best) {
+                            best = d2;
+                            arg = j;
+                        }
+                    }
+                    hr = z.re >= 0.0 ? pts[arg].x : -pts[arg].x;
+                    hi_ = z.im >= 0.0 ? pts[arg].y : -pts[arg].y;
                 }
-                hr = pts[arg].x;
-                hi_ = pts[arg].y;
             }
             // conj(h) * z
             nr += hr * z.re + hi_ * z.im;
@@ -180,10 +199,12 @@ extern "C" int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t
                                  const float *dummy, int32_t iters, float *out, double *gains,
                                  double *ws, void *stream) {
     if (npool < 0 || nsym < 0 || iters < 0 || iters > CPE_MAX_ITERS) return ATSC3_E_ARG;
-    if (m0 < 0 || m1 < 0 || m0 > CPE_MAX_POINTS || m1 > CPE_MAX_POINTS) return ATSC3_E_SIZE;
+    if (m0 > CPE_MAX_POINTS || m1 > CPE_MAX_POINTS || m0 < -CPE_MAX_POINTS / 4 ||
+        m1 < -CPE_MAX_POINTS / 4)
+        return ATSC3_E_SIZE;
     if (npool == 0 || nsym == 0) return ATSC3_OK;
     if (!pool || !cells || !seg || !region || !out || !gains || !ws) return ATSC3_E_ARG;
-    if ((m0 > 0 && !pts0) || (m1 > 0 && !pts1)) return ATSC3_E_ARG;
+    if ((m0 != 0 && !pts0) || (m1 != 0 && !pts1)) return ATSC3_E_ARG;
     hipStream_t st = as_stream(stream);
     const int np = atsc3_cpe_ws_doubles(npool);
     hipLaunchKernelGGL(cpe_power, dim3(np), dim3(CPE_WG), 0, st,

@@ -261,7 +261,10 @@ int32_t atsc3_demap_weighted_dev(const float *pool, const float *pool_csi,
  *   pool [npool] complex64; cells/seg: symbol s = pool cells
  *   cells[seg[s] .. seg[s+1]); region [npool] u8; dummy [npool] complex64
  *   (may be NULL without region-2 cells); iters <= 8; alphabets pts0/pts1
- *   complex128 (re, im pairs), at most 4096 points; out [npool] complex64 (cells of no symbol are not written);
+ *   complex128 (re, im pairs): m > 0 points searched exhaustively, or
+ *   m < 0: the -m first-quadrant base points w of a quadrant-rule alphabet
+ *   [w, -conj w, conj w, -w] (nearest point searched in z's quadrant); at
+ *   most 4096 points; out [npool] complex64 (cells of no symbol are not written);
  *   gains [nsym] complex128 (the product of a symbol's rounds);
  *   ws atsc3_cpe_ws_doubles(npool) doubles.
  * atsc3_dummy_snr_dev replaces lab/m16_margin.py:303-318 frame_snr_db:

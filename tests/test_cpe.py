@@ -138,6 +138,36 @@ def test_cpe_matches_oracle(device, esn0, rot):
 
 
 @pytest.mark.gpu
+def test_cpe_exhaustive_and_folded_searches(device):
+    """A 256-NUC region (searched through its 64 base points) and an
+    alphabet without the quadrant symmetry (searched exhaustively)."""
+    import torch
+    from atsc3_amd.cpe import Cpe
+    p256 = bicm.points("256QAM", "10/15")
+    skew = P64 * np.exp(0.2j)                   # breaks the quadrant rule
+    rng = np.random.default_rng(40)
+    nsym, per = 8, 1200
+    npool = nsym * per
+    symbol_of = np.repeat(np.arange(nsym), per)
+    region = np.where(rng.random(npool) < 0.5, 0, 1).astype(np.int8)
+    x = np.where(region == 0, p256[rng.integers(256, size=npool)],
+                 skew[rng.integers(64, size=npool)])
+    g = np.exp(1j * rng.uniform(-0.05, 0.05, nsym))[symbol_of]
+    y = (g * x + 0.02 * (rng.standard_normal(npool) + 1j * rng.standard_normal(npool)))
+    y = y.astype(np.complex64)
+    want, wg = OC.cpe_correct(y, symbol_of, [p256, skew], region, None)
+    cpe = Cpe(symbol_of, region, [p256, skew], None, device)
+    assert cpe.msizes == [-64, 64]
+    print("[gpu] cpe folded/exhaustive", flush=True)
+    out, gains = cpe.run(torch.from_numpy(y).to(device))
+    torch.cuda.synchronize()
+    assert np.abs(out.cpu().numpy() - want).max() < 2e-6
+    gains = gains.cpu().numpy()
+    for k, s_ in enumerate(cpe.symbols):
+        assert abs(gains[k] - wg[int(s_)]) < 1e-9 * abs(wg[int(s_)])
+
+
+@pytest.mark.gpu
 def test_dummy_snr_matches_oracle(device):
     import torch
     from atsc3_amd.cpe import frame_snr_db
