@@ -14,8 +14,12 @@ struct CodeTab {
 };
 static_assert(sizeof(CodeTab) == 96, "code table descriptor layout");
 
-enum { H_ARRIVE = 0, H_GEN, H_ABORT, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS,
-       H_WORDS = 16 };
+enum { H_ABORT = 2, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS, H_WORDS = 16 };
+// Grid-barrier words, each on its own 128-byte line (32 ints) of the barrier
+// region: the global arrival count, the generation, then one arrival count
+// per workgroup group (workgroup w belongs to group w mod BATCH_BAR_GROUPS).
+#define BATCH_BAR_GROUPS 16
+enum { BAR_ARRIVE = 0, BAR_GEN = 32, BAR_GROUP = 64, BAR_WORDS = 64 + 32 * BATCH_BAR_GROUPS };
 // Var-list entry modes.  RETIRE_INIT: the codeword failed at its ladder rung
 // and starts the next one in the same slot (result kept first if taken).
 enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2, M_RETIRE_INIT = 3 };
@@ -24,7 +28,7 @@ enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2, M_RETIRE_INIT = 3 };
 __host__ __device__ inline int round4(int x) { return (x + 3) & ~3; }
 
 struct CtlLayout {
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best, words;
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best, bar, words;
     __host__ __device__ CtlLayout(int nblk, int slots) {
         var_list = H_WORDS;
         var_pre = var_list + 4 * slots;
@@ -35,7 +39,8 @@ struct CtlLayout {
         pend_list = free_list + round4(slots);
         steps = pend_list + round4(slots);
         best = steps + round4(nblk);
-        words = best + round4(nblk);
+        bar = (best + round4(nblk) + 31) & ~31;
+        words = bar + BAR_WORDS;
     }
 };
 
@@ -53,7 +58,7 @@ struct BatchArgs {
     void *msg_ws;
     long long tot_stride, msg_stride;   // elements per slot
     int *ctl;
-    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best;
+    int var_list, var_pre, chk_list, chk_pre, unsat, free_list, pend_list, steps, best, bar;
     uint8_t *bits, *conv;
     int *its, *bad;
 };

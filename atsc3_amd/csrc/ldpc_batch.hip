@@ -101,6 +101,7 @@ extern "C" int32_t atsc3_minsum_batch_dev(
     b.pend_list = L.pend_list;
     b.steps = L.steps;
     b.best = L.best;
+    b.bar = L.bar;
     b.bits = bits;
     b.conv = conv;
     b.its = its;

@@ -205,16 +205,34 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // (L2/L1 invalidation, the workgroup's waves share the CU's L1) before the
 // workgroup proceeds.  One write-back and one invalidation per workgroup
 // instead of one per wave: the barrier cost scales with their number.
+//
+// Arrival is two-level: a workgroup counts itself in its group's word and
+// the last of a group counts the group in the global word, so at most
+// BATCH_BAR_GROUPS atomics meet on any one line (512 same-line atomics
+// serialise); waiters poll the generation word, on a line of its own.
 __device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact, int *s_flag) {
     unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
+    unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
     wait_vmem();
     __syncthreads();
     if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x;
+        const unsigned ng = G < BATCH_BAR_GROUPS ? G : BATCH_BAR_GROUPS;
+        const unsigned g = blockIdx.x % ng;
+        const unsigned gsize = (G - g + ng - 1) / ng;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned old =
-            __hip_atomic_fetch_add(h + H_ARRIVE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_flag[0] = old + 1u == epoch * gridDim.x;
-        if (s_flag[0]) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const unsigned old = __hip_atomic_fetch_add(bar + BAR_GROUP + 32 * g, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        int last = 0;
+        if (old + 1u == epoch * gsize) {
+            // the group's arrivals happen-before the group's count
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            const unsigned o2 = __hip_atomic_fetch_add(bar + BAR_ARRIVE, 1u, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            last = o2 + 1u == epoch * ng;
+            if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        s_flag[0] = last;
     }
     __syncthreads();
     if (s_flag[0]) {
@@ -223,11 +241,11 @@ __device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact
         __syncthreads();
         if (threadIdx.x == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(h + H_GEN, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + BAR_GEN, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else if (threadIdx.x == 0) {
         unsigned spins = 0;
-        while (__hip_atomic_load(h + H_GEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        while (__hip_atomic_load(bar + BAR_GEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             if (__hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
             if (++spins >= BATCH_SPIN_LIMIT) {
                 __hip_atomic_store(h + H_ABORT, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -452,6 +470,7 @@ void batch_kernel(BatchArgs b) {
 __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
     int *h = b.ctl;
     for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) h[k] = 0;
+    for (int k = threadIdx.x; k < BAR_WORDS; k += ATSC3_WG) b.ctl[b.bar + k] = 0;
     for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG) b.ctl[b.free_list + k] = b.slots - 1 - k;
     for (int k = threadIdx.x; k < b.nblk; k += ATSC3_WG) b.ctl[b.best + k] = 0x7fffffff;
     if (threadIdx.x == 0) h[H_NFREE] = b.slots;
