@@ -31,6 +31,8 @@
 // wait is bounded: a workgroup that waits beyond the limit sets the abort
 // word and every workgroup leaves at its next barrier, so the kernel always
 // ends (the host reads the abort word to report the failure).
+#include <cstdlib>
+
 #include "ldpc_batch.h"
 
 extern "C" int32_t atsc3_minsum_batch_ctl_words(int32_t nblk, int32_t slots) {
@@ -84,6 +86,12 @@ extern "C" int32_t atsc3_minsum_batch_dev(
         const long long per = (long long)b.nrungs * (iters + 2) + 1;
         const long long mr = ((long long)nblk + 1) * per + 8;
         b.max_rounds = (int)(mr < (1LL << 30) ? mr : (1LL << 30));
+        // test hook: a lower round limit forces the abort path from the host
+        // (the kernel then leaves through its normal abort exit)
+        if (const char *e = getenv("ATSC3_BATCH_MAX_ROUNDS")) {
+            const int v = atoi(e);
+            if (v > 0 && v < b.max_rounds) b.max_rounds = v;
+        }
     }
     b.alpha = b.ladder[0];
     b.msg_scale = msg_scale;

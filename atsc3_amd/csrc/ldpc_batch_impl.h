@@ -472,7 +472,15 @@ __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
     for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) h[k] = 0;
     for (int k = threadIdx.x; k < BAR_WORDS; k += ATSC3_WG) b.ctl[b.bar + k] = 0;
     for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG) b.ctl[b.free_list + k] = b.slots - 1 - k;
-    for (int k = threadIdx.x; k < b.nblk; k += ATSC3_WG) b.ctl[b.best + k] = 0x7fffffff;
+    for (int k = threadIdx.x; k < b.nblk; k += ATSC3_WG) {
+        b.ctl[b.best + k] = 0x7fffffff;
+        // "not decoded" until the codeword's result is written: a launch
+        // that ends on the abort word leaves its = -1 on every codeword it
+        // did not finish, so the outputs themselves carry the failure
+        b.conv[k] = 0;
+        b.its[k] = -1;
+        b.bad[k] = -1;
+    }
     if (threadIdx.x == 0) h[H_NFREE] = b.slots;
     __syncthreads();
     compact(b);

@@ -211,8 +211,12 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
 
 
 def _to_numpy(bits, conv, its, bad):
-    return (bits.cpu().numpy(), conv.cpu().numpy().astype(bool),
-            its.cpu().numpy(), bad.cpu().numpy())
+    its_h = its.cpu().numpy()
+    if its_h.size and its_h.min() < 0:
+        # the batched engine marks codewords it did not finish (a grid
+        # barrier wait timed out) with its = -1
+        raise RuntimeError(f"decode aborted: {int((its_h < 0).sum())} codewords not decoded")
+    return (bits.cpu().numpy(), conv.cpu().numpy().astype(bool), its_h, bad.cpu().numpy())
 
 
 def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float32,

@@ -150,7 +150,9 @@ int32_t atsc3_minsum_qc_phased_dev(const float *llr, int32_t nblk, int32_t n,
  *   msg_ws    slots * max_nbe * 384 elements of the message format
  *   ctl_ws    atsc3_minsum_batch_ctl_words(nblk, slots) int32; after the
  *             call ctl_ws[atsc3_minsum_batch_status_word()] != 0 means a
- *             workgroup timed out waiting at a grid barrier (outputs invalid)
+ *             workgroup timed out waiting at a grid barrier; every codeword
+ *             the launch did not finish then reads its = -1, bad = -1
+ *             (its >= 0 for every codeword of a completed launch)
  * ------------------------------------------------------------------------ */
 typedef struct atsc3_code_tab {
     int32_t n, ncg, nvg, nbe, nsc, nsv, type_a, max_vdeg;

@@ -228,3 +228,31 @@ def test_single_rung_ladder_is_plain_decode(device):
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+def test_abort_marks_undecoded_codewords(device, monkeypatch):
+    """A round limit lowered from the host (ATSC3_BATCH_MAX_ROUNDS) makes the
+    compaction set the abort word; the kernel leaves normally, the status
+    word is set, unfinished codewords read its = -1 and the host conversion
+    raises instead of returning them."""
+    import torch
+    from atsc3_amd import ldpc
+    from atsc3_amd.batch import BatchPlan
+    code = get_code("8/15", 16200)
+    llr = synth.llr_bpsk_like(synth.codewords("8/15", 16200, 6, 3), -1.0, 4)
+    monkeypatch.setenv("ATSC3_BATCH_MAX_ROUNDS", "4")
+    print("[gpu] forced abort (round limit 4)", flush=True)
+    plan = BatchPlan([code] * 6, device, 50, 0.75, "float16")
+    plan.llr.copy_(torch.from_numpy(llr.reshape(-1)))
+    plan.run()
+    torch.cuda.synchronize()
+    with pytest.raises(RuntimeError):
+        plan.check()
+    assert (plan.its.cpu().numpy() == -1).all() and (plan.bad.cpu().numpy() == -1).all()
+    with pytest.raises(RuntimeError):
+        ldpc.min_sum_decode_batch(llr, code.checks, 16200, 50, 0.75, np.float32,
+                                  msg_dtype="float16", engine="batch", device=device)
+    monkeypatch.delenv("ATSC3_BATCH_MAX_ROUNDS")
+    out = ldpc.min_sum_decode_batch(llr, code.checks, 16200, 50, 0.75, np.float32,
+                                    msg_dtype="float16", engine="batch", device=device)
+    assert (out[2] >= 0).all()
