@@ -488,11 +488,11 @@ __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
 
 // INT8 message step per codeword (given, or 8 * mean|llr| / 127).
 __global__ __launch_bounds__(ATSC3_WG) void batch_steps(BatchArgs b) {
-    __shared__ float fslots[ATSC3_WG / 64];
+    __shared__ int s_hist[256];
     const int blk = blockIdx.x;
     const QC_CONST CodeTab *ct = b.codes + b.blk_code[blk];
     const float s = b.msg_scale ? b.msg_scale[blk]
-                                : int8_step_of_block(b.llr + b.blk_off[blk], ct->n, fslots);
+                                : int8_step_of_block(b.llr + b.blk_off[blk], ct->n, s_hist);
     if (threadIdx.x == 0) b.ctl[b.steps + blk] = __builtin_bit_cast(int, s);
 }
 

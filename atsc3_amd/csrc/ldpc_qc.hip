@@ -56,11 +56,11 @@ __global__ __launch_bounds__(ATSC3_WG) void minsum_qc_kernel(QcArgs a) {
     float *tot = a.tot_ws + (size_t)blk * a.nvg * QC_ROW;
     T *msg = reinterpret_cast<T *>(a.msg_ws) + (size_t)blk * a.nbe * QC_ROW;
     __shared__ int slots[ATSC3_WG / 64];
-    __shared__ float fslots[ATSC3_WG / 64];
+    __shared__ int s_hist[256];
 
     float step = 1.0f, inv_step = 1.0f;
     if (FMT == 2) {
-        step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, n, fslots);
+        step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, n, s_hist);
         inv_step = 1.0f / step;
     }
     if (tid < ATSC3_Z)

@@ -78,20 +78,48 @@ struct Msg<2> {
     }
 };
 
-// INT8 step when the caller gives none: clip level = 8 * mean|llr| of the
-// block, 127 steps.  Deterministic fixed-order reduction.
-__device__ inline float int8_step_of_block(const float *lam, int n, float *fslots) {
-    float acc = 0.0f;
-    for (int v = threadIdx.x; v < n; v += ATSC3_WG) acc += fabsf(lam[v]);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) fslots[threadIdx.x >> 6] = acc;
+// INT8 step when the caller gives none: 6 * median|llr| / 127 (the lower
+// median, the ((n-1)/2)-th smallest |llr|).  A mean is pulled up by the heavy
+// tail of faded high-order-constellation LLRs (too coarse a step), while the
+// median tracks the bulk for every channel and constellation.  The median is
+// found to 8 mantissa bits by two integer histograms in LDS -- the binary
+// exponent, then the top 8 mantissa bits inside the median's exponent -- so
+// the result does not depend on the order of the atomic updates.  Called by
+// every thread of the workgroup; `hist` is 256 ints of LDS.
+__device__ inline float int8_step_of_block(const float *lam, int n, int *hist) {
+    __shared__ int sel[2];
+    const int rank = (n - 1) / 2;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    float t = 0.0f;
-#pragma unroll
-    for (int w = 0; w < ATSC3_WG / 64; ++w) t += fslots[w];
+    for (int v = threadIdx.x; v < n; v += blockDim.x)
+        atomicAdd(&hist[(__float_as_uint(fabsf(lam[v])) >> 23) & 255], 1);
     __syncthreads();
-    const float mean = t / (float)n;
-    return fmaxf(8.0f * mean / 127.0f, 1e-30f);
+    if (threadIdx.x == 0) {
+        int c = 0, e = 0;
+        for (; e < 255 && c + hist[e] <= rank; ++e) c += hist[e];
+        sel[0] = e;
+        sel[1] = rank - c;
+    }
+    __syncthreads();
+    const unsigned e = (unsigned)sel[0];
+    const int r2 = sel[1];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (int v = threadIdx.x; v < n; v += blockDim.x) {
+        const unsigned u = __float_as_uint(fabsf(lam[v]));
+        if (((u >> 23) & 255) == e) atomicAdd(&hist[(u >> 15) & 255], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0, m = 0;
+        for (; m < 255 && c + hist[m] <= r2; ++m) c += hist[m];
+        sel[0] = m;
+    }
+    __syncthreads();
+    // middle of the median's 8-bit mantissa bucket
+    const float med = __uint_as_float((e << 23) | ((unsigned)sel[0] << 15) | (1u << 14));
+    __syncthreads();
+    return fmaxf(6.0f * med / 127.0f, 1e-30f);
 }
 
 __device__ __forceinline__ unsigned wrap360(unsigned p) { return min(p, p - 360u); }

@@ -39,7 +39,7 @@ __device__ __forceinline__ int *active_list(const PhaseCtl &c, int sel) {
 // tot = lam (padded rows), active list = all blocks, counters zeroed.
 template <int FMT>
 __global__ __launch_bounds__(ATSC3_WG) void phase_init(QcArgs a, PhaseCtl c, int nblk) {
-    __shared__ float fslots[ATSC3_WG / 64];
+    __shared__ int s_hist[256];
     const int blk = blockIdx.x;
     const unsigned tid = threadIdx.x;
     const float *lam = a.llr + (size_t)blk * a.n;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(ATSC3_WG) void phase_init(QcArgs a, PhaseCtl c, int
     if (tid < ATSC3_Z)
         for (int t = 0; t < a.nvg; ++t) tot[t * QC_ROW + tid] = lam[t * ATSC3_Z + tid];
     if (FMT == 2) {
-        const float s = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, a.n, fslots);
+        const float s = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, a.n, s_hist);
         if (tid == 0) c.step[blk] = s;
     }
     if (tid == 0) {

@@ -74,7 +74,8 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
  *       lab/ldpc_kernel.c on the same LLRs),
  *   1 = FP16 storage, 2 = INT8 storage with a per-block scale
  *       (msg_scale[b] = value of one INT8 step for block b; NULL -> derived
- *       on the device as 8 * mean|llr| / 127, saturating at +-127 steps).
+ *       on the device as 6 * median|llr| / 127, the median to 8 mantissa
+ *       bits; saturating at +-127 steps).
  * Variable sums always accumulate in FP32.  Any check degree; variable
  * degree <= 32 (every A/322 code), else ATSC3_E_DEGREE.
  * Workspace (contents on entry ignored): tot_ws [nblk * nvg * 384] f32;
