@@ -14,7 +14,11 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
   lab/m10_cti.py:108-152: q(i) = i + Nrows ((start_row + i) mod Nrows).
 * ``CtiStream`` restates the receive stream of lab/m44_ldm.py:1187-1304
   (frames appended, missing ones zero-filled, complete FEC Blocks taken once
-  the CTI's reach Nrows (Nrows - 1) is buffered) on an unbounded array.
+  the CTI's reach Nrows (Nrows - 1) is buffered) on an unbounded array; with
+  ``cap_frames`` it applies the upstream's capacity rule (lab/m44_ldm.py:
+  1262-1283): cells before the next output are released first, and when the
+  consumer has fallen further behind the oldest cells are dropped and the
+  next output moves up to the first cell still held (an overrun).
 
 Pinning (tests/test_ti.py): A/327 Figure 6.5's printed 4x3 example with one
 virtual FEC Block (output b g a f d e c h, lab/m6_tbi.py:41-42); the read
@@ -106,13 +110,15 @@ def solve_C(fec_block_start, start_row, nrows):
 
 
 class CtiStream:
-    """Frames in, complete FEC Blocks out (no capacity limit)."""
+    """Frames in, complete FEC Blocks out (capacity-limited with cap_frames)."""
 
-    def __init__(self, nrows, ncell, plp_size):
+    def __init__(self, nrows, ncell, plp_size, cap_frames=None):
         self.nrows, self.ncell, self.plp_size = nrows, ncell, plp_size
         self.reach = nrows * (nrows - 1)
+        self.cap = None if cap_frames is None else max(cap_frames, 3) * plp_size
         self.cells = np.zeros(0, np.complex64)
         self.next_frame = None
+        self.overruns = 0
 
     def reset(self, origin_frame, start_row, fec_block_start):
         self.start_row = start_row % self.nrows
@@ -123,15 +129,30 @@ class CtiStream:
         self.next_frame = origin_frame
         self.out_ptr = self.C0
         self.block_id = 0
+        self.base = 0
+
+    def _room(self):
+        if self.cap is None:
+            return
+        end, n = len(self.cells), self.plp_size
+        if end - self.base + n > self.cap:
+            self.base = max(self.base, self.out_ptr)
+            if end - self.base + n > self.cap:
+                self.base += end - self.base + n - self.cap
+                self.out_ptr = max(self.out_ptr, self.base)
+                self.overruns += 1
 
     def push(self, frame_idx, cells):
         if self.next_frame is None or frame_idx < self.next_frame:
             return
-        gap = frame_idx - self.next_frame
         zero = np.zeros(self.plp_size, np.complex64)
-        parts = [self.cells] + [zero] * gap
-        parts.append(zero if cells is None else np.asarray(cells, np.complex64))
-        self.cells = np.concatenate(parts)
+        while self.next_frame < frame_idx:
+            self._room()
+            self.cells = np.concatenate([self.cells, zero])
+            self.next_frame += 1
+        self._room()
+        self.cells = np.concatenate([self.cells, zero if cells is None
+                                     else np.asarray(cells, np.complex64)])
         self.next_frame = frame_idx + 1
 
     def available_blocks(self):

@@ -116,6 +116,38 @@ def test_oracle_cti_stream_returns_the_transmitted_blocks():
         assert np.array_equal(cells, pre[lo:lo + ncell])
 
 
+def test_cti_stream_bookkeeping_with_overrun_on_cpu():
+    """The device stream's host bookkeeping (ring wrap, zero-filled gaps, the
+    overrun rule) against the oracle, with a CPU ring: the consumer takes
+    nothing for 4 frames (an overrun), then keeps up."""
+    import torch
+    nrows, ncell, plp, C, start = 16, 300, 1000, 77, 5
+    pre, frames, fbs = _cti_stream(nrows, ncell, plp, 14, start, C, 11)
+    ref = OT.CtiStream(nrows, ncell, plp, cap_frames=3)
+    dev = ti.CtiStreamDev(nrows, ncell, plp, cap_frames=3, device="cpu")
+    ref.reset(0, start, fbs)
+    dev.reset(0, start, fbs)
+    got_ref = []
+    for f, fr in enumerate(frames):
+        if f == 9:
+            continue                                  # lost frame
+        ref.push(f, fr)
+        dev.push(f, torch.from_numpy(fr))
+        assert (dev.out_ptr, dev.end) == (ref.out_ptr, len(ref.cells))
+        if 2 <= f <= 5:
+            continue                                  # consumer stalls
+        want = ref.take_blocks()
+        first, cells = dev.take_blocks()
+        got_ref += want
+        if want:
+            assert first == want[0][0]
+            assert np.array_equal(cells.numpy(), np.stack([c for _, c in want]))
+        else:
+            assert cells.shape[0] == 0
+    assert ref.overruns >= 1 and dev.stats["buffer_overrun"] == ref.overruns
+    assert dev.stats["zero_filled_frames"] == 1 and len(got_ref) >= 20
+
+
 # ---------------------------------------------------------------------------
 # device
 # ---------------------------------------------------------------------------
