@@ -70,8 +70,8 @@ _SIGS = {
     "atsc3_demap_weighted_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                                  c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_cpe_ws_doubles": [c_i32],
-    "atsc3_cpe_dev": [c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_i32,
-                      c_vp, c_vp, c_vp, c_vp],
+    "atsc3_cpe_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i32,
+                      c_vp, c_i32, c_vp, c_vp, c_vp, c_vp],
     "atsc3_dummy_snr_dev": [c_vp, c_i32, c_vp, c_vp, c_vp],
     "atsc3_sp_ctl_words": [c_i32],
     "atsc3_sum_product_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp,

@@ -56,10 +56,9 @@ class Cpe:
         if (region == 2).any() and dummy_values is None:
             raise ValueError("dummy cells need their values")
         self.symbols, inv = np.unique(symbol_of, return_inverse=True)
-        order = np.argsort(inv, kind="stable")
-        seg = np.concatenate([[0], np.cumsum(np.bincount(inv, minlength=len(self.symbols)))])
-        self.cells = torch.from_numpy(order.astype(np.int32)).to(dev)
-        self.seg = torch.from_numpy(seg.astype(np.int32)).to(dev)
+        self._inv = inv
+        self.cells, self.seg = self._lists(np.arange(self.npool))
+        self._est = {1: (self.cells, self.seg)}
         self.region = torch.from_numpy(region.astype(np.uint8)).to(dev)
         # quadrant-rule alphabets are searched through their base points
         self.alph, self.msizes = [], []
@@ -78,9 +77,19 @@ class Cpe:
         self.ws = torch.empty(_lib.lib().atsc3_cpe_ws_doubles(self.npool), dtype=torch.float64,
                               device=dev)
 
-    def run(self, pool, iters=3, out=None):
+    def _lists(self, which):
+        """Per-symbol CSR list (device) of the pool cells ``which``."""
+        inv = self._inv[which]
+        order = which[np.argsort(inv, kind="stable")]
+        seg = np.concatenate([[0], np.cumsum(np.bincount(inv, minlength=len(self.symbols)))])
+        return (torch.from_numpy(order.astype(np.int32)).to(self.device),
+                torch.from_numpy(seg.astype(np.int32)).to(self.device))
+
+    def run(self, pool, iters=3, out=None, decim=1):
         """pool (npool,) complex64 device tensor -> (corrected cells,
-        gains (nsym,) complex128 in the order of ``self.symbols``)."""
+        gains (nsym,) complex128 in the order of ``self.symbols``).  With
+        decim > 1 only cells at pool indices divisible by decim estimate the
+        gains (the upstream's fast path, decim 4); every cell is corrected."""
         if not (pool.is_cuda and pool.dtype == torch.complex64 and pool.numel() == self.npool):
             raise ValueError(f"pool must be a ({self.npool},) complex64 device tensor")
         if not 0 <= iters <= self.MAX_ITERS:
@@ -93,8 +102,12 @@ class Cpe:
         p1 = self.alph[1].data_ptr() if len(self.alph) > 1 else None
         m0 = self.msizes[0] if len(self.alph) > 0 else 0
         m1 = self.msizes[1] if len(self.alph) > 1 else 0
+        if decim not in self._est:
+            self._est[decim] = self._lists(np.arange(0, self.npool, int(decim)))
+        ec, es = self._est[decim]
         _lib.call("atsc3_cpe_dev", torch.view_as_real(pool).data_ptr(), self.npool,
-                  self.cells.data_ptr(), self.seg.data_ptr(), len(self.symbols),
+                  self.cells.data_ptr(), self.seg.data_ptr(), ec.data_ptr(), es.data_ptr(),
+                  len(self.symbols),
                   self.region.data_ptr(), p0, m0, p1, m1,
                   None if self.dummy is None else torch.view_as_real(self.dummy).data_ptr(),
                   int(iters), torch.view_as_real(out).data_ptr(),

@@ -5,12 +5,14 @@
 //   (the per-part noise weights of the weighted-LLR path are in demap.hip).
 //
 // CPE: the pool is normalised by its RMS (one FP64 sum over the pool, split
-// into per-workgroup partials that every symbol workgroup re-adds in the same
-// order), then each symbol -- a list of pool cells -- is handled by one
+// into at most 256 per-workgroup partials that every symbol workgroup re-adds
+// in the same tree order), then each symbol -- a list of pool cells -- is handled by one
 // workgroup that runs all `iters` rounds itself: symbols do not interact, so
 // no grid-wide step is needed.  Per round: each cell's value is the
-// normalised pool value divided by the gains found so far (in order), its
-// hard decision the nearest point of its region's alphabet (FP64 squared
+// normalised pool value divided by the gains found so far (in order); the
+// cells that estimate the gain (a second per-symbol list: every cell, or the
+// upstream fast path's every 4th pool cell, lab/m9_fast.py:667-678) take their
+// hard decision, the nearest point of its region's alphabet (FP64 squared
 // distances to the FP64 alphabet, first point on ties) or its known dummy
 // value, and the round's
 // gain c = sum(conj(h) z) / max(sum |h|^2, 1e-12) with FP64 sums reduced in a
@@ -64,21 +66,27 @@ __global__ __launch_bounds__(CPE_WG) void cpe_power(const float2 *__restrict__ p
 __global__ __launch_bounds__(CPE_WG) void cpe_symbols(
     const float2 *__restrict__ pool, int npool, const double *__restrict__ partial, int nparts,
     const int32_t *__restrict__ cells, const int32_t *__restrict__ seg,
+    const int32_t *__restrict__ ecells, const int32_t *__restrict__ eseg,
     const uint8_t *__restrict__ region, const double2 *__restrict__ pts0, int m0,
     const double2 *__restrict__ pts1, int m1, const float2 *__restrict__ dummy, int iters,
     float2 *__restrict__ out, double *__restrict__ gains) {
     __shared__ double red[CPE_WG / 64];
     __shared__ C64 cs[CPE_MAX_ITERS];
     const int sym = blockIdx.x;
-    const int lo = seg[sym], hi = seg[sym + 1];
-    double tot = 0.0;
-    for (int q = 0; q < nparts; ++q) tot += partial[q];
+    // the pool power: one partial per thread (nparts <= CPE_WG), summed in
+    // the same fixed tree order by every symbol workgroup
+    const double tot =
+        wg_sum_f64((int)threadIdx.x < nparts ? partial[threadIdx.x] : 0.0, red);
     const double rms = sqrt(tot / (double)npool);
     C64 ctot = {1.0, 0.0};
     for (int it = 0; it <= iters; ++it) {
         double nr = 0.0, ni = 0.0, dn = 0.0;
+        // decision rounds walk the estimating cells, the last pass every cell
+        const int32_t *list = it == iters ? cells : ecells;
+        const int lo = it == iters ? seg[sym] : eseg[sym];
+        const int hi = it == iters ? seg[sym + 1] : eseg[sym + 1];
         for (int k = lo + (int)threadIdx.x; k < hi; k += CPE_WG) {
-            const int c = cells[k];
+            const int c = list[k];
             const float2 p = pool[c];
             C64 z = {(double)p.x / rms, (double)p.y / rms};
             for (int q = 0; q < it; ++q) z = cdiv(z, cs[q]);
@@ -194,11 +202,17 @@ extern "C" int32_t atsc3_cpe_ws_doubles(int32_t npool) {
 }
 
 extern "C" int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t *cells,
-                                 const int32_t *seg, int32_t nsym, const uint8_t *region,
+                                 const int32_t *seg, const int32_t *est_cells,
+                                 const int32_t *est_seg, int32_t nsym, const uint8_t *region,
                                  const double *pts0, int32_t m0, const double *pts1, int32_t m1,
                                  const float *dummy, int32_t iters, float *out, double *gains,
                                  double *ws, void *stream) {
     if (npool < 0 || nsym < 0 || iters < 0 || iters > CPE_MAX_ITERS) return ATSC3_E_ARG;
+    if ((est_cells == nullptr) != (est_seg == nullptr)) return ATSC3_E_ARG;
+    if (!est_cells) {
+        est_cells = cells;
+        est_seg = seg;
+    }
     if (m0 > CPE_MAX_POINTS || m1 > CPE_MAX_POINTS || m0 < -CPE_MAX_POINTS / 4 ||
         m1 < -CPE_MAX_POINTS / 4)
         return ATSC3_E_SIZE;
@@ -210,7 +224,8 @@ extern "C" int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t
     hipLaunchKernelGGL(cpe_power, dim3(np), dim3(CPE_WG), 0, st,
                        reinterpret_cast<const float2 *>(pool), npool, ws);
     hipLaunchKernelGGL(cpe_symbols, dim3(nsym), dim3(CPE_WG), 0, st,
-                       reinterpret_cast<const float2 *>(pool), npool, ws, np, cells, seg, region,
+                       reinterpret_cast<const float2 *>(pool), npool, ws, np, cells, seg,
+                       est_cells, est_seg, region,
                        reinterpret_cast<const double2 *>(pts0), m0,
                        reinterpret_cast<const double2 *>(pts1), m1,
                        reinterpret_cast<const float2 *>(dummy), iters,

@@ -267,7 +267,10 @@ int32_t atsc3_demap_weighted_dev(const float *pool, const float *pool_csi,
  *   complex128 (re, im pairs): m > 0 points searched exhaustively, or
  *   m < 0: the -m first-quadrant base points w of a quadrant-rule alphabet
  *   [w, -conj w, conj w, -w] (nearest point searched in z's quadrant); at
- *   most 4096 points; out [npool] complex64 (cells of no symbol are not written);
+ *   most 4096 points; est_cells/est_seg: the cells of symbol s that estimate
+ *   its gains, est_cells[est_seg[s] .. est_seg[s+1]) (the upstream's fast
+ *   path takes every 4th pool cell, lab/m9_fast.py:667-678), or both NULL:
+ *   every cell; every cell of a symbol is corrected; out [npool] complex64 (cells of no symbol are not written);
  *   gains [nsym] complex128 (the product of a symbol's rounds);
  *   ws atsc3_cpe_ws_doubles(npool) doubles.
  * atsc3_dummy_snr_dev replaces lab/m16_margin.py:303-318 frame_snr_db:
@@ -277,7 +280,8 @@ int32_t atsc3_demap_weighted_dev(const float *pool, const float *pool_csi,
  * ------------------------------------------------------------------------ */
 int32_t atsc3_cpe_ws_doubles(int32_t npool);
 int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t *cells,
-                      const int32_t *seg, int32_t nsym, const uint8_t *region,
+                      const int32_t *seg, const int32_t *est_cells,
+                      const int32_t *est_seg, int32_t nsym, const uint8_t *region,
                       const double *pts0, int32_t m0, const double *pts1,
                       int32_t m1, const float *dummy, int32_t iters,
                       float *out, double *gains, double *ws, void *stream);

@@ -43,7 +43,9 @@ def nearest(z, pts):
     return out
 
 
-def cpe_correct(pool, symbol_of, alphabets, region, dummy_values, iters=3):
+def cpe_correct(pool, symbol_of, alphabets, region, dummy_values, iters=3, decim=1):
+    """decim > 1: the gain of a symbol is estimated from its cells at pool
+    indices divisible by decim (lab/m9_fast.py:667-678), then applied to all."""
     pool = np.asarray(pool, np.complex128)
     out = pool / np.sqrt(np.mean(np.abs(pool) ** 2))
     gains = {}
@@ -51,17 +53,19 @@ def cpe_correct(pool, symbol_of, alphabets, region, dummy_values, iters=3):
         m = np.flatnonzero(symbol_of == sym)
         z = out[m]
         r = np.asarray(region)[m]
+        est = m % decim == 0
         c_tot = 1.0 + 0j
         for _ in range(iters):
-            hard = np.empty_like(z)
+            zs, rs, ms = z[est], r[est], m[est]
+            hard = np.empty_like(zs)
             for rid, pts in enumerate(alphabets):
-                sel = r == rid
+                sel = rs == rid
                 if sel.any():
-                    hard[sel] = nearest(z[sel], pts)
-            sel = r == 2
+                    hard[sel] = nearest(zs[sel], pts)
+            sel = rs == 2
             if sel.any():
-                hard[sel] = np.asarray(dummy_values)[m[sel]]
-            c = np.vdot(hard, z) / max(np.vdot(hard, hard).real, 1e-12)
+                hard[sel] = np.asarray(dummy_values)[ms[sel]]
+            c = np.vdot(hard, zs) / max(np.vdot(hard, hard).real, 1e-12)
             z = z / c
             c_tot *= c
         out[m] = z

@@ -65,6 +65,21 @@ def test_noiseless_gains_come_back():
         assert abs(gains[int(s)] - g / rms) < 1e-6 * abs(g / rms)
 
 
+def test_decimated_estimate_uses_every_kth_pool_cell():
+    """decim = k: the gain comes from the pool cells at indices divisible by
+    k only -- perturbing every other cell leaves the gains unchanged."""
+    f = _frame(6, 14.0)
+    pool = f["pool"].astype(np.complex128)
+    keep = np.arange(len(pool)) % 4 == 0
+    noisy = pool.copy()
+    noisy[~keep] *= np.exp(0.7j)
+    s = np.sqrt(np.mean(np.abs(pool) ** 2) / np.mean(np.abs(noisy) ** 2))
+    _, g1 = OC.cpe_correct(pool, f["symbol_of"], [P64, PQ], f["region"], f["dv"], decim=4)
+    _, g2 = OC.cpe_correct(noisy * s, f["symbol_of"], [P64, PQ], f["region"], f["dv"], decim=4)
+    for k in g1:
+        assert abs(g1[k] - g2[k]) < 1e-9 * abs(g1[k])
+
+
 def test_dummy_only_symbol_is_least_squares():
     rng = np.random.default_rng(2)
     n = 900
@@ -117,15 +132,17 @@ def test_weighted_llr_is_demap_with_part_weights():
 # ---------------------------------------------------------------------------
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("esn0,rot", [(None, 0.15), (16.0, 0.15), (12.0, 0.3)])
-def test_cpe_matches_oracle(device, esn0, rot):
+@pytest.mark.parametrize("esn0,rot,decim", [(None, 0.15, 1), (16.0, 0.15, 1), (12.0, 0.3, 1),
+                                            (12.0, 0.3, 4)])
+def test_cpe_matches_oracle(device, esn0, rot, decim):
     import torch
     from atsc3_amd.cpe import Cpe
     f = _frame(10 + int(esn0 or 0), esn0, rot=rot)
-    want, wg = OC.cpe_correct(f["pool"], f["symbol_of"], [P64, PQ], f["region"], f["dv"])
+    want, wg = OC.cpe_correct(f["pool"], f["symbol_of"], [P64, PQ], f["region"], f["dv"],
+                              decim=decim)
     cpe = Cpe(f["symbol_of"], f["region"], [P64, PQ], f["dv"], device)
-    print(f"[gpu] cpe esn0={esn0} rot={rot}", flush=True)
-    out, gains = cpe.run(torch.from_numpy(f["pool"]).to(device))
+    print(f"[gpu] cpe esn0={esn0} rot={rot} decim={decim}", flush=True)
+    out, gains = cpe.run(torch.from_numpy(f["pool"]).to(device), decim=decim)
     torch.cuda.synchronize()
     out = out.cpu().numpy()
     gains = gains.cpu().numpy()
