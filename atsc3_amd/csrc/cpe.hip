@@ -208,15 +208,15 @@ extern "C" int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t
                                  const float *dummy, int32_t iters, float *out, double *gains,
                                  double *ws, void *stream) {
     if (npool < 0 || nsym < 0 || iters < 0 || iters > CPE_MAX_ITERS) return ATSC3_E_ARG;
+    if (m0 > CPE_MAX_POINTS || m1 > CPE_MAX_POINTS || m0 < -CPE_MAX_POINTS / 4 ||
+        m1 < -CPE_MAX_POINTS / 4)
+        return ATSC3_E_SIZE;
+    if (npool == 0 || nsym == 0) return ATSC3_OK;
     if ((est_cells == nullptr) != (est_seg == nullptr)) return ATSC3_E_ARG;
     if (!est_cells) {
         est_cells = cells;
         est_seg = seg;
     }
-    if (m0 > CPE_MAX_POINTS || m1 > CPE_MAX_POINTS || m0 < -CPE_MAX_POINTS / 4 ||
-        m1 < -CPE_MAX_POINTS / 4)
-        return ATSC3_E_SIZE;
-    if (npool == 0 || nsym == 0) return ATSC3_OK;
     if (!pool || !cells || !seg || !region || !out || !gains || !ws) return ATSC3_E_ARG;
     if ((m0 != 0 && !pts0) || (m1 != 0 && !pts1)) return ATSC3_E_ARG;
     hipStream_t st = as_stream(stream);

@@ -134,10 +134,13 @@ def rescue_dev(llr, bits, conv, its, bad, code, near_max=1500, max_blocks=8, ite
     i32, count [1] i32, sp_conv [max_blocks] u8, sp_shed [max_blocks] u8):
     the rows offered, how many, and which converged / ran out of time."""
     dev = llr.device
-    sp = tables_for(code, dev)
     nblk = conv.numel()
     rows = torch.zeros(max_blocks, dtype=torch.int32, device=dev)
     count = torch.zeros(1, dtype=torch.int32, device=dev)
+    if nblk == 0:
+        z = torch.zeros(max_blocks, dtype=torch.uint8, device=dev)
+        return rows, count, z, z.clone()
+    sp = tables_for(code, dev)
     st = torch.cuda.current_stream(dev).cuda_stream
     _lib.call("atsc3_sp_select_dev", conv.data_ptr(), bad.data_ptr(),
               None if blk_code is None else blk_code.data_ptr(), int(code_id), nblk,

@@ -223,3 +223,13 @@ def test_weighted_demap_matches_oracle(device, gathered):
     assert np.abs(s2g.cpu().numpy() - s2).max() <= 1e-5 * s2.max()
     got = llr.cpu().numpy().astype(np.float64)
     assert OD.normwise_rel_err(got, want).max() < 1e-5
+
+
+@pytest.mark.gpu
+def test_empty_inputs(device):
+    import torch
+    from atsc3_amd.cpe import Cpe, frame_snr_db
+    cpe = Cpe(np.zeros(0, int), np.zeros(0, np.int8), [P64], None, device)
+    out, gains = cpe.run(torch.zeros(0, dtype=torch.complex64, device=device))
+    assert out.numel() == 0 and gains.numel() == 0
+    assert frame_snr_db(torch.zeros(10, dtype=torch.complex64, device=device), 5) is None

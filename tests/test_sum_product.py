@@ -273,3 +273,53 @@ def test_rescue_in_mixed_batch_only_touches_its_code(device):
             assert conv[b] == conv0[b]
         elif conv[b]:
             assert np.array_equal(bits[plan.off[b]:plan.off[b + 1]], cw_a[b // 2])
+
+
+@pytest.mark.gpu
+def test_widest_checks_degree_86(device):
+    """64800 r7/15 has checks of degree 86: the second sign word and an
+    80 + 6 pairwise row sum over the padded width."""
+    import torch
+    from atsc3_amd.rescue import SumProduct
+    code = get_code("7/15", 64800)
+    sp = SumProduct(code, device)
+    assert sp.dmax == 86
+    lam = (np.random.default_rng(70).standard_normal((2, 64800)) * 0.5).astype(np.float32)
+    print("[gpu] sum-product degree 86 totals", flush=True)
+    sp.decode(torch.from_numpy(lam).to(device), iters=2)
+    torch.cuda.synchronize()
+    tot = sp.workspace(2)[0].view(2, 64800).cpu().numpy()
+    want = SP.iterate(lam, code.checks, 64800, 2)
+    assert (np.abs(tot - want) / (1.0 + np.abs(want))).max() < 1e-9
+    cw = synth.codewords("7/15", 64800, 2, 71)
+    llr = synth.llr_bpsk_like(cw, 1.6, 72)
+    want = SP.sum_product(llr, code.checks, 64800, 30)
+    got = [x.cpu().numpy() for x in sp.decode(torch.from_numpy(llr).to(device), iters=30)]
+    assert want[1].all() and np.array_equal(want[0], cw)
+    for name, g, w in zip(("bits", "conv", "its", "bad", "shed"), got, want):
+        assert np.array_equal(g.astype(w.dtype), w), name
+
+
+@pytest.mark.gpu
+def test_rescue_with_nothing_to_offer(device):
+    """An all-converged batch and an empty batch: nothing is offered and the
+    batch results are untouched."""
+    import torch
+    from atsc3_amd import ldpc
+    from atsc3_amd.rescue import rescue_dev
+    code, cw, llr = _near_threshold("8/15", 4.0, 3, 81)
+    t = torch.from_numpy(llr).to(device)
+    bits, conv, its, bad = ldpc.decode_dev(t, code, 50, 0.75)
+    before = [x.clone() for x in (bits, conv, its, bad)]
+    rows, count, _, _ = rescue_dev(t, bits, conv, its, bad, code, budget_s=None)
+    torch.cuda.synchronize()
+    assert int(count.item()) == 0 and bool(conv.all())
+    for x, y in zip(before, (bits, conv, its, bad)):
+        assert torch.equal(x, y)
+    e = torch.empty((0, 16200), dtype=torch.float32, device=device)
+    eb = torch.empty((0, 16200), dtype=torch.uint8, device=device)
+    ec = torch.empty(0, dtype=torch.uint8, device=device)
+    ei = torch.empty(0, dtype=torch.int32, device=device)
+    rows, count, _, _ = rescue_dev(e, eb, ec, ei, ei.clone(), code, budget_s=None)
+    torch.cuda.synchronize()
+    assert int(count.item()) == 0
