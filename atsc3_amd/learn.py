@@ -5,7 +5,7 @@ codewords: QPSK over AWGN with Eb/N0 uniform in a range, cells demapped by
 the differentiable soft demapper, T unrolled iterations, binary
 cross-entropy of the output LLRs against the sent bits, Adam on the (T,
 classes) factors initialised at the plain value.  The learned table is then
-judged against the plain decoder (one factor for every iteration and
+compared with the plain decoder (one factor for every iteration and
 degree) on fresh blocks; the plain bit-exact min-sum is the fallback for
 blocks the learned decoder does not bring to a zero syndrome.
 

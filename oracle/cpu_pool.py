@@ -1,6 +1,6 @@
 """CPU baseline harness: the upstream's multi-process decode topology.
 
-TEST / BASELINE INFRASTRUCTURE ONLY (see oracle/__init__.py): bench.py's
+TEST / CPU-BASELINE INFRASTRUCTURE ONLY (see oracle/__init__.py): bench.py's
 ``cpu_baseline`` leg times the CPU path with it; nothing in the product
 imports it.
 
