@@ -65,6 +65,7 @@ _SIGS = {
                                  c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_baseband_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp],
+    "atsc3_gather_cells_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp],
     "atsc3_demap_gather_dev": [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp,
                                c_vp, c_vp, c_vp],
     "atsc3_demap_weighted_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,

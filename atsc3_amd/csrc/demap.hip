@@ -346,3 +346,42 @@ extern "C" int32_t atsc3_demap_weighted_dev(
                        s2_out);
     return atsc3::launch_status();
 }
+
+namespace atsc3 {
+// out[b * ncell + s] = pool[cell(b, s)] through the gather map.
+template <int GM>
+__global__ __launch_bounds__(DEMAP_WG) void gather_cells(const float2 *__restrict__ pool,
+                                                         Gather g, int ncell,
+                                                         float2 *__restrict__ out) {
+    const int b = blockIdx.y;
+    const int s = blockIdx.x * DEMAP_WG + threadIdx.x;
+    if (s < ncell) out[(size_t)b * ncell + s] = pool[cell_of<GM>(g, b, ncell, s)];
+}
+}  // namespace atsc3
+
+extern "C" int32_t atsc3_gather_cells_dev(const float *pool, const atsc3_gather *gather,
+                                          int32_t nblk, int32_t ncell, float *out,
+                                          void *stream) {
+    if (nblk < 0 || ncell < 0 || !gather) return ATSC3_E_ARG;
+    if (nblk == 0 || ncell == 0) return ATSC3_OK;
+    if (!pool || !out) return ATSC3_E_ARG;
+    atsc3::Gather g{gather->idx, gather->first, gather->ring, gather->nrows, gather->start_row};
+    const dim3 grid((ncell + DEMAP_WG - 1) / DEMAP_WG, nblk);
+    hipStream_t st = atsc3::as_stream(stream);
+    const float2 *p = reinterpret_cast<const float2 *>(pool);
+    float2 *o = reinterpret_cast<float2 *>(out);
+    if (gather->mode == ATSC3_GATHER_TABLE) {
+        if (!g.idx) return ATSC3_E_ARG;
+        hipLaunchKernelGGL(atsc3::gather_cells<atsc3::GATHER_TABLE>, grid, dim3(DEMAP_WG), 0, st,
+                           p, g, ncell, o);
+    } else if (gather->mode == ATSC3_GATHER_CTI) {
+        if (g.nrows <= 0 || g.ring <= 0 || g.first < 0 || g.start_row < 0 ||
+            g.start_row >= g.nrows)
+            return ATSC3_E_ARG;
+        hipLaunchKernelGGL(atsc3::gather_cells<atsc3::GATHER_CTI>, grid, dim3(DEMAP_WG), 0, st,
+                           p, g, ncell, o);
+    } else {
+        return ATSC3_E_ARG;
+    }
+    return atsc3::launch_status();
+}

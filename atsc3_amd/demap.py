@@ -128,6 +128,19 @@ def demap_gather_q_dev(pool, pts_dev, nb, lam_of_q_dev, spec, nblk, ncell, csi=N
     return llr, s2out
 
 
+def gather_cells_dev(pool, spec, nblk, ncell, out=None):
+    """(nblk, ncell) complex64 device tensor of the cells ``spec`` maps to."""
+    if not (pool.is_cuda and pool.dtype == torch.complex64 and pool.is_contiguous()):
+        raise TypeError("pool must be a contiguous complex64 device tensor")
+    out = out if out is not None else torch.empty((nblk, ncell), dtype=torch.complex64,
+                                                  device=pool.device)
+    if nblk and ncell:
+        _lib.call("atsc3_gather_cells_dev", torch.view_as_real(pool).data_ptr(),
+                  ctypes.byref(spec), nblk, ncell, torch.view_as_real(out).data_ptr(),
+                  _stream(pool.device))
+    return out
+
+
 @functools.lru_cache(maxsize=None)
 def _points_dev(key, device):
     mod, rate, scale = key

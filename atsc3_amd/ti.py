@@ -12,10 +12,12 @@ interleaver's output before demapping it:
   Nrows) (lab/m10_cti.py:108-146), buffered until the interleaver's reach is
   in (lab/m44_ldm.py:1187-1304 CtiStream).
 
-Here neither gather is materialised: ``HtiGather`` keeps the index table on
-the device and the demap kernel reads through it; ``CtiStreamDev`` keeps the
-received frames in a device ring buffer and the demap kernel evaluates q(i)
-itself.
+Here the demap kernel reads through either map instead of a materialised
+gather: ``HtiGather`` keeps the index table on the device; ``CtiStreamDev``
+keeps the received frames in a device ring buffer and the kernel evaluates
+q(i) itself.  Where a consumer needs the de-interleaved cells themselves
+(the LDM cancellation subtracts the re-modulated core layer from them),
+``gather`` / ``take_blocks`` materialise them with one gather kernel.
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ import collections
 import numpy as np
 import torch
 
-from .demap import GATHER_CTI, GATHER_TABLE, GatherSpec
+from .demap import GATHER_CTI, GATHER_TABLE, GatherSpec, gather_cells_dev
 
 
 def _virtual_before(i, nrows, ncols, nv):
@@ -98,7 +100,7 @@ class HtiGather:
 
     def gather(self, cells):
         """The gathered blocks themselves ((nblk, ncell) device tensor)."""
-        return cells[self.idx.long()].view(self.nblk, self.ncell)
+        return gather_cells_dev(cells.contiguous(), self.spec, self.nblk, self.ncell)
 
 
 class CtiStreamDev:
@@ -223,10 +225,12 @@ class CtiStreamDev:
         if nb <= 0:
             return self.block_id, torch.zeros((0, self.ncell), dtype=torch.complex64,
                                               device=self.device)
-        i = torch.arange(self.out_ptr, self.out_ptr + nb * self.ncell, dtype=torch.int64,
-                         device=self.device)
-        q = i + self.nrows * ((self.start_row + i) % self.nrows)
-        cells = self.ring[q % self.cap].view(nb, self.ncell)
+        if self.device.type == "cuda":
+            cells = gather_cells_dev(self.ring, self._spec(), nb, self.ncell)
+        else:                                  # host ring (bookkeeping checks)
+            i = torch.arange(self.out_ptr, self.out_ptr + nb * self.ncell, dtype=torch.int64)
+            q = i + self.nrows * ((self.start_row + i) % self.nrows)
+            cells = self.ring[q % self.cap].view(nb, self.ncell)
         first = self.block_id
         self.out_ptr += nb * self.ncell
         self.block_id += nb

@@ -269,6 +269,69 @@ def learn_bench(args, dev, steps, eval_blocks, ebn0_points=(1.0, 1.5, 2.0, 2.5, 
             "fer_block_errors": fer}
 
 
+def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4):
+    """The LDM chain fed as the upstream receives it (lab/m44_ldm.py:1187-1304):
+    the combined two-layer cells go through a convolutional time interleaver
+    (Nrows 1024), are cut into frames of 1,216,939 cells and pushed one frame
+    at a time into the device ring; every ``every`` frames the complete FEC
+    Blocks are de-interleaved on the device and decoded by the two-layer
+    chain.  Real time = decoded cells / the frame cell rate."""
+    import torch
+    from atsc3_amd.ldm import LdmPipeline, LdmStream
+    from atsc3_amd.ti import CtiStreamDev
+    from atsc3_amd import synth
+    pipe = LdmPipeline(device=dev, msg_dtype=fmt)
+    stream = LdmStream(pipe, 64, args.seed + 29)
+    plp, ncell, nframes = LDM_FRAME_CELLS, pipe.ncell_c, args.ldm_frames
+    c0, s0 = 1234, 345
+    total = nframes * plp
+    nblk = (total - c0) // ncell
+    n0 = synth.n0_of(args.ldm_esn0_cti)
+    y, ic, ie = stream.batch(nblk, n0, start=3 * 10 ** 5)
+    pre = torch.zeros(total, dtype=torch.complex64, device=dev)
+    pre[c0:c0 + nblk * ncell] = y.reshape(-1)
+    i = torch.arange(total, device=dev)
+    q = i + nrows * ((s0 + i) % nrows)
+    ok = q < total
+    tx = torch.zeros(total, dtype=torch.complex64, device=dev)
+    tx[q[ok]] = pre[i[ok]]
+    frames = tx.view(nframes, plp)
+    del pre, i, q, ok, y
+    cti = CtiStreamDev(nrows, ncell, plp, cap_frames=every + 3, device=dev)
+    cti.reset(0, s0, c0 + nrows * ((s0 + c0) % nrows))
+    ic_h, ie_h = ic, ie.view(nblk, pipe.ratio)
+    done, ec, ee = 0, 0, 0
+    pipe.run(stream.batch(2, n0)[0], n0)             # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(nframes):
+        cti.push(f, frames[f])
+        if (f + 1) % every and f + 1 < nframes:
+            continue
+        first, cells = cti.take_blocks()
+        nb = cells.shape[0]
+        if nb == 0:
+            continue
+        out = pipe.run(cells, n0)
+        sel = ic_h[first:first + nb]
+        ec += (out["core"][0] != stream.cw_c_dev.index_select(0, sel)).any(1).sum()
+        esel = ie_h[first:first + nb].reshape(-1)
+        ee += (out["enh"][0] != stream.cw_e_dev.index_select(0, esel)).any(1).sum()
+        done += nb
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    rt = done * ncell / plp * FRAME_SEC / (ms / 1e3)
+    res = {"messages": fmt, "nrows": nrows, "frames_pushed": nframes, "take_every": every,
+           "core_blocks_decoded": done, "enh_blocks_decoded": done * pipe.ratio,
+           "esn0_db": args.ldm_esn0_cti, "core_errors": int(ec), "enh_errors": int(ee),
+           "ms_total": round(ms, 2), "realtime_factor": round(rt, 2),
+           "zero_filled_frames": int(cti.stats["zero_filled_frames"]),
+           "overruns": int(cti.stats["buffer_overrun"])}
+    log(f"[ldm] CTI stream {fmt}: {done} core blocks over {nframes} frames in {ms:.1f} ms = "
+        f"{rt:.1f}x real time; errors core {int(ec)}, enhanced {int(ee)}")
+    return res
+
+
 def rescue_bench(args, dev, rate="10/15", snr=2.8, nblk=64, max_blocks=8, iters=100):
     """Near-threshold batch of one 64800-bit code: the alpha ladder inside the
     batched kernel (against one host-synchronised decode per rung), then the
@@ -368,6 +431,8 @@ def main():
     ap.add_argument("--no-rescue", action="store_true",
                     help="skip the ladder + sum-product rescue summary")
     ap.add_argument("--ldm-esn0", type=float, default=LDM_ESN0)
+    ap.add_argument("--ldm-esn0-cti", type=float, default=21.0,
+                    help="Es/N0 of the CTI-streamed LDM leg (both layers decode)")
     ap.add_argument("--no-ldm", action="store_true",
                     help="skip the LDM summary in the live-frame line")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -394,6 +459,7 @@ def main():
         del a, b
     if args.workload == "ldm":
         res = ldm_bench(args, dev, (args.msg,))[args.msg]
+        res["cti_stream"] = ldm_cti_bench(args, dev, args.msg)
         print(json.dumps({
             "metric": "LDM frame real-time factor (core + enhanced demap, decode, SIC)",
             "value": res["realtime_factor"], "unit": "x real time", "n_gpus": 1,
@@ -683,6 +749,8 @@ def main():
     if not args.no_ldm:
         torch.cuda.empty_cache()
         ldm = ldm_bench(args, dev)
+        torch.cuda.empty_cache()
+        ldm["cti_stream"] = ldm_cti_bench(args, dev)
     rescue = None
     if not args.no_rescue:
         torch.cuda.empty_cache()

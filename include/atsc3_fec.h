@@ -223,6 +223,12 @@ typedef struct atsc3_gather {
     int64_t first, ring;
 } atsc3_gather;
 
+/* The gathered cells themselves: out [nblk][ncell] complex64 = the cells
+ * atsc3_demap_gather_dev would read (for consumers that need the
+ * de-interleaved cells, e.g. the LDM cancellation). */
+int32_t atsc3_gather_cells_dev(const float *pool, const atsc3_gather *gather,
+                               int32_t nblk, int32_t ncell, float *out,
+                               void *stream);
 int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_csi,
                                const atsc3_gather *gather,
                                const double *sigma2, int32_t nblk,
