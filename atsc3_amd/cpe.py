@@ -151,8 +151,10 @@ class WeightedDemap:
         self.nblk, self.ncell = part.shape
         if self.ncell != demapper.ncell:
             raise ValueError("part table does not match the demapper's block size")
-        self.nparts = int(part.max()) + 1 if nparts is None else int(nparts)
-        if part.min() < 0 or part.max() >= self.nparts:
+        if nparts is None:
+            nparts = int(part.max()) + 1 if part.size else 1
+        self.nparts = int(nparts)
+        if part.size and (part.min() < 0 or part.max() >= self.nparts):
             raise ValueError("part ids outside 0 .. nparts-1")
         flat = part.reshape(-1)
         order = np.argsort(flat, kind="stable")
@@ -167,7 +169,7 @@ class WeightedDemap:
             if gi.size != flat.size:
                 raise ValueError("one gather index per block cell")
             self.idx = torch.from_numpy(gi.astype(np.int32)).to(dev)
-            self.idx_end = int(gi.max()) + 1
+            self.idx_end = int(gi.max()) + 1 if gi.size else 0
         self.d2 = torch.empty(flat.size, dtype=torch.float32, device=dev)
         self.ws = torch.empty(max(_lib.lib().atsc3_demap_ws_doubles(self.nblk, self.ncell), 1),
                               dtype=torch.float64, device=dev)
@@ -179,6 +181,8 @@ class WeightedDemap:
         nb = self.dm.nb
         llr = torch.empty((self.nblk, self.ncell * nb), dtype=torch.float32, device=dev)
         s2 = torch.empty(self.nparts, dtype=torch.float32, device=dev)
+        if self.nblk == 0:
+            return llr, s2
         spec = None
         if self.idx is not None:
             spec = GatherSpec(GATHER_TABLE, 0, 0, 0, self.idx.data_ptr(), 0, 0)

@@ -233,3 +233,9 @@ def test_empty_inputs(device):
     out, gains = cpe.run(torch.zeros(0, dtype=torch.complex64, device=device))
     assert out.numel() == 0 and gains.numel() == 0
     assert frame_snr_db(torch.zeros(10, dtype=torch.complex64, device=device), 5) is None
+    from atsc3_amd.demap import Demapper
+    from atsc3_amd.cpe import WeightedDemap
+    dm = Demapper(16200, "64QAM", "11/15", device)
+    wd = WeightedDemap(dm, np.zeros((0, dm.ncell), np.int32), None, None, device)
+    llr, s2 = wd.run(torch.zeros((0, dm.ncell), dtype=torch.complex64, device=device))
+    assert llr.shape == (0, 16200)
