@@ -82,9 +82,20 @@ class HtiGather:
     """Device index table of one PLP layout; demaps every FEC Block of the
     PLP's cells through it."""
 
-    def __init__(self, plp_size, nti, nrows, ncols, n_virtual=0, device="cuda"):
-        self.plp_size = int(plp_size)
-        idx = hti_index(plp_size, nti, nrows, ncols, n_virtual)
+    def __init__(self, plp_size, nti, nrows, ncols, n_virtual=0, device="cuda", frames=1,
+                 stride=None):
+        """frames > 1: the same layout in ``frames`` consecutive frames of
+        ``stride`` cells (the PLP at the start of each), one table."""
+        idx = hti_index(plp_size, nti, nrows, ncols, n_virtual).astype(np.int64)
+        if frames > 1:
+            stride = int(stride)
+            if stride < plp_size:
+                raise ValueError("frame stride shorter than the PLP")
+            idx = (np.arange(frames)[:, None, None] * stride + idx[None]).reshape(-1, nrows)
+            if idx.max() >= 2 ** 31:
+                raise ValueError("frames too large for 32-bit cell indices")
+        self.plp_size = int(plp_size) if frames == 1 else (frames - 1) * stride + int(plp_size)
+        idx = idx.astype(np.int32)
         self.nblk, self.ncell = idx.shape
         self.idx = torch.from_numpy(np.ascontiguousarray(idx.reshape(-1))).to(device)
         self.spec = GatherSpec(GATHER_TABLE, 0, 0, 0, self.idx.data_ptr(), 0, 0)

@@ -332,6 +332,108 @@ def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4):
     return res
 
 
+def frame_chain_bench(args, dev, esn0=19.0, frames=(1, 8)):
+    """One RF33-like frame through the whole device chain, per frame: CPE of
+    the 211,472-cell pool, the 74-block 64QAM 11/15 PLP demapped through the
+    hybrid time de-interleaver's gather, the alpha ladder in the batched
+    decoder, the sum-product rescue, BCH + descrambling; the QPSK 2/15 PLP
+    demapped and decoded beside it.  Synthetic frames (random information
+    bits, so the BCH verdict is not the check here; block errors are counted
+    against the sent codewords), per-symbol gain and AWGN.  Timed for one
+    frame per step (latency) and ``frames[-1]`` frames per step."""
+    import torch
+    from atsc3_amd import bicm, ldpc, synth
+    from atsc3_amd.baseband import Baseband
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.cpe import Cpe, dummy_values
+    from atsc3_amd.demap import Demapper
+    from atsc3_amd.rescue import rescue_dev
+    from atsc3_amd.ti import HtiGather, hti_index
+    rng = np.random.default_rng(args.seed + 41)
+    code0, code16 = get_code("11/15", 16200), get_code("2/15", 16200)
+    n0c, dstart = 74 * 2700, 74 * 2700 + 8100
+    npool = dstart + 3572
+    dv = dummy_values(npool, dstart)
+    region = np.full(npool, 2, np.int8)
+    region[:n0c] = 0
+    region[n0c:dstart] = 1
+    symbol_of = np.concatenate([np.full(2700, -1), np.arange(npool - 2700) // 5800])
+    idx = hti_index(n0c, 1, 2700, 74)
+    out = {}
+    for nf in frames:
+        cw0 = synth.codewords("11/15", 16200, 74 * nf, args.seed + 43 + nf)
+        cw16 = synth.codewords("2/15", 16200, nf, args.seed + 44 + nf)
+        x0 = synth.modulate(cw0, 16200, "64QAM", "11/15").reshape(nf, 74, 2700)
+        x16 = synth.modulate(cw16, 16200, "QPSK", "2/15")
+        pools = np.empty((nf, npool), np.complex128)
+        for f in range(nf):
+            tx = np.empty(n0c, np.complex128)
+            tx[idx.reshape(-1)] = x0[f].reshape(-1)       # interleave = scatter through the map
+            pools[f] = np.concatenate([tx, x16[f], dv[dstart:]])
+        syms = np.unique(symbol_of)
+        g = ((1 + rng.uniform(-0.1, 0.1, (nf, len(syms))))
+             * np.exp(1j * rng.uniform(-0.1, 0.1, (nf, len(syms)))))
+        pos = np.searchsorted(syms, symbol_of)
+        nz = (rng.standard_normal(pools.shape) + 1j * rng.standard_normal(pools.shape))
+        y = (g[:, pos] * pools + nz * np.sqrt(synth.n0_of(esn0) / 2)).astype(np.complex64)
+        # nf frames as one pool of nf * npool cells (symbols numbered per frame)
+        sym_all = (np.arange(nf)[:, None] * (syms.max() + 2) + symbol_of[None, :] + 1).reshape(-1)
+        cpe = Cpe(sym_all, np.tile(region, nf),
+                  [bicm.points("64QAM", "11/15"), bicm.points("QPSK", "2/15")],
+                  np.tile(dv, nf), dev)
+        yd = torch.from_numpy(y.reshape(-1)).to(dev)
+        hti = HtiGather(n0c, 1, 2700, 74, 0, dev, frames=nf, stride=npool)
+        dm0 = Demapper(16200, "64QAM", "11/15", dev)
+        dm16 = Demapper(16200, "QPSK", "2/15", dev)
+        bb = Baseband("11/15", 16200, dev)
+        sel16 = torch.from_numpy((np.arange(nf)[:, None] * npool + n0c
+                                  + np.arange(8100)[None, :]).reshape(-1)).to(dev)
+        cur = torch.cuda.current_stream(dev)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+
+        def step():
+            ev[0].record(cur)
+            z, _ = cpe.run(yd)
+            ev[1].record(cur)
+            llr0, _ = hti.demap(dm0, z)
+            ev[2].record(cur)
+            bits, conv, its, bad = ldpc.ladder_dev(llr0, code0, (1.0, 0.85, 0.75), 50,
+                                                   engine="batch")
+            ev[3].record(cur)
+            rescue_dev(llr0, bits, conv, its, bad, code0, budget_s=0.12)
+            ev[4].record(cur)
+            bb.run(bits)
+            llr16, _ = dm16.demap_dev(z.index_select(0, sel16).view(nf, 8100))
+            r16 = ldpc.decode_dev(llr16, code16, 50, 0.75)
+            ev[5].record(cur)
+            return bits, r16
+        step()
+        torch.cuda.synchronize()
+        acc = np.zeros(5)
+        for _ in range(args.steps):
+            bits, r16 = step()
+            torch.cuda.synchronize()
+            acc += [ev[k].elapsed_time(ev[k + 1]) for k in range(5)]
+        acc /= args.steps
+        e0 = int((bits.cpu() != torch.from_numpy(cw0)).any(1).sum())
+        e16 = int((r16[0].cpu() != torch.from_numpy(cw16)).any(1).sum())
+        tot = float(acc.sum())
+        out[f"{nf}_frames"] = {
+            "ms_per_step": round(tot, 3), "ms_per_frame": round(tot / nf, 3),
+            "stages_ms": {k: round(float(v), 3) for k, v in
+                          zip(("cpe", "hti_demap", "ladder_decode", "rescue",
+                               "baseband_plp16"), acc)},
+            "realtime_factor": round(nf * FRAME_SEC / (tot / 1e3), 1),
+            "block_errors_plp0": e0, "blocks_plp0": 74 * nf, "block_errors_plp16": e16}
+        log(f"[bench] frame chain {nf} frame(s)/step at {esn0:g} dB: {tot:.3f} ms "
+            f"({', '.join(f'{k} {v:.3f}' for k, v in out[f'{nf}_frames']['stages_ms'].items())}); "
+            f"errors {e0}/{74 * nf} + {e16}/{nf}")
+        del cpe, yd, hti
+        torch.cuda.empty_cache()
+    out["esn0_db"] = esn0
+    return out
+
+
 def rescue_bench(args, dev, rate="10/15", snr=2.8, nblk=64, max_blocks=8, iters=100):
     """Near-threshold batch of one 64800-bit code: the alpha ladder inside the
     batched kernel (against one host-synchronised decode per rung), then the
@@ -428,6 +530,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the uneven-frame summary in the live-frame line")
     ap.add_argument("--ldm-frames", type=int, default=50)
+    ap.add_argument("--no-frame-chain", action="store_true",
+                    help="skip the whole-chain per-frame timing")
     ap.add_argument("--no-rescue", action="store_true",
                     help="skip the ladder + sum-product rescue summary")
     ap.add_argument("--ldm-esn0", type=float, default=LDM_ESN0)
@@ -755,6 +859,10 @@ def main():
     if not args.no_rescue:
         torch.cuda.empty_cache()
         rescue = rescue_bench(args, dev)
+    frame_chain = None
+    if not args.no_frame_chain:
+        torch.cuda.empty_cache()
+        frame_chain = frame_chain_bench(args, dev)
 
     info_gbps = value * code.K / 1e9
     rt_factor = value / (FRAME_CELLS / ncell / FRAME_SEC)
@@ -777,7 +885,7 @@ def main():
                    "demap_ms": round(demap_ms, 4), "decode_ms": round(decode_ms, 4),
                    "fer_sweep": {f"{k:g}": v for k, v in fer.items()},
                    "forms": forms, "mixed": mixed, "ldm": ldm, "learn": learn,
-                   "rescue": rescue},
+                   "rescue": rescue, "frame_chain": frame_chain},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

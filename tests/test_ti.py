@@ -66,6 +66,14 @@ def test_product_hti_index_equals_permutation_inverse(plp, nti, nr, nc, nv):
     assert got.dtype == np.int32 and np.array_equal(got, want)
 
 
+def test_hti_gather_table_over_several_frames():
+    g = ti.HtiGather(3 * 37 * 2700, 3, 2700, 37, 0, device="cpu", frames=4, stride=400000)
+    one = OT.hti_frame_index(3 * 37 * 2700, 3, 2700, 37, 0)
+    want = np.concatenate([f * 400000 + one for f in range(4)])
+    assert g.nblk == 4 * 111 and np.array_equal(g.idx.numpy().reshape(g.nblk, 2700), want)
+    assert g.plp_size == 3 * 400000 + 3 * 37 * 2700
+
+
 def test_cti_structure_and_round_trip():
     for nrows, start in ((8, 5), (512, 191), (1024, 344)):
         n = 3 * nrows * nrows
