@@ -126,13 +126,19 @@ def tables_for(code, device):
 
 
 def rescue_dev(llr, bits, conv, its, bad, code, near_max=1500, max_blocks=8, iters=100,
-               budget_s=0.12, row_off=None, blk_code=None, code_id=0):
+               budget_s=0.12, row_off=None, blk_code=None, code_id=0, skip_if_idle=False):
     """Sum-product rescue of the near misses of a decoded batch, in place on
     the device tensors (bits (B, n) or flat with ``row_off``, conv u8, its
     i32, bad i32).  With ``blk_code`` (a mixed batch) only the rows of code
     ``code_id`` (= ``code``) are offered.  Returns device (rows [max_blocks]
     i32, count [1] i32, sp_conv [max_blocks] u8, sp_shed [max_blocks] u8):
-    the rows offered, how many, and which converged / ran out of time."""
+    the rows offered, how many, and which converged / ran out of time.
+
+    With ``skip_if_idle`` the offered count is read on the host (one small
+    synchronisation, as the upstream's host-side selection implies) and the
+    decode is not launched when nothing is offered; otherwise every step is
+    enqueued without synchronisation (an idle decode's launches return at
+    once)."""
     dev = llr.device
     nblk = conv.numel()
     rows = torch.zeros(max_blocks, dtype=torch.int32, device=dev)
@@ -145,6 +151,9 @@ def rescue_dev(llr, bits, conv, its, bad, code, near_max=1500, max_blocks=8, ite
     _lib.call("atsc3_sp_select_dev", conv.data_ptr(), bad.data_ptr(),
               None if blk_code is None else blk_code.data_ptr(), int(code_id), nblk,
               int(near_max), int(max_blocks), rows.data_ptr(), count.data_ptr(), st)
+    if skip_if_idle and int(count.item()) == 0:
+        z = torch.zeros(max_blocks, dtype=torch.uint8, device=dev)
+        return rows, count, z, z.clone()
     ro = None if row_off is None else row_off
     b2, c2, i2, _, s2 = sp.decode_rows(llr, max_blocks, rows, count, ro, iters, budget_s)
     _lib.call("atsc3_sp_merge_dev", rows.data_ptr(), count.data_ptr(), max_blocks, sp.n,

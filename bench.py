@@ -400,7 +400,7 @@ def frame_chain_bench(args, dev, esn0=19.0, frames=(1, 8)):
             bits, conv, its, bad = ldpc.ladder_dev(llr0, code0, (1.0, 0.85, 0.75), 50,
                                                    engine="batch")
             ev[3].record(cur)
-            rescue_dev(llr0, bits, conv, its, bad, code0, budget_s=0.12)
+            rescue_dev(llr0, bits, conv, its, bad, code0, budget_s=0.12, skip_if_idle=True)
             ev[4].record(cur)
             bb.run(bits)
             llr16, _ = dm16.demap_dev(z.index_select(0, sel16).view(nf, 8100))

@@ -314,6 +314,9 @@ def test_rescue_with_nothing_to_offer(device):
     rows, count, _, _ = rescue_dev(t, bits, conv, its, bad, code, budget_s=None)
     torch.cuda.synchronize()
     assert int(count.item()) == 0 and bool(conv.all())
+    rows, count, c2, _ = rescue_dev(t, bits, conv, its, bad, code, budget_s=None,
+                                    skip_if_idle=True)
+    assert int(count.item()) == 0 and not bool(c2.any())
     for x, y in zip(before, (bits, conv, its, bad)):
         assert torch.equal(x, y)
     e = torch.empty((0, 16200), dtype=torch.float32, device=device)
