@@ -21,6 +21,7 @@
 #include "fec_common.h"
 
 #define CPE_WG 256
+#define CPE_SYM_WG 1024     // one symbol per workgroup: wide for single-frame latency
 #define CPE_MAX_ITERS 8
 #define CPE_MAX_POINTS 4096
 
@@ -63,17 +64,17 @@ __global__ __launch_bounds__(CPE_WG) void cpe_power(const float2 *__restrict__ p
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(CPE_WG) void cpe_symbols(
+__global__ __launch_bounds__(CPE_SYM_WG) void cpe_symbols(
     const float2 *__restrict__ pool, int npool, const double *__restrict__ partial, int nparts,
     const int32_t *__restrict__ cells, const int32_t *__restrict__ seg,
     const int32_t *__restrict__ ecells, const int32_t *__restrict__ eseg,
     const uint8_t *__restrict__ region, const double2 *__restrict__ pts0, int m0,
     const double2 *__restrict__ pts1, int m1, const float2 *__restrict__ dummy, int iters,
     float2 *__restrict__ out, double *__restrict__ gains) {
-    __shared__ double red[CPE_WG / 64];
+    __shared__ double red[CPE_SYM_WG / 64];
     __shared__ C64 cs[CPE_MAX_ITERS];
     const int sym = blockIdx.x;
-    // the pool power: one partial per thread (nparts <= CPE_WG), summed in
+    // the pool power: one partial per thread (nparts <= CPE_WG <= CPE_SYM_WG), summed in
     // the same fixed tree order by every symbol workgroup
     const double tot =
         wg_sum_f64((int)threadIdx.x < nparts ? partial[threadIdx.x] : 0.0, red);
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(CPE_WG) void cpe_symbols(
         const int32_t *list = it == iters ? cells : ecells;
         const int lo = it == iters ? seg[sym] : eseg[sym];
         const int hi = it == iters ? seg[sym + 1] : eseg[sym + 1];
-        for (int k = lo + (int)threadIdx.x; k < hi; k += CPE_WG) {
+        for (int k = lo + (int)threadIdx.x; k < hi; k += CPE_SYM_WG) {
             const int c = list[k];
             const float2 p = pool[c];
             C64 z = {(double)p.x / rms, (double)p.y / rms};
@@ -223,7 +224,7 @@ extern "C" int32_t atsc3_cpe_dev(const float *pool, int32_t npool, const int32_t
     const int np = atsc3_cpe_ws_doubles(npool);
     hipLaunchKernelGGL(cpe_power, dim3(np), dim3(CPE_WG), 0, st,
                        reinterpret_cast<const float2 *>(pool), npool, ws);
-    hipLaunchKernelGGL(cpe_symbols, dim3(nsym), dim3(CPE_WG), 0, st,
+    hipLaunchKernelGGL(cpe_symbols, dim3(nsym), dim3(CPE_SYM_WG), 0, st,
                        reinterpret_cast<const float2 *>(pool), npool, ws, np, cells, seg,
                        est_cells, est_seg, region,
                        reinterpret_cast<const double2 *>(pts0), m0,
