@@ -84,7 +84,9 @@ class SumProduct:
     def decode_rows(self, llr, rows, sel=None, nsel=None, row_off=None, iters=100,
                     budget_s=None):
         """Enqueue the decode of ``rows`` call rows; returns device
-        (bits (rows, n) u8, conv u8, its i32, bad i32, shed u8)."""
+        (bits (rows, n) u8, conv u8, its i32, bad i32, shed u8).  Calls with
+        the same row count share this object's workspace: enqueue them on
+        one stream (calls on different streams would overlap on it)."""
         dev = self.device
         bits = torch.empty((rows, self.n), dtype=torch.uint8, device=dev)
         conv = torch.empty(rows, dtype=torch.uint8, device=dev)
