@@ -16,10 +16,14 @@ static_assert(sizeof(CodeTab) == 96, "code table descriptor layout");
 
 enum { H_ABORT = 2, H_QNEXT, H_NVAR, H_NCHK, H_NFREE, H_NPEND, H_DONE, H_ROUNDS, H_WORDS = 16 };
 // Grid-barrier words, each on its own 128-byte line (32 ints) of the barrier
-// region: the global arrival count, the generation, then one arrival count
-// per workgroup group (workgroup w belongs to group w mod BATCH_BAR_GROUPS).
-#define BATCH_BAR_GROUPS 16
-enum { BAR_ARRIVE = 0, BAR_GEN = 32, BAR_GROUP = 64, BAR_WORDS = 64 + 32 * BATCH_BAR_GROUPS };
+// region: the global arrival count, the generation, one arrival count per
+// XCD (a workgroup's group is the XCD it runs on, read from the hardware),
+// then the launch-time census: its arrival count, its release flag and the
+// number of workgroups found on each XCD.
+#define BATCH_BAR_GROUPS 16            // XCC_ID is a 4-bit field
+enum { BAR_ARRIVE = 0, BAR_GEN = 32, BAR_GROUP = 64,
+       BAR_CENSUS_ARRIVE = 64 + 32 * BATCH_BAR_GROUPS, BAR_CENSUS_GO = BAR_CENSUS_ARRIVE + 32,
+       BAR_CENSUS = BAR_CENSUS_GO + 32, BAR_WORDS = BAR_CENSUS + 32 };
 // Var-list entry modes.  RETIRE_INIT: the codeword failed at its ladder rung
 // and starts the next one in the same slot (result kept first if taken).
 enum { M_CONT = 0, M_RETIRE = 1, M_INIT = 2, M_RETIRE_INIT = 3 };

@@ -194,42 +194,98 @@ __device__ void compact(const BatchArgs &b) {
 // s_waitcnt vmcnt(0), expcnt and lgkmcnt left unconstrained.
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// The XCD (accelerator complex die) this wave runs on: HW_REG_XCC_ID
+// (hardware register 20), bits [3:0].  Each XCD has its own L2.
+__device__ __forceinline__ unsigned xcc_id() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & (BATCH_BAR_GROUPS - 1);
+}
+
+// Where this workgroup arrives in the grid barrier: its XCD's group, the
+// group's size and the number of non-empty groups, from a census taken once
+// at launch (every workgroup counts itself on its XCD, then a counted
+// one-time barrier).  False when the census wait timed out.
+struct BarGroup {
+    unsigned g, gsize, ng;
+};
+
+__device__ bool bar_census(const BatchArgs &b, BarGroup &bg, int *s_flag) {
+    unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
+    unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
+    __shared__ unsigned s_cnt[BATCH_BAR_GROUPS];
+    const unsigned g = xcc_id();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar + BAR_CENSUS + g, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned old = __hip_atomic_fetch_add(bar + BAR_CENSUS_ARRIVE, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == gridDim.x) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_store(bar + BAR_CENSUS_GO, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(bar + BAR_CENSUS_GO, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                if (__hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                if (++spins >= BATCH_SPIN_LIMIT) {
+                    __hip_atomic_store(h + H_ABORT, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        for (int q = 0; q < BATCH_BAR_GROUPS; ++q)
+            s_cnt[q] = __hip_atomic_load(bar + BAR_CENSUS + q, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        s_flag[1] = __hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    unsigned ng = 0;
+    for (int q = 0; q < BATCH_BAR_GROUPS; ++q) ng += s_cnt[q] ? 1u : 0u;
+    bg.g = (unsigned)uni((int)g);
+    bg.gsize = (unsigned)uni((int)s_cnt[g]);
+    bg.ng = (unsigned)uni((int)ng);
+    const bool ok = s_flag[1] == 0;
+    __syncthreads();
+    return ok;
+}
+
 // Grid barrier number `epoch` (1, 2, ...); the last workgroup to arrive runs
 // the compaction first when asked.  Returns false when the abort word is set.
 //
 // Visibility across XCDs (each has its own L2): every wave first waits for
-// its own stores to complete at its L2; after the workgroup barrier ONE wave
-// issues the agent-scope release (an L2 write-back, which covers the whole
-// workgroup's stores since they are all in that L2) before arriving, and
-// after the release of the barrier ONE wave issues the agent-scope acquire
-// (L2/L1 invalidation, the workgroup's waves share the CU's L1) before the
-// workgroup proceeds.  One write-back and one invalidation per workgroup
-// instead of one per wave: the barrier cost scales with their number.
-//
-// Arrival is two-level: a workgroup counts itself in its group's word and
-// the last of a group counts the group in the global word, so at most
-// BATCH_BAR_GROUPS atomics meet on any one line (512 same-line atomics
-// serialise); waiters poll the generation word, on a line of its own.
-__device__ bool grid_barrier(const BatchArgs &b, unsigned epoch, bool do_compact, int *s_flag) {
+// its own stores to complete at its XCD's L2, then one wave per workgroup
+// arrives in the XCD's group count.  The last workgroup of an XCD to arrive
+// issues the agent-scope release -- an L2 write-back, which covers the stores
+// of every workgroup of that XCD, all of them complete in that L2 before
+// their arrivals -- and counts the XCD in the global word; the last XCD's
+// last workgroup acquires, runs the compaction, releases and bumps the
+// generation.  After the generation changes ONE wave per workgroup issues the
+// agent-scope acquire (L2/L1 invalidation, the workgroup's waves share the
+// CU's L1).  One write-back per XCD and one invalidation per workgroup: a
+// write-back per workgroup made a one-codeword round 20% slower.  Counts on
+// separate 128-byte lines (a flat counter serialised 512 same-line atomics);
+// waiters poll the generation word on a line of its own.
+__device__ bool grid_barrier(const BatchArgs &b, const BarGroup &bg, unsigned epoch,
+                             bool do_compact, int *s_flag) {
     unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
     unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
     wait_vmem();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned G = gridDim.x;
-        const unsigned ng = G < BATCH_BAR_GROUPS ? G : BATCH_BAR_GROUPS;
-        const unsigned g = blockIdx.x % ng;
-        const unsigned gsize = (G - g + ng - 1) / ng;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned old = __hip_atomic_fetch_add(bar + BAR_GROUP + 32 * g, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned old = __hip_atomic_fetch_add(bar + BAR_GROUP + 32 * bg.g, 1u,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int last = 0;
-        if (old + 1u == epoch * gsize) {
-            // the group's arrivals happen-before the group's count
+        if (old + 1u == epoch * bg.gsize) {
+            // every workgroup of this XCD has arrived: write its L2 back once
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             const unsigned o2 = __hip_atomic_fetch_add(bar + BAR_ARRIVE, 1u, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT);
-            last = o2 + 1u == epoch * ng;
+            last = o2 + 1u == epoch * bg.ng;
             if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         s_flag[0] = last;
@@ -456,13 +512,15 @@ __global__ __launch_bounds__(ATSC3_WG) __attribute__((amdgpu_waves_per_eu(4)))
 void batch_kernel(BatchArgs b) {
     __shared__ int s_k[1];
     __shared__ int s_flag[2];
+    BarGroup bg;
+    if (!bar_census(b, bg, s_flag)) return;
     unsigned epoch = 0;
     for (;;) {
         var_phase<FMT, VMAX>(b, s_k);
-        if (!grid_barrier(b, ++epoch, false, s_flag)) return;
+        if (!grid_barrier(b, bg, ++epoch, false, s_flag)) return;
         if (uni(ldw(b.ctl + H_DONE))) return;
         check_phase<FMT, 16>(b, s_k);
-        if (!grid_barrier(b, ++epoch, true, s_flag)) return;
+        if (!grid_barrier(b, bg, ++epoch, true, s_flag)) return;
     }
 }
 
