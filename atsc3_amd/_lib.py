@@ -84,10 +84,23 @@ _SIGS = {
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_resident_ws_bytes": [c_i32, c_i32],
+    "atsc3_minsum_resident_grid": [],
+    "atsc3_minsum_resident_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
+                                  c_i32, c_i32, c_f32, c_i32, c_vp, ctypes.c_int64, c_vp, c_vp,
+                                  c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
 }
+
+# functions whose C return type is not int32_t
+_RESTYPES = {"atsc3_minsum_resident_ws_bytes": ctypes.c_int64}
+
+
+def restype_of(name):
+    return _RESTYPES.get(name, c_i32)
+
 
 ERRORS = {-1: "check degree exceeds the supported maximum",
           -2: "inconsistent sizes or NULL pointers",
@@ -113,7 +126,7 @@ def lib():
             L = ctypes.CDLL(LIB_PATH)
             for name, args in _SIGS.items():
                 f = getattr(L, name)
-                f.restype = c_i32
+                f.restype = restype_of(name)
                 f.argtypes = args
             if L.ldpc_kernel_abi() != 1:
                 raise RuntimeError("libatsc3fec ABI mismatch")
@@ -124,7 +137,8 @@ def lib():
 SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
-                "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles")
+                "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
+                "atsc3_minsum_resident_grid")
 
 
 def call(name, *args):

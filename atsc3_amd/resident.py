@@ -1,0 +1,252 @@
+"""Resident FP16 min-sum: one workgroup per codeword, posteriors in LDS.
+
+``atsc3_minsum_resident_dev`` decodes a queue of codewords of mixed A/322
+codes with FP16 check-to-variable messages (each check's messages kept as
+one compressed record: FP16 alpha*min1 and alpha*min2, the edge taking min2,
+the signs) and the codeword's variable posteriors held in the workgroup's
+LDS as FP16.  The sums are FP32 in the reference's order; the posterior is
+rounded to FP16 where it is stored.  It stands where the upstream decodes a
+frame's blocks with per-code sub-batched min-sum calls (lab/m9_fast.py:
+843-925, lab/m44_ldm.py:2497-2576 over lab/ldpc_kernel.c:65-146), as the
+FP16 message form of that decode.
+
+``ResidentPlan`` owns the device buffers of one batch composition, so a
+decode enqueues two launches and can be captured in a HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ldpc import _stream_ptr
+
+Z = 360
+MAX_N = 64800
+MAX_DC = 89
+MAX_DV = 32
+
+
+_TABLES = ("grp_c", "edge_c", "grp_v", "edge_v", "c_steps", "c_tasks", "v_steps", "v_tasks")
+
+
+class _ResCode(ctypes.Structure):
+    """struct atsc3_res_code (include/atsc3_fec.h)."""
+    _fields_ = ([(f, ctypes.c_int32) for f in
+                 ("n", "ncg", "nvg", "nbe", "max_dc", "max_dv", "nsc", "nsv")]
+                + [(f, ctypes.c_void_p) for f in _TABLES])
+
+
+assert ctypes.sizeof(_ResCode) == 96
+
+# tasks packed per wave step by degree: check steps of degree <= 16 take 2
+# tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
+CHECK_PACK = ((16, 2),)
+VAR_PACK = ((4, 4), (8, 2), (16, 1))
+FAST_MAX = 16
+
+
+def _pack_tasks(deg, generic, pack):
+    """Tasks (group | c6 << 8: rows c6*64 .. c6*64+63 of a 360-row group) in
+    the order groups sorted by (generic, -degree), cut into steps
+    D | G << 7 | start << 10 | generic << 22 of G equal-degree tasks
+    (G in {1, 2, 4}); a generic step holds one task."""
+    deg = np.asarray(deg, np.int64)
+    generic = np.asarray(generic, bool)
+    order = np.lexsort((np.arange(len(deg)), -deg, generic))
+    tasks = np.array([g | (c6 << 8) for g in order for c6 in range(6)], np.int64)
+    steps, i = [], 0
+    while i < len(tasks):
+        g = tasks[i] & 255
+        d = int(deg[g])
+        if generic[g]:
+            steps.append(min(d, 127) | (1 << 7) | (i << 10) | (1 << 22))
+            i += 1
+            continue
+        gmax = next(gg for lim, gg in pack if d <= lim)
+        run = 1
+        while (run < gmax and i + run < len(tasks) and not generic[tasks[i + run] & 255]
+               and deg[tasks[i + run] & 255] == d):
+            run += 1
+        k = 1 << (run.bit_length() - 1)
+        steps.append(d | (k << 7) | (i << 10))
+        i += k
+    return tasks, np.asarray(steps, np.int64)
+
+
+@functools.lru_cache(maxsize=None)
+def _host_tables(code):
+    """int32 tables of the resident kernel, from the code's QC form.
+
+    grp_c[g]      dc | b0 << 7 | has_missing << 18  (check groups)
+    edge_c[b]     s | t << 9 | (missing j + 1) << 18  (base edges, check-major)
+    grp_v[t]      dv | e0 << 6 | n0 << 17             (variable groups)
+    edge_v[2e]    u | g << 9 | k << 17: entry e of the variable list
+                  (summation order at p = 0) meets check (g, (p + u) % 360)
+                  as that check's edge k
+    edge_v[2e+1]  missing variable position + 1 (0 = full circulant)
+    c_steps, c_tasks, v_steps, v_tasks  the work of each phase (_pack_tasks);
+                  generic: a missing entry, degree > 16, or (variables) an
+                  entry naming a check edge >= 25 (its sign is in word 2)
+    """
+    q = code.qc()
+    cg = q["cg_ptr"].astype(np.int64)
+    be = q["be_word"].astype(np.int64)
+    vg = q["vg_ptr"].astype(np.int64)
+    ve = q["ve_word"].astype(np.int64)
+    nbe = len(be)
+    dc = np.diff(cg)
+    dv = np.diff(vg)
+    miss = (be >> 18) & 1023
+    hasmiss = np.array([int((miss[cg[g]:cg[g + 1]] > 0).any()) for g in range(len(dc))],
+                       np.int64)
+    vb, vu, vreg, vmiss = ve & 2047, (ve >> 11) & 511, (ve >> 20) & 1, (ve >> 21) & 1023
+    g_of_b = np.searchsorted(cg, np.arange(nbe), side="right") - 1
+    k_of_b = np.arange(nbe) - cg[g_of_b]
+    n0 = np.array([int((vreg[vg[t]:vg[t + 1]] == 0).sum()) for t in range(len(dv))], np.int64)
+    if (dc.max() > MAX_DC or dv.max() > MAX_DV or code.n > MAX_N or len(dc) > 255
+            or nbe >= 2048):
+        raise ValueError(f"code {code.rate} {code.n} exceeds the resident kernel's tables")
+    vkmax = np.array([int(k_of_b[vb[vg[t]:vg[t + 1]]].max()) for t in range(len(dv))])
+    vmissg = np.array([int((vmiss[vg[t]:vg[t + 1]] > 0).any()) for t in range(len(dv))])
+    c_tasks, c_steps = _pack_tasks(dc, (hasmiss > 0) | (dc > FAST_MAX), CHECK_PACK)
+    v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25), VAR_PACK)
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    return dict(n=code.n, ncg=len(dc), nvg=len(dv), nbe=nbe, max_dc=int(dc.max()),
+                max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
+                c_steps=i32(c_steps), c_tasks=i32(c_tasks), v_steps=i32(v_steps),
+                v_tasks=i32(v_tasks),
+                grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
+                edge_c=i32(be & ((1 << 28) - 1)),
+                grp_v=i32(dv | (vg[:-1] << 6) | (n0 << 17)),
+                edge_v=i32(np.stack([vu | (g_of_b[vb] << 9) | (k_of_b[vb] << 17), vmiss],
+                                    1).ravel()))
+
+
+class ResidentBank:
+    """Device tables of several codes plus their descriptor array."""
+
+    def __init__(self, codes, device):
+        self.device = torch.device(device)
+        self.codes = list(codes)
+        self.host = [_host_tables(c) for c in self.codes]
+        self.dev = []
+        arr = (_ResCode * len(self.codes))()
+        for i, h in enumerate(self.host):
+            d = {k: torch.from_numpy(h[k]).to(self.device) for k in _TABLES}
+            self.dev.append(d)
+            r = arr[i]
+            for f in ("n", "ncg", "nvg", "nbe", "max_dc", "max_dv", "nsc", "nsv"):
+                setattr(r, f, h[f])
+            for f in _TABLES:
+                setattr(r, f, d[f].data_ptr())
+        raw = np.frombuffer(bytes(arr), np.uint8).copy()
+        self.desc = torch.from_numpy(raw).to(self.device)
+        self.max_checks = max(h["ncg"] * Z for h in self.host)
+        self.max_n = max(h["n"] for h in self.host)
+        self.max_dc = max(h["max_dc"] for h in self.host)
+        self.max_dv = max(h["max_dv"] for h in self.host)
+
+    def __len__(self):
+        return len(self.codes)
+
+
+def default_grid():
+    """One workgroup per compute unit of the current device."""
+    g = _lib.lib().atsc3_minsum_resident_grid()
+    if g <= 0:
+        raise _lib.FecError("atsc3_minsum_resident_grid", g)
+    return g
+
+
+class ResidentPlan:
+    """Buffers and launch parameters for decoding one batch composition.
+
+    block_codes: one LdpcCode per codeword, in queue order; codeword b's
+    LLRs occupy ``llr[off[b] : off[b] + n_b]`` of the flat ``llr`` buffer,
+    its bits the same range of ``bits``."""
+
+    def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None):
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        uniq, idx = [], []
+        for c in block_codes:
+            for i, u in enumerate(uniq):
+                if u is c:
+                    idx.append(i)
+                    break
+            else:
+                uniq.append(c)
+                idx.append(len(uniq) - 1)
+        self.bank = ResidentBank(uniq, self.device)
+        self.nblk = len(idx)
+        self.iters, self.alpha = int(iters), float(np.float32(alpha))
+        ns = np.array([uniq[i].n for i in idx], np.int64)
+        self.off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+        self.total = int(self.off[-1])
+        self.code_idx = np.array(idx, np.int32)
+        g = default_grid() if grid is None else int(grid)
+        if g < 1:
+            raise ValueError("grid must be >= 1")
+        self.grid = max(1, min(g, self.nblk))
+        dev = self.device
+        self.blk_code = torch.from_numpy(self.code_idx).to(dev)
+        self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
+        self.llr = torch.zeros(self.total, dtype=torch.float32, device=dev)
+        self.bits = torch.empty(self.total, dtype=torch.uint8, device=dev)
+        self.conv = torch.empty(self.nblk, dtype=torch.uint8, device=dev)
+        self.its = torch.empty(self.nblk, dtype=torch.int32, device=dev)
+        self.bad = torch.empty(self.nblk, dtype=torch.int32, device=dev)
+        nbytes = _lib.lib().atsc3_minsum_resident_ws_bytes(self.grid, self.bank.max_checks)
+        if nbytes <= 0:
+            raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
+        self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.ws_bytes = int(nbytes)
+
+    def run(self, llr=None):
+        """Enqueue the decode of ``llr`` (flat, or None: self.llr as filled)."""
+        src = self.llr if llr is None else llr
+        if src.numel() != self.total or src.dtype != torch.float32 or not src.is_contiguous():
+            raise ValueError("llr must be a contiguous float32 tensor of the plan's length")
+        b = self.bank
+        _lib.call("atsc3_minsum_resident_dev", src.data_ptr(), self.nblk,
+                  self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
+                  len(b), b.max_checks, b.max_n, b.max_dc, b.max_dv, self.iters, self.alpha,
+                  self.grid, self.ws.data_ptr(), self.ws_bytes, self.bits.data_ptr(),
+                  self.conv.data_ptr(), self.its.data_ptr(), self.bad.data_ptr(),
+                  _stream_ptr(self.device))
+        return self.bits, self.conv, self.its, self.bad
+
+    def block_bits(self, b):
+        return self.bits[int(self.off[b]):int(self.off[b + 1])]
+
+
+_PLANS = {}
+
+
+def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
+    """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad)."""
+    if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
+        raise TypeError("decode_resident_dev needs a CUDA (HIP) tensor")
+    llr = llr.contiguous().float()
+    nblk, n = llr.shape
+    if n != code.n:
+        raise ValueError(f"LLR width {n} != code length {code.n}")
+    if nblk == 0:
+        dev = llr.device
+        return (torch.empty((0, n), dtype=torch.uint8, device=dev),
+                torch.empty(0, dtype=torch.uint8, device=dev),
+                torch.empty(0, dtype=torch.int32, device=dev),
+                torch.empty(0, dtype=torch.int32, device=dev))
+    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(llr.device))
+    plan = _PLANS.get(key)
+    if plan is None or plan.bank.codes[0] is not code:
+        plan = ResidentPlan([code] * nblk, llr.device, iters, alpha, grid)
+        _PLANS.clear()
+        _PLANS[key] = plan
+    bits, conv, its, bad = plan.run(llr.reshape(-1))
+    return bits.view(nblk, n).clone(), conv.clone(), its.clone(), bad.clone()

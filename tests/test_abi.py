@@ -13,7 +13,14 @@ HEADER = os.path.join(ROOT, "include", "atsc3_fec.h")
 def declared():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\bint32_t\s+(\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\bint(?:32|64)_t\s+(\w+)\s*\(", txt)))
+
+
+def return_types():
+    """name -> declared return type ("int32_t" or "int64_t")."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return {n: r for r, n in re.findall(r"\b(int(?:32|64)_t)\s+(\w+)\s*\(", txt)}
 
 
 def prototypes():
@@ -21,7 +28,7 @@ def prototypes():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     out = {}
-    for name, params in re.findall(r"\bint32_t\s+(\w+)\s*\(([^)]*)\)\s*;", txt):
+    for name, params in re.findall(r"\bint(?:32|64)_t\s+(\w+)\s*\(([^)]*)\)\s*;", txt):
         params = " ".join(params.split())
         if params in ("", "void"):
             out[name] = []
@@ -77,6 +84,14 @@ def test_ctypes_signatures_match_prototypes():
     for name, args in L._SIGS.items():
         want = [_ctype_of(t) for t in protos[name]]
         assert list(args) == want, name
+
+
+def test_ctypes_return_types_match_header():
+    import ctypes
+    from atsc3_amd import _lib as L
+    for name, r in return_types().items():
+        want = ctypes.c_int64 if r == "int64_t" else ctypes.c_int32
+        assert L.restype_of(name) is want, name
 
 
 def test_size_queries():

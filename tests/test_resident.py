@@ -1,0 +1,256 @@
+"""Resident FP16 min-sum (atsc3_minsum_resident_dev).
+
+CPU: the numpy oracle of the resident decode (oracle/resident_ref.py), run
+with FP32 storage, reproduces the upstream kernel's own golden outputs bit
+for bit (bits, conv, its, bad) -- every code length, Type A and B, alpha
+0.75/0.85/1.0, ties, +-inf/NaN LLRs; the packed tables the kernel reads
+decode back to the code's QC base graph.
+
+GPU: the kernel equals the oracle with FP16 storage bit for bit on the same
+LLRs (golden-fixture LLRs, plus synthetic blocks of the codes that take the
+generic paths: check degree > 16 and > 25, variable degree > 16, Type A),
+one code per launch and mixed codes in one launch with fewer workgroups
+than codewords; at the live-frame operating point (256-NUC r10/15, Rayleigh,
+21 dB) every block decodes to the transmitted codeword, with mean iterations
+within 3% of the plain FP32 decode; a HIP-graph replay equals the eager run.
+"""
+import numpy as np
+import pytest
+
+from oracle.resident_ref import resident_minsum
+from tests import _fixtures as F
+
+# fixtures fast enough for the CPU suite (the 64800-bit ones at 2-5 s each)
+CPU_FIXTURES = [n for n in F.fixture_names() if n.startswith(("g16k", "s16k"))] + [
+    "g64k_r10_nuc20", "g64k_r4", "g64k_r12"]
+GPU_FIXTURES = ["g16k_r2", "g16k_r6", "g16k_r11", "s16k_r3_infnan", "s16k_r6_ties",
+                "s16k_r4_a100", "s16k_r9_near", "g64k_r10_nuc20", "g64k_r10_nuc16", "g64k_r4"]
+# (rate, n, snr_db): check degree 33 (third record word), 86 (Type A, > 64
+# signs), variable degree 27 and 32 (generic variable path), Type A 2/15
+SYNTH = [("13/15", 64800, 5.5), ("7/15", 64800, 0.8), ("6/15", 64800, 1.5),
+         ("8/15", 16200, 2.0), ("2/15", 64800, -4.0), ("5/15", 64800, 0.0)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_built():
+    F.ensure_oracle_built()
+
+
+@pytest.mark.parametrize("name", CPU_FIXTURES)
+def test_oracle_fp32_storage_reproduces_reference_fixtures(name):
+    code, llr, iters, alpha, want = F.load(name)
+    got = resident_minsum(llr, code, iters, alpha, "float32")
+    F.assert_same(got, want, name)
+
+
+def test_oracle_fp16_storage_converges_to_the_same_codewords():
+    """FP16 storage changes iteration counts at most slightly and decodes the
+    converged blocks of a fixture to the same codewords as FP32."""
+    code, llr, iters, alpha, want = F.load("g16k_r4")
+    bits, conv, its, bad = resident_minsum(llr, code, iters, alpha, "float16")
+    assert np.array_equal(conv, want["conv"])
+    assert np.array_equal(bits[conv], want["bits"][conv])
+    assert np.abs(its.astype(int) - want["its"]).max() <= 2
+
+
+def test_oracle_detects_a_planted_order_change():
+    """Negative control: summing a variable's messages in table order instead
+    of ascending check index is a different FP32 computation; the golden
+    outputs must reject it (the order the oracle follows is checked, not
+    assumed).  The fixtures that see it are the ones that run all 50
+    iterations without converging, where rounding differences accumulate."""
+    missed = 0
+    for name in ("g16k_r6", "s16k_r9_near"):
+        code, llr, iters, alpha, want = F.load(name)
+        got = resident_minsum(llr, code, iters, alpha, "float32", _plant="list_order")
+        missed += not all(np.array_equal(np.asarray(g), want[k]) for g, k in
+                          zip(got, ("bits", "conv", "its", "bad")))
+    assert missed == 2
+
+
+@pytest.mark.parametrize("n,rate", [(64800, "10/15"), (64800, "2/15"), (64800, "7/15"),
+                                    (64800, "13/15"), (16200, "8/15"), (16200, "4/15")])
+def test_packed_tables_decode_to_the_base_graph(n, rate):
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.resident import _host_tables
+    code = get_code(rate, n)
+    q = code.qc()
+    h = _host_tables(code)
+    cg, be = q["cg_ptr"], q["be_word"].astype(np.int64)
+    gc = h["grp_c"].astype(np.int64)
+    assert np.array_equal(gc & 127, np.diff(cg)) and np.array_equal((gc >> 7) & 2047, cg[:-1])
+    ev = h["edge_v"].astype(np.int64).reshape(-1, 2)
+    gv = h["grp_v"].astype(np.int64)
+    seen = np.zeros(len(be), int)
+    for t in range(q["nvg"]):
+        dv, e0 = gv[t] & 63, (gv[t] >> 6) & 2047
+        for e in range(e0, e0 + dv):
+            u, g, k = ev[e, 0] & 511, (ev[e, 0] >> 9) & 255, (ev[e, 0] >> 17) & 127
+            b = cg[g] + k
+            assert b < cg[g + 1]
+            s, tb, miss = be[b] & 511, (be[b] >> 9) & 511, (be[b] >> 18) & 1023
+            assert tb == t and (s + u) % 360 == 0
+            assert ev[e, 1] == (0 if miss == 0 else (miss - 1 + s) % 360 + 1)
+            seen[b] += 1
+    assert (seen == 1).all()
+    # every 64-row task of both phases is covered once, in steps of G tasks
+    # of one degree; fast steps only over full circulants of degree <= 16
+    for steps, tasks, ngrp, deg_of in (("c_steps", "c_tasks", q["ncg"], gc & 127),
+                                       ("v_steps", "v_tasks", q["nvg"], gv & 63)):
+        st = h[steps].astype(np.int64)
+        tk = h[tasks].astype(np.int64)
+        cover = np.zeros(len(tk), int)
+        for w in st:
+            d, g, start, generic = w & 127, (w >> 7) & 7, (w >> 10) & 4095, (w >> 22) & 1
+            assert g in (1, 2, 4) and (not generic or g == 1)
+            cover[start:start + g] += 1
+            grp = tk[start:start + g] & 255
+            if not generic:
+                assert (deg_of[grp] == d).all() and d <= 16
+        assert (cover == 1).all()
+        assert sorted((tk & 255) * 8 + (tk >> 8)) == sorted(
+            t * 8 + c for t in range(ngrp) for c in range(6))
+
+
+# ---------------------------------------------------------------------------
+# device
+# ---------------------------------------------------------------------------
+
+def _synth_llr(rate, n, snr, nblk=3, seed=11):
+    from atsc3_amd import synth
+    from atsc3_amd.codes import get_code
+    cw = synth.codewords(rate, n, nblk, seed)
+    return get_code(rate, n), synth.llr_bpsk_like(cw, snr, seed + 1)
+
+
+def _cases():
+    out = [("fixture", name) for name in GPU_FIXTURES]
+    out += [("synth", f"{r} {n} {s}") for r, n, s in SYNTH]
+    return out
+
+
+def _case(kind, key):
+    if kind == "fixture":
+        code, llr, iters, alpha, _ = F.load(key)
+        return code, llr, iters, alpha
+    r, n, s = key.split()
+    code, llr = _synth_llr(r, int(n), float(s))
+    return code, llr, 50, 0.75
+
+
+_ORACLE = {}
+
+
+def _oracle(kind, key):
+    if (kind, key) not in _ORACLE:
+        code, llr, iters, alpha = _case(kind, key)
+        _ORACLE[(kind, key)] = resident_minsum(llr, code, iters, alpha, "float16")
+    return _ORACLE[(kind, key)]
+
+
+def _check(got, want, tag):
+    bits, conv, its, bad = got
+    assert np.array_equal(np.asarray(conv).astype(bool), want[1]), f"{tag} conv"
+    assert np.array_equal(np.asarray(its), want[2]), f"{tag} its"
+    assert np.array_equal(np.asarray(bad), want[3]), f"{tag} bad"
+    assert np.array_equal(np.asarray(bits), want[0]), f"{tag} bits"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,key", _cases())
+def test_kernel_equals_oracle(device, kind, key):
+    import torch
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr, iters, alpha = _case(kind, key)
+    want = _oracle(kind, key)
+    print(f"[gpu] resident {kind} {key} ({code.rate} {code.n}, {len(llr)} blocks)", flush=True)
+    out = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, alpha)
+    torch.cuda.synchronize()
+    _check([o.cpu().numpy() for o in out], want, f"{kind} {key}")
+
+
+@pytest.mark.gpu
+def test_mixed_codes_one_launch_few_workgroups(device):
+    """Blocks of six codes interleaved in one queue, 3 workgroups: each
+    workgroup decodes many codewords of different codes in turn."""
+    import torch
+    from atsc3_amd.resident import ResidentPlan
+    keys = [("fixture", "g16k_r2"), ("synth", "13/15 64800 5.5"), ("fixture", "s16k_r9_near"),
+            ("synth", "7/15 64800 0.8"), ("fixture", "g64k_r10_nuc20"),
+            ("synth", "8/15 16200 2.0")]
+    blocks = []
+    for kind, key in keys:
+        code, llr, iters, alpha = _case(kind, key)
+        want = _oracle(kind, key)
+        for b in range(len(llr)):
+            blocks.append((code, llr[b], [w[b] for w in want], f"{key}#{b}"))
+    order = np.random.default_rng(5).permutation(len(blocks))
+    blocks = [blocks[i] for i in order]
+    # every case above decodes with iters 50, alpha 0.75
+    for kind, key in keys:
+        _, _, iters, alpha = _case(kind, key)
+        assert (iters, alpha) == (50, 0.75), key
+    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, grid=3)
+    llr = np.concatenate([b[1] for b in blocks]).astype(np.float32)
+    print(f"[gpu] resident mixed launch {len(blocks)} blocks grid 3", flush=True)
+    bits, conv, its, bad = plan.run(torch.from_numpy(llr).to(device))
+    torch.cuda.synchronize()
+    bits, conv, its, bad = (x.cpu().numpy() for x in (bits, conv, its, bad))
+    for i, (code, _, want, tag) in enumerate(blocks):
+        got = (bits[plan.off[i]:plan.off[i + 1]], bool(conv[i]), its[i], bad[i])
+        assert got[1] == bool(want[1]) and got[2] == want[2] and got[3] == want[3], tag
+        assert np.array_equal(got[0], want[0]), tag
+
+
+@pytest.mark.gpu
+def test_live_frame_operating_point(device):
+    """256-NUC r10/15 under Rayleigh fading at 21 dB, 512 blocks: every block
+    returns the transmitted codeword; mean iterations within 3% of plain FP32."""
+    import torch
+    from atsc3_amd import ldpc, synth
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.demap import Demapper
+    from atsc3_amd.resident import decode_resident_dev
+    code = get_code("10/15", 64800)
+    st = synth.FadingStream("10/15", 64800, "256QAM", 64, 7, device)
+    dm = Demapper(64800, "256QAM", "10/15", device)
+    print("[gpu] resident live-frame point 512 blocks", flush=True)
+    z, w, idx, n0 = st.batch(512, 21.0)
+    llr, _ = dm.demap_dev(z, w, n0)
+    bits, conv, its, bad = decode_resident_dev(llr, code, 50, 0.75)
+    b32, c32, i32, _ = ldpc.decode_dev(llr, code, 50, 0.75, "float32", "phase")
+    torch.cuda.synchronize()
+    cw = st.cw_dev.index_select(0, idx)
+    assert bool(conv.all()) and bool(c32.all())
+    assert torch.equal(bits, cw)
+    m16, m32 = its.float().mean().item(), i32.float().mean().item()
+    assert abs(m16 - m32) <= 0.03 * m32, (m16, m32)
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager(device):
+    import torch
+    from atsc3_amd.resident import ResidentPlan
+    code, llr, iters, alpha = _case("fixture", "g64k_r10_nuc20")
+    plan = ResidentPlan([code] * len(llr), device, iters, alpha)
+    src = torch.from_numpy(llr.reshape(-1)).to(device)
+    print("[gpu] resident graph replay", flush=True)
+    eager = [x.clone() for x in plan.run(src)]
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        plan.run(src)                        # warm-up on the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            plan.run(src)
+    torch.cuda.synchronize()
+    for r in range(3):
+        for x in (plan.bits, plan.conv, plan.its, plan.bad):
+            x.fill_(0xFF if x.dtype == torch.uint8 else -7)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip((plan.bits, plan.conv, plan.its, plan.bad), eager):
+            assert torch.equal(a, b), f"replay {r}"
