@@ -18,7 +18,9 @@
 //         edges 25..88, touched only by checks of degree > 25.  8 bytes per
 //         check in this workgroup's slot in global memory (21,600 checks =
 //         173 KB for rate 10/15), against 2 bytes per EDGE for stored FP16
-//         messages.
+//         messages.  Each record is written twice, at positions j and j + 360
+//         of its group's 720-record row, so the variable side reads check
+//         position p + u without reducing it mod 360.
 //
 // Per iteration (plain semantics, lab/ldpc_kernel.c:82-143):
 //   CHECK  lane j of a wave takes check (g, j): reads its old record and the
@@ -50,6 +52,7 @@
 #define RES_MAX_DC 89
 #define RES_MAX_DV 32
 #define RES_SIGN_LO 25         // sign bits held in the second record word
+#define RES_RROW 720           // records per check group: positions 0..359 twice
 
 namespace atsc3 {
 namespace {
@@ -75,7 +78,7 @@ struct ResArgs {
     const QC_CONST ResTab *codes;
     int nblk, iters;
     float alpha;
-    long long stride;            // checks per workgroup slot
+    long long stride;            // records per workgroup slot (2 per check)
     unsigned *p0, *p1;
     unsigned long long *p2;
     int *queue;
@@ -93,6 +96,29 @@ __device__ __forceinline__ float h_val(unsigned short h) {
     return __half2float(__ushort_as_half(h));
 }
 
+// Bit 31 set iff edge kk's message is negative (record words w1, w2).
+__device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2, unsigned kk) {
+    return (kk < RES_SIGN_LO ? w1 << (31u - kk) : (unsigned)(w2 >> (kk - RES_SIGN_LO)) << 31) &
+           0x80000000u;
+}
+
+// Message of edge kk of a check from its record words w0 (a1 | a2 << 16),
+// w1 (signs 0..24 | a2 edge << 25) and w2 (signs 25..88).  a1, a2 are
+// non-negative (alpha >= 0), so setting the sign bit negates.
+__device__ __forceinline__ float rec_msg(unsigned w0, unsigned w1, unsigned long long w2,
+                                         unsigned kk) {
+    const unsigned h = kk == (w1 >> RES_SIGN_LO) ? (w0 >> 16) : w0;
+    const float v = h_val((unsigned short)h);
+    return __uint_as_float(__float_as_uint(v) | rec_sign(w1, w2, kk));
+}
+
+// Record store at row position j and its copy at j + 360.
+template <typename T>
+__device__ __forceinline__ void rec_put(T *plane, unsigned r, T v) {
+    at(plane, r) = v;
+    at(plane, r + (unsigned)ATSC3_Z) = v;
+}
+
 // G check tasks of degree D <= 16 with every circulant full: the syndrome
 // bits of the current posteriors and, unless `update` is false, the new
 // compressed records.  Every load of the G tasks (old records, posteriors)
@@ -102,7 +128,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
                                                 unsigned *p0, unsigned *p1, int start,
                                                 unsigned lane, bool first, bool update,
                                                 float alpha) {
-    unsigned sidx[G], w0[G], w1[G];
+    unsigned rix[G], w0[G], w1[G];
     bool valid[G];
     float tv[G][D];
 #pragma unroll
@@ -112,12 +138,12 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         const unsigned j = (unsigned)(task >> 8) * 64u + lane;
         valid[gi] = j < (unsigned)ATSC3_Z;
         const unsigned jj = valid[gi] ? j : 0u;
-        sidx[gi] = (unsigned)g * ATSC3_Z + jj;
+        rix[gi] = (unsigned)g * RES_RROW + jj;
         w0[gi] = 0u;
         w1[gi] = 0u;
         if (!first) {
-            w0[gi] = p0[sidx[gi]];
-            w1[gi] = p1[sidx[gi]];
+            w0[gi] = at(p0, rix[gi]);
+            w1[gi] = at(p1, rix[gi]);
         }
         const int b0 = (ct->grp_c[g] >> 7) & 2047;
 #pragma unroll
@@ -134,51 +160,49 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         const float a1o = h_val((unsigned short)(w0[gi] & 0xffffu));
         const float a2o = h_val((unsigned short)(w0[gi] >> 16));
         const unsigned idxo = w1[gi] >> RES_SIGN_LO;
-        unsigned syn = 0u, par = 0u, idx = 0u, neg = 0u;
+        bool syn = false, par = false;
+        unsigned idx = 0u, neg = 0u;
         float m1 = INFINITY, m2 = INFINITY;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            float mo = (unsigned)k == idxo ? a2o : a1o;
-            if ((w1[gi] >> k) & 1u) mo = -mo;
+            const float ao = (unsigned)k == idxo ? a2o : a1o;
+            const float mo =
+                __uint_as_float(__float_as_uint(ao) | ((w1[gi] << (31 - k)) & 0x80000000u));
             const float x = tv[gi][k] - mo;
-            syn ^= (unsigned)(tv[gi][k] < 0.0f);
-            const unsigned ng = (unsigned)(x < 0.0f);
+            syn ^= tv[gi][k] < 0.0f;
+            const bool ng = x < 0.0f;
             par ^= ng;
-            neg |= ng << k;
+            neg |= ng ? 1u << k : 0u;
             const float mag = clamp_mag(x);
-            if (mag < m1) {
-                m2 = m1;
-                m1 = mag;
-                idx = (unsigned)k;
-            } else {
-                m2 = fminf(m2, mag);
-            }
+            idx = mag < m1 ? (unsigned)k : idx;
+            m2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
+            m1 = fminf(m1, mag);
         }
         if (update && valid[gi]) {
             const unsigned sg = par ? ~neg : neg;
-            p0[sidx[gi]] =
-                (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16);
-            p1[sidx[gi]] = (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
+            rec_put(p0, rix[gi],
+                    (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16));
+            rec_put(p1, rix[gi], (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
         }
-        syn_total += valid[gi] ? syn : 0u;
+        syn_total += (valid[gi] && syn) ? 1u : 0u;
     }
     return syn_total;
 }
 
 // Any check degree up to RES_MAX_DC (records of degree > 25 use the third
-// word): the same steps as check_task with a runtime edge loop.
+// word): the same steps as check_multi with a runtime edge loop.
 __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, unsigned *p0,
                                   unsigned *p1, unsigned long long *p2, int g, int b0, int dc,
                                   bool hasmiss, unsigned j, bool valid, bool first, bool update,
                                   float alpha) {
     const unsigned jj = valid ? j : 0u;
-    const unsigned sidx = (unsigned)g * ATSC3_Z + jj;
+    const unsigned rix = (unsigned)g * RES_RROW + jj;
     unsigned w0 = 0u, w1 = 0u;
     unsigned long long w2 = 0ull;
     if (!first) {
-        w0 = p0[sidx];
-        w1 = p1[sidx];
-        if (dc > RES_SIGN_LO) w2 = p2[sidx];
+        w0 = at(p0, rix);
+        w1 = at(p1, rix);
+        if (dc > RES_SIGN_LO) w2 = at(p2, rix);
     }
     const float a1o = h_val((unsigned short)(w0 & 0xffffu));
     const float a2o = h_val((unsigned short)(w0 >> 16));
@@ -191,15 +215,13 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
         if (hasmiss && ((ew >> 18) & 1023u) == jj + 1u) continue;
         const unsigned p = wrap360(jj + (ew & 511u));
         const float tv = __half2float(tot[((ew >> 9) & 511u) * ATSC3_Z + p]);
-        const bool lo = k < RES_SIGN_LO;
-        const unsigned sgk = lo ? (w1 >> k) & 1u : (unsigned)(w2 >> (k - RES_SIGN_LO)) & 1u;
-        float mo = (unsigned)k == idxo ? a2o : a1o;
-        if (sgk) mo = -mo;
+        const float ao = (unsigned)k == idxo ? a2o : a1o;
+        const float mo = __uint_as_float(__float_as_uint(ao) | rec_sign(w1, w2, (unsigned)k));
         const float x = tv - mo;
         syn ^= (unsigned)(tv < 0.0f);
         const unsigned ng = (unsigned)(x < 0.0f);
         par ^= ng;
-        if (lo)
+        if (k < RES_SIGN_LO)
             neg_lo |= ng << k;
         else
             neg_hi |= (unsigned long long)ng << (k - RES_SIGN_LO);
@@ -214,22 +236,12 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
     }
     if (update && valid) {
         const unsigned sg_lo = par ? ~neg_lo : neg_lo;
-        p0[sidx] = (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16);
-        p1[sidx] = (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
-        if (dc > RES_SIGN_LO) p2[sidx] = par ? ~neg_hi : neg_hi;
+        rec_put(p0, rix,
+                (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16));
+        rec_put(p1, rix, (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
+        if (dc > RES_SIGN_LO) rec_put(p2, rix, par ? ~neg_hi : neg_hi);
     }
     return valid ? syn : 0u;
-}
-
-// Message of edge kk of a check from its record words w0 (a1 | a2 << 16),
-// w1 (signs 0..24 | a2 edge << 25) and w2 (signs 25..88).
-__device__ __forceinline__ float rec_msg(unsigned w0, unsigned w1, unsigned long long w2,
-                                         unsigned kk) {
-    const bool take_a2 = kk == (w1 >> RES_SIGN_LO);
-    const float v = h_val((unsigned short)(take_a2 ? (w0 >> 16) : (w0 & 0xffffu)));
-    const unsigned sg =
-        kk < RES_SIGN_LO ? (w1 >> kk) & 1u : (unsigned)(w2 >> (kk - RES_SIGN_LO)) & 1u;
-    return sg ? -v : v;
 }
 
 // G variable tasks of degree D <= 16 whose entries are all full circulants
@@ -237,12 +249,15 @@ __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w1, unsigned long
 // record and LLR load of the G tasks first, then the sums in ascending check
 // index.  Within a region the entries are sorted by u, so the ones whose
 // check position wrapped past 359 (p + u >= 360) are a suffix of the
-// region's list; the order is that suffix, then the prefix.
+// region's list; the order is that suffix, then the prefix.  A skipped
+// message is added as +0.0, which leaves the sum unchanged (it starts at
+// +0.0 and can never become -0.0).
 template <int D, int G>
 __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot,
                                           const float *lam, const unsigned *p0,
                                           const unsigned *p1, int start, unsigned lane) {
-    unsigned w0[G][D], w1[G][D], wrapped[G], dst[G];
+    unsigned w0[G][D], w1[G][D], dst[G];
+    bool wr[G][D];
     float lv[G];
     bool valid[G];
     int e0[G], n0[G];
@@ -257,16 +272,15 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
         const int vw = ct->grp_v[t];
         e0[gi] = (vw >> 6) & 2047;
         n0[gi] = (vw >> 17) & 63;
-        lv[gi] = lam[dst[gi]];
-        wrapped[gi] = 0u;
+        lv[gi] = at(lam, dst[gi]);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u;
-            const unsigned sidx = g * ATSC3_Z + wrap360(pp + u);
-            w0[gi][k] = p0[sidx];
-            w1[gi][k] = p1[sidx];
-            wrapped[gi] |= (unsigned)(pp + u >= (unsigned)ATSC3_Z) << k;
+            const unsigned r = g * RES_RROW + pp + u;
+            w0[gi][k] = at(p0, r);
+            w1[gi][k] = at(p1, r);
+            wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -281,16 +295,18 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
         float s = 0.0f;
         if (n0[gi] > 0) {
 #pragma unroll
-            for (int w = 1; w >= 0; --w)
-#pragma unroll
-                for (int k = 0; k < D; ++k)
-                    if (k < n0[gi] && ((wrapped[gi] >> k) & 1u) == (unsigned)w) s += m[k];
-        }
-#pragma unroll
-        for (int w = 1; w >= 0; --w)
+            for (int k = 0; k < D; ++k)
+                if (k < n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
 #pragma unroll
             for (int k = 0; k < D; ++k)
-                if (k >= n0[gi] && ((wrapped[gi] >> k) & 1u) == (unsigned)w) s += m[k];
+                if (k < n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (k >= n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         if (valid[gi]) tot[dst[gi]] = __ushort_as_half(h_bits_sat(lv[gi] + s));
     }
 }
@@ -317,9 +333,9 @@ __device__ void var_generic(const QC_CONST ResTab *ct, __half *tot, const float 
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0 + k)];
             const unsigned vm = (unsigned)ct->edge_v[2 * (e0 + k) + 1];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
-            const unsigned sidx = g * ATSC3_Z + wrap360(pp + u);
-            const unsigned long long w2 = kk >= RES_SIGN_LO ? p2[sidx] : 0ull;
-            const float mv = rec_msg(p0[sidx], p1[sidx], w2, kk);
+            const unsigned rix = g * RES_RROW + pp + u;
+            const unsigned long long w2 = kk >= RES_SIGN_LO ? at(p2, rix) : 0ull;
+            const float mv = rec_msg(at(p0, rix), at(p1, rix), w2, kk);
             if (vm != pp + 1u) s += mv;
         }
     }
@@ -449,7 +465,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
 
 extern "C" int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks) {
     if (grid < 1 || max_checks < 1) return ATSC3_E_ARG;
-    const int64_t stride = ((int64_t)max_checks + 63) & ~(int64_t)63;
+    const int64_t stride = (2 * (int64_t)max_checks + 63) & ~(int64_t)63;
     return 256 + (int64_t)grid * stride * 16;
 }
 
@@ -469,7 +485,7 @@ extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
                                              int32_t grid, void *ws, int64_t ws_bytes,
                                              uint8_t *bits, uint8_t *conv, int32_t *its,
                                              int32_t *bad, void *stream) {
-    if (nblk < 0 || iters < 0 || iters >= (1 << 24)) return ATSC3_E_ARG;
+    if (nblk < 0 || iters < 0 || iters >= (1 << 24) || !(alpha >= 0.0f)) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!llr || !blk_off || !blk_code || !codes || !ws || !bits || !conv || !its || !bad)
         return ATSC3_E_ARG;
@@ -479,7 +495,7 @@ extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
         return ATSC3_E_DEGREE;
     const int64_t need = atsc3_minsum_resident_ws_bytes(grid, max_checks);
     if (need < 0 || ws_bytes < need) return ATSC3_E_SIZE;
-    const int64_t stride = ((int64_t)max_checks + 63) & ~(int64_t)63;
+    const int64_t stride = (2 * (int64_t)max_checks + 63) & ~(int64_t)63;
     char *base = static_cast<char *>(ws);
     atsc3::ResArgs a{};
     a.llr = llr;

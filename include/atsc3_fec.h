@@ -193,8 +193,10 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
  *   blk_code, blk_off  as atsc3_minsum_batch_dev
  *   max_checks largest ncg * 360 over the codes
  *   max_n, max_dc, max_dv  maxima over the codes
+ *   alpha      normalization, >= 0 (the records hold magnitudes)
  *   ws         device workspace of atsc3_minsum_resident_ws_bytes(grid,
- *              max_checks) bytes (queue word + per-workgroup records)
+ *              max_checks) bytes (queue word + per-workgroup records, each
+ *              check's record stored twice)
  * Outputs bits/conv/its/bad with the plain semantics (its = iteration at
  * which the syndrome read zero, else iters; bad = unsatisfied checks).
  * Two launches on `stream` (queue reset, decode).
