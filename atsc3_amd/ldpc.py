@@ -134,8 +134,10 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     engine: "qc" (fused 360-lifted kernel, one workgroup per codeword),
     "phase" (the same decode as per-iteration phase kernels, for large
     batches), "batch" (the persistent batched engine, atsc3_amd.batch),
-    "csr" (generic kernel; FP32 only) or "auto" (the QC engines by batch size
-    and message format when the code has a QC form, else csr).
+    "csr" (generic kernel; FP32 only), "resident" (FP16 only: one
+    workgroup per codeword with FP16 posteriors in LDS, atsc3_amd.resident)
+    or "auto" (the QC engines by batch size and message format when the
+    code has a QC form, else csr).
     """
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_dev needs a CUDA (HIP) tensor")
@@ -161,6 +163,13 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
             engine = "qc"
         else:
             engine = "phase" if fmt == 0 else "batch"
+    if engine == "resident":
+        if fmt != 1:
+            raise ValueError("the resident engine stores FP16 messages only")
+        if msg_scale is not None:
+            raise ValueError("msg_scale applies to INT8 messages only")
+        from .resident import decode_resident_dev
+        return decode_resident_dev(llr, code, iters, alpha)
     if engine == "batch":
         if tabs.qc is None:
             raise ValueError("this code has no QC form with variable degree <= 32")

@@ -207,19 +207,24 @@ class ResidentPlan:
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
         self.ws_bytes = int(nbytes)
 
-    def run(self, llr=None):
-        """Enqueue the decode of ``llr`` (flat, or None: self.llr as filled)."""
+    def run(self, llr=None, out=None):
+        """Enqueue the decode of ``llr`` (flat, or None: self.llr as filled)
+        into ``out`` = (bits, conv, its, bad) device tensors of the plan's
+        sizes (None: the plan's own buffers, overwritten by the next run)."""
         src = self.llr if llr is None else llr
         if src.numel() != self.total or src.dtype != torch.float32 or not src.is_contiguous():
             raise ValueError("llr must be a contiguous float32 tensor of the plan's length")
+        bits, conv, its, bad = (self.bits, self.conv, self.its, self.bad) if out is None else out
+        if (bits.numel() != self.total or conv.numel() != self.nblk or its.numel() != self.nblk
+                or bad.numel() != self.nblk):
+            raise ValueError("output tensors do not match the plan")
         b = self.bank
         _lib.call("atsc3_minsum_resident_dev", src.data_ptr(), self.nblk,
                   self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
                   len(b), b.max_checks, b.max_n, b.max_dc, b.max_dv, self.iters, self.alpha,
-                  self.grid, self.ws.data_ptr(), self.ws_bytes, self.bits.data_ptr(),
-                  self.conv.data_ptr(), self.its.data_ptr(), self.bad.data_ptr(),
-                  _stream_ptr(self.device))
-        return self.bits, self.conv, self.its, self.bad
+                  self.grid, self.ws.data_ptr(), self.ws_bytes, bits.data_ptr(),
+                  conv.data_ptr(), its.data_ptr(), bad.data_ptr(), _stream_ptr(self.device))
+        return bits, conv, its, bad
 
     def block_bits(self, b):
         return self.bits[int(self.off[b]):int(self.off[b + 1])]
@@ -229,24 +234,30 @@ _PLANS = {}
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
-    """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad)."""
+    """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
+
+    Enqueues two launches and no host synchronisation; the plan (tables,
+    queue, per-workgroup records) is kept for the next call of the same
+    shape."""
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_resident_dev needs a CUDA (HIP) tensor")
-    llr = llr.contiguous().float()
+    llr = llr.contiguous()
+    if llr.dtype != torch.float32:
+        llr = llr.float()
     nblk, n = llr.shape
     if n != code.n:
         raise ValueError(f"LLR width {n} != code length {code.n}")
+    dev = llr.device
+    bits = torch.empty((nblk, n), dtype=torch.uint8, device=dev)
+    conv = torch.empty(nblk, dtype=torch.uint8, device=dev)
+    its = torch.empty(nblk, dtype=torch.int32, device=dev)
+    bad = torch.empty(nblk, dtype=torch.int32, device=dev)
     if nblk == 0:
-        dev = llr.device
-        return (torch.empty((0, n), dtype=torch.uint8, device=dev),
-                torch.empty(0, dtype=torch.uint8, device=dev),
-                torch.empty(0, dtype=torch.int32, device=dev),
-                torch.empty(0, dtype=torch.int32, device=dev))
-    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(llr.device))
+        return bits, conv, its, bad
+    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev))
     plan = _PLANS.get(key)
     if plan is None or plan.bank.codes[0] is not code:
-        plan = ResidentPlan([code] * nblk, llr.device, iters, alpha, grid)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid)
         _PLANS.clear()
         _PLANS[key] = plan
-    bits, conv, its, bad = plan.run(llr.reshape(-1))
-    return bits.view(nblk, n).clone(), conv.clone(), its.clone(), bad.clone()
+    return plan.run(llr.reshape(-1), (bits, conv, its, bad))

@@ -64,9 +64,21 @@ def log(*a):
 
 
 def model_bytes(its, n, e, s_msg):
-    """Compulsory HBM bytes of one decode launch (per-block sum)."""
+    """Compulsory HBM bytes of one decode launch of the engines that keep
+    per-edge messages and FP32 posteriors in device memory (per-block sum):
+    LLRs in, bits out, and per iteration the messages read and written once
+    and the posteriors written once."""
     its = np.asarray(its, np.float64)
     return float(np.sum(4.0 * n + n + its * (2.0 * e * s_msg + 4.0 * n)))
+
+
+def resident_model_bytes(its, n, m):
+    """Compulsory bytes of one resident-engine launch (posteriors in LDS):
+    LLRs in, bits out, and per iteration each check's 8-byte record read
+    and written by the check phase and read once by the variable phase,
+    plus the LLRs the variable phase re-reads (tot = lam + sum)."""
+    its = np.asarray(its, np.float64)
+    return float(np.sum(4.0 * n + n + its * (3.0 * 8.0 * m + 4.0 * n)))
 
 
 def load_traffic(workload_key):
@@ -510,13 +522,17 @@ def main():
     ap.add_argument("--msg", default="float16", choices=["float32", "float16", "int8"],
                     help="message storage of the headline decode (float16 = the batched "
                          "FP16 form; plain FP32 is always measured beside it)")
+    ap.add_argument("--engine", default="resident", choices=["resident", "auto"],
+                    help="decoder of the FP16 headline: resident (one workgroup per "
+                         "codeword, FP16 posteriors in LDS) or auto (the batched engine)")
     ap.add_argument("--esn0", type=float, default=None, help="skip the sweep")
     ap.add_argument("--fer-blocks", type=int, default=30000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP16 / INT8 message-storage comparison")
-    ap.add_argument("--form-fer-windows", type=int, default=4)
+    ap.add_argument("--form-fer-windows", type=int, default=4,
+                    help="windows of blocks per FER point of the message-storage forms")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="live", choices=["live", "ldm", "mixed", "learn"])
     ap.add_argument("--learn-steps", type=int, default=300)
@@ -611,6 +627,7 @@ def main():
     ncell = N // 8
     nblk = args.frames * FRAME_CELLS // ncell
     s_msg = {"float32": 4, "float16": 2, "int8": 1}[args.msg]
+    engine = args.engine if args.msg == "float16" else "auto"
     log(f"[bench] {nblk} blocks per step ({args.frames} frames x {FRAME_CELLS} cells)")
     t0 = time.time()
     stream = synth.FadingStream(RATE, N, MOD, 128, args.seed, dev)
@@ -628,7 +645,7 @@ def main():
 
     def _msg_pass(es, k):
         z, w, idx, n0 = stream.batch(nblk, es, start=k * nblk)
-        return idx, run_window(z, w, n0, args.msg)
+        return idx, run_window(z, w, n0, args.msg, engine)
 
     # -- operating point ------------------------------------------------------
     fer = {}
@@ -657,7 +674,8 @@ def main():
                 llr_k, _ = dm.demap_dev(z, csi=w, sigma2=n0)
                 tot_e += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, "float32"))
                 if args.msg != "float32":
-                    e_msg += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, args.msg))
+                    e_msg += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, args.msg,
+                                                         engine))
                 tot_b += nblk
                 k += 1
             fer[op].update(blocks=tot_b, errors=tot_e, **({f"errors_{args.msg}": e_msg}
@@ -677,7 +695,7 @@ def main():
     # -- timed steps ------------------------------------------------------------
     z, w, idx, n0 = stream.batch(nblk, op, start=10 ** 6)
     for _ in range(args.warmup):
-        out = run_window(z, w, n0, args.msg)
+        out = run_window(z, w, n0, args.msg, engine)
     torch.cuda.synchronize()
     cur = torch.cuda.current_stream(dev)
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
@@ -686,7 +704,7 @@ def main():
         ev[k][0].record(cur)
         llr, _ = dm.demap_dev(z, csi=w, sigma2=n0)
         ev[k][1].record(cur)
-        out = ldpc.decode_dev(llr, code, 50, 0.75, args.msg)
+        out = ldpc.decode_dev(llr, code, 50, 0.75, args.msg, engine)
         ev[k][2].record(cur)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
@@ -697,10 +715,14 @@ def main():
     conv = out[1].cpu().numpy().astype(bool)
     step_errors = errors(idx, out)
     value = nblk / (ms_per_step / 1e3)
-    achieved = model_bytes(its, N, E, s_msg) / (decode_ms / 1e3) / 1e9
-    key = f"live_frame_{args.msg}_{args.frames}f_es{op:g}"
+    if engine == "resident":
+        achieved = resident_model_bytes(its, N, code.nc) / (decode_ms / 1e3) / 1e9
+        key = f"live_frame_{args.msg}_resident_{args.frames}f_es{op:g}"
+    else:
+        achieved = model_bytes(its, N, E, s_msg) / (decode_ms / 1e3) / 1e9
+        key = f"live_frame_{args.msg}_{args.frames}f_es{op:g}"
     traffic = load_traffic(key)          # measured HBM bytes per decode call
-    log(f"[bench] Es/N0 {op:g} dB, {args.msg}: {ms_per_step:.3f} ms/step (demap {demap_ms:.3f} ms, "
+    log(f"[bench] Es/N0 {op:g} dB, {args.msg} ({engine}): {ms_per_step:.3f} ms/step (demap {demap_ms:.3f} ms, "
         f"decode {decode_ms:.3f} ms), mean its {its.mean():.2f}, "
         f"conv {conv.mean():.4f}, errors {step_errors}")
     # the Baseband Packet step on the decoded window (BCH syndrome + descramble);
@@ -772,9 +794,12 @@ def main():
     # -- message-storage forms at the same operating point ----------------------
     forms = {}
     if not args.no_forms:
-        for fmt in ("float32", "float16", "int8"):
+        for name, fmt, engines in (("float32", "float32", ("phase", "batch")),
+                                   ("float16", "float16", ("phase", "batch")),
+                                   ("float16_resident", "float16", ("resident",)),
+                                   ("int8", "int8", ("phase", "batch"))):
             eng_ms = {}
-            for eng in ("phase", "batch"):
+            for eng in engines:
                 run_window(z, w, n0, fmt, eng)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -787,26 +812,31 @@ def main():
             errs, blocks = errors(idx, fo), nblk
             for k in range(1, args.form_fer_windows):
                 z2, w2, idx2, _ = stream.batch(nblk, op, start=(7 + k) * nblk)
-                errs += errors(idx2, run_window(z2, w2, n0, fmt))
+                errs += errors(idx2, run_window(z2, w2, n0, fmt, best))
                 blocks += nblk
             # FER curve below the operating point (same blocks for every form)
             curve = []
             for es in FORM_SWEEP:
-                z3, w3, idx3, n03 = stream.batch(nblk, es, start=int(es * 4) * 7919)
-                o3 = run_window(z3, w3, n03, fmt)
-                curve.append([es, errors(idx3, o3), nblk,
-                              round(float(o3[2].float().mean().item()), 3)])
-            forms[fmt] = {"blocks_per_s": round(nblk / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                ce, cits = 0, 0.0
+                for k in range(args.form_fer_windows):
+                    z3, w3, idx3, n03 = stream.batch(nblk, es,
+                                                     start=(int(es * 4) * 7919 + k) * nblk)
+                    o3 = run_window(z3, w3, n03, fmt, best)
+                    ce += errors(idx3, o3)
+                    cits += float(o3[2].float().mean().item())
+                curve.append([es, ce, nblk * args.form_fer_windows,
+                              round(cits / args.form_fer_windows, 3)])
+            forms[name] = {"blocks_per_s": round(nblk / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
                           "engine": best, "ms_by_engine": {k: round(v, 3) for k, v in eng_ms.items()},
                           "mean_its": round(float(fo[2].float().mean().item()), 3),
                           "fer_blocks": blocks, "fer_errors": errs, "fer_curve": curve}
-            log(f"[bench] form {fmt}: {nblk / (ms / 1e3):.0f} blocks/s ({best}; "
+            log(f"[bench] form {name}: {nblk / (ms / 1e3):.0f} blocks/s ({best}; "
                 f"{', '.join(f'{k} {v:.2f} ms' for k, v in eng_ms.items())}), "
                 f"{errs}/{blocks} block errors; curve {[(c[0], c[1]) for c in curve]}")
         if "float32" in forms:
-            for fmt in ("float16", "int8"):
-                forms[fmt]["gap_db_at_fer_1e-2"] = fer_gap_db(forms["float32"]["fer_curve"],
-                                                              forms[fmt]["fer_curve"], 1e-2)
+            for name in ("float16", "float16_resident", "int8"):
+                forms[name]["gap_db_at_fer_1e-2"] = fer_gap_db(forms["float32"]["fer_curve"],
+                                                               forms[name]["fer_curve"], 1e-2)
 
     # -- CPU baseline (same blocks, bounded sample) ----------------------------
     cpu = None
@@ -876,6 +906,8 @@ def main():
                                 f"perfect CSI, {args.frames} frames x {FRAME_CELLS} cells = "
                                 f"{nblk} blocks per step, demap + decode"),
                    "esn0_db": op, "messages": args.msg, "accumulation": "float32",
+                   "engine": engine,
+                   "posteriors": ("float16 in LDS" if engine == "resident" else "float32"),
                    "plain_fp32": plain, "iters_max": 50, "alpha": 0.75,
                    "baseband_ms": round(baseband_ms, 4),
                    "demap_ms_plain_vs_hti_gather": demap_times,
