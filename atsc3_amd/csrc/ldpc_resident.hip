@@ -52,7 +52,11 @@
 #define RES_MAX_DC 89
 #define RES_MAX_DV 32
 #define RES_SIGN_LO 25         // sign bits held in the second record word
-#define RES_RROW 720           // records per check group: positions 0..359 twice
+// Records per check-group row: 720 = each record stored twice (positions j
+// and j + 360), so variable-side reads need no mod 360; 360 = stored once.
+#ifndef RES_RROW
+#define RES_RROW 360
+#endif
 
 namespace atsc3 {
 namespace {
@@ -66,7 +70,7 @@ struct ResTab {
     const QC_CONST int *edge_v;   // per entry, two words: u | g << 9 | k << 17; missing p + 1
     const QC_CONST int *c_steps;  // check steps: D | G << 7 | start << 10 | generic << 22
     const QC_CONST int *c_tasks;  // task: group | c6 << 8 (rows c6*64 .. c6*64+63)
-    const QC_CONST int *v_steps;  // variable steps, same packing
+    const QC_CONST int *v_steps;  // variable steps, same packing (+ Type A region << 23)
     const QC_CONST int *v_tasks;
 };
 static_assert(sizeof(ResTab) == 96, "resident code descriptor layout");
@@ -116,7 +120,13 @@ __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w1, unsigned long
 template <typename T>
 __device__ __forceinline__ void rec_put(T *plane, unsigned r, T v) {
     at(plane, r) = v;
-    at(plane, r + (unsigned)ATSC3_Z) = v;
+    if (RES_RROW == 2 * ATSC3_Z) at(plane, r + (unsigned)ATSC3_Z) = v;
+}
+
+// Row position of check position p + u (0 <= p, u < 360) whose wrap flag is
+// `wr` (p + u >= 360).
+__device__ __forceinline__ unsigned rec_pos(unsigned pu, bool wr) {
+    return RES_RROW == 2 * ATSC3_Z ? pu : pu - (wr ? (unsigned)ATSC3_Z : 0u);
 }
 
 // G check tasks of degree D <= 16 with every circulant full: the syndrome
@@ -251,8 +261,11 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
 // check position wrapped past 359 (p + u >= 360) are a suffix of the
 // region's list; the order is that suffix, then the prefix.  A skipped
 // message is added as +0.0, which leaves the sum unchanged (it starts at
-// +0.0 and can never become -0.0).
-template <int D, int G>
+// +0.0 and can never become -0.0).  TA: the group has a first region
+// (Type A codes, n0 > 0 entries); else one region.  With one region of two
+// entries the order does not matter: (0 + a) + b == (0 + b) + a in IEEE
+// arithmetic, signed zeros included.
+template <int D, int G, bool TA>
 __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot,
                                           const float *lam, const unsigned *p0,
                                           const unsigned *p1, int start, unsigned lane) {
@@ -271,16 +284,16 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
         dst[gi] = (unsigned)t * ATSC3_Z + pp;
         const int vw = ct->grp_v[t];
         e0[gi] = (vw >> 6) & 2047;
-        n0[gi] = (vw >> 17) & 63;
+        n0[gi] = TA ? (vw >> 17) & 63 : 0;
         lv[gi] = at(lam, dst[gi]);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u;
-            const unsigned r = g * RES_RROW + pp + u;
+            wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
+            const unsigned r = g * RES_RROW + rec_pos(pp + u, wr[gi][k]);
             w0[gi][k] = at(p0, r);
             w1[gi][k] = at(p1, r);
-            wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -290,23 +303,29 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned kk = ((unsigned)ct->edge_v[2 * (e0[gi] + k)] >> 17) & 127u;
-            m[k] = rec_msg(w0[gi][k], w1[gi][k], 0ull, kk);
+            const unsigned h = kk == (w1[gi][k] >> RES_SIGN_LO) ? (w0[gi][k] >> 16) : w0[gi][k];
+            m[k] = __uint_as_float(__float_as_uint(h_val((unsigned short)h)) |
+                                   ((w1[gi][k] << (31u - kk)) & 0x80000000u));
         }
         float s = 0.0f;
-        if (n0[gi] > 0) {
+        if (!TA && D == 2) {
+            s = (s + m[0]) + m[1];
+        } else {
+            if (TA) {
+#pragma unroll
+                for (int k = 0; k < D; ++k)
+                    if (k < n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k)
+                    if (k < n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
+            }
 #pragma unroll
             for (int k = 0; k < D; ++k)
-                if (k < n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
+                if (k >= n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
 #pragma unroll
             for (int k = 0; k < D; ++k)
-                if (k < n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
+                if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         }
-#pragma unroll
-        for (int k = 0; k < D; ++k)
-            if (k >= n0[gi]) s += wr[gi][k] ? m[k] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < D; ++k)
-            if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         if (valid[gi]) tot[dst[gi]] = __ushort_as_half(h_bits_sat(lv[gi] + s));
     }
 }
@@ -333,7 +352,7 @@ __device__ void var_generic(const QC_CONST ResTab *ct, __half *tot, const float 
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0 + k)];
             const unsigned vm = (unsigned)ct->edge_v[2 * (e0 + k) + 1];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
-            const unsigned rix = g * RES_RROW + pp + u;
+            const unsigned rix = g * RES_RROW + rec_pos(pp + u, pp + u >= (unsigned)ATSC3_Z);
             const unsigned long long w2 = kk >= RES_SIGN_LO ? at(p2, rix) : 0ull;
             const float mv = rec_msg(at(p0, rix), at(p1, rix), w2, kk);
             if (vm != pp + 1u) s += mv;
@@ -430,18 +449,28 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                                 (vw >> 17) & 63, p, p < (unsigned)ATSC3_Z);
                     continue;
                 }
-                switch (G * 128 + D) {
-#define RES_V1(DD)                                              \
-    case 128 + DD:                                              \
-        var_multi<DD, 1>(ct, tot, lam, p0, p1, start, lane);    \
+                const bool ta = (sw >> 23) & 1;
+                switch ((ta ? 1024 : 0) + G * 128 + D) {
+#define RES_V1(DD)                                                      \
+    case 128 + DD:                                                      \
+        var_multi<DD, 1, false>(ct, tot, lam, p0, p1, start, lane);     \
+        break;                                                          \
+    case 1024 + 128 + DD:                                               \
+        var_multi<DD, 1, true>(ct, tot, lam, p0, p1, start, lane);      \
         break;
-#define RES_V2(DD)                                              \
-    case 256 + DD:                                              \
-        var_multi<DD, 2>(ct, tot, lam, p0, p1, start, lane);    \
+#define RES_V2(DD)                                                      \
+    case 256 + DD:                                                      \
+        var_multi<DD, 2, false>(ct, tot, lam, p0, p1, start, lane);     \
+        break;                                                          \
+    case 1024 + 256 + DD:                                               \
+        var_multi<DD, 2, true>(ct, tot, lam, p0, p1, start, lane);      \
         break;
-#define RES_V4(DD)                                              \
-    case 512 + DD:                                              \
-        var_multi<DD, 4>(ct, tot, lam, p0, p1, start, lane);    \
+#define RES_V4(DD)                                                      \
+    case 512 + DD:                                                      \
+        var_multi<DD, 4, false>(ct, tot, lam, p0, p1, start, lane);     \
+        break;                                                          \
+    case 1024 + 512 + DD:                                               \
+        var_multi<DD, 4, true>(ct, tot, lam, p0, p1, start, lane);      \
         break;
                     QC_CASES16(RES_V1)
                     QC_CASES8(RES_V2)
@@ -465,7 +494,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
 
 extern "C" int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks) {
     if (grid < 1 || max_checks < 1) return ATSC3_E_ARG;
-    const int64_t stride = (2 * (int64_t)max_checks + 63) & ~(int64_t)63;
+    const int64_t stride = ((int64_t)max_checks * (RES_RROW / ATSC3_Z) + 63) & ~(int64_t)63;
     return 256 + (int64_t)grid * stride * 16;
 }
 
@@ -495,7 +524,7 @@ extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
         return ATSC3_E_DEGREE;
     const int64_t need = atsc3_minsum_resident_ws_bytes(grid, max_checks);
     if (need < 0 || ws_bytes < need) return ATSC3_E_SIZE;
-    const int64_t stride = (2 * (int64_t)max_checks + 63) & ~(int64_t)63;
+    const int64_t stride = ((int64_t)max_checks * (RES_RROW / ATSC3_Z) + 63) & ~(int64_t)63;
     char *base = static_cast<char *>(ws);
     atsc3::ResArgs a{};
     a.llr = llr;

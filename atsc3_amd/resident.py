@@ -114,6 +114,14 @@ def _host_tables(code):
     vmissg = np.array([int((vmiss[vg[t]:vg[t + 1]] > 0).any()) for t in range(len(dv))])
     c_tasks, c_steps = _pack_tasks(dc, (hasmiss > 0) | (dc > FAST_MAX), CHECK_PACK)
     v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25), VAR_PACK)
+    # a variable step with any group that has a first region (Type A: n0 > 0
+    # entries summed before the rest) carries bit 23
+    ta = n0 > 0
+
+    def _ta(w):
+        first, cnt = (w >> 10) & 4095, (w >> 7) & 7
+        return any(ta[v_tasks[first + i] & 255] for i in range(cnt))
+    v_steps = np.array([w | (1 << 23 if _ta(w) else 0) for w in v_steps], np.int64)
     i32 = lambda a: np.ascontiguousarray(a, np.int32)
     return dict(n=code.n, ncg=len(dc), nvg=len(dv), nbe=nbe, max_dc=int(dc.max()),
                 max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
