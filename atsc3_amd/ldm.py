@@ -38,11 +38,15 @@ def powers(inj_db):
 
 class LdmPipeline:
     def __init__(self, core=(64800, "QPSK", "4/15"), enh=(64800, "64QAM", "10/15"),
-                 inj_db=5.0, iters=50, alpha=0.75, device="cuda", msg_dtype="float32"):
+                 inj_db=5.0, iters=50, alpha=0.75, device="cuda", msg_dtype="float32",
+                 engine="auto"):
+        """engine: the decoder of both layers (ldpc.decode_dev's choice;
+        "resident" needs msg_dtype float16)."""
         self.device = torch.device(device)
         self.core, self.enh = core, enh
         self.p_core, self.p_enh = powers(inj_db)
         self.iters, self.alpha, self.msg = iters, alpha, msg_dtype
+        self.engine = engine
         n_c, mod_c, r_c = core
         n_e, mod_e, r_e = enh
         self.code_c, self.code_e = get_code(r_c, n_c), get_code(r_e, n_e)
@@ -80,11 +84,13 @@ class LdmPipeline:
         device tensors: core (bits, conv, its, bad), enh (bits, conv, its, bad)
         with 3 enhanced blocks per core block, and the per-block gains."""
         llr_c, _ = self.dm_c.demap_dev(y, sigma2=self.p_enh + n0)
-        core = ldpc.decode_dev(llr_c, self.code_c, self.iters, self.alpha, self.msg)
+        core = ldpc.decode_dev(llr_c, self.code_c, self.iters, self.alpha, self.msg,
+                               self.engine)
         r, gain = self.cancel(y, core[0], core[1])
         r_e = r.reshape(-1, self.ncell_e)
         llr_e, _ = self.dm_e.demap_dev(r_e, sigma2=n0)
-        enh = ldpc.decode_dev(llr_e, self.code_e, self.iters, self.alpha, self.msg)
+        enh = ldpc.decode_dev(llr_e, self.code_e, self.iters, self.alpha, self.msg,
+                              self.engine)
         return dict(core=core, enh=enh, gain=gain, llr_core=llr_c, llr_enh=llr_e, resid=r)
 
 

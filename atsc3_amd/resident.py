@@ -238,7 +238,8 @@ class ResidentPlan:
         return self.bits[int(self.off[b]):int(self.off[b + 1])]
 
 
-_PLANS = {}
+_PLANS = {}            # a few recent plans (LRU): callers alternate codes, e.g. LDM layers
+_PLANS_MAX = 8
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
@@ -263,9 +264,10 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
     if nblk == 0:
         return bits, conv, its, bad
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev))
-    plan = _PLANS.get(key)
+    plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid)
-        _PLANS.clear()
-        _PLANS[key] = plan
+    _PLANS[key] = plan
+    while len(_PLANS) > _PLANS_MAX:
+        _PLANS.pop(next(iter(_PLANS)))
     return plan.run(llr.reshape(-1), (bits, conv, its, bad))

@@ -111,7 +111,7 @@ def fer_gap_db(ref, other, target):
     return None if a is None or b is None else round(b - a, 3)
 
 
-def ldm_bench(args, dev, msg_formats=("float32", "float16")):
+def ldm_bench(args, dev, msg_formats=("float32", "float16", "float16_resident")):
     """Two-layer LDM chain over ``args.ldm_frames`` frames per step.
 
     Core: QPSK 4/15 (32400 cells per block), enhanced: 64-NUC 10/15 (10800
@@ -125,7 +125,9 @@ def ldm_bench(args, dev, msg_formats=("float32", "float16")):
 
     res = {}
     for fmt in msg_formats:
-        pipe = LdmPipeline(device=dev, msg_dtype=fmt)
+        msg, engine = (("float16", "resident") if fmt == "float16_resident"
+                       else (fmt, "auto"))
+        pipe = LdmPipeline(device=dev, msg_dtype=msg, engine=engine)
         if "stream" not in res:
             res["stream"] = LdmStream(pipe, 64, args.seed + 17)
         stream = res["stream"]
@@ -281,7 +283,7 @@ def learn_bench(args, dev, steps, eval_blocks, ebn0_points=(1.0, 1.5, 2.0, 2.5, 
             "fer_block_errors": fer}
 
 
-def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4):
+def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4, engine="resident"):
     """The LDM chain fed as the upstream receives it (lab/m44_ldm.py:1187-1304):
     the combined two-layer cells go through a convolutional time interleaver
     (Nrows 1024), are cut into frames of 1,216,939 cells and pushed one frame
@@ -292,7 +294,8 @@ def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4):
     from atsc3_amd.ldm import LdmPipeline, LdmStream
     from atsc3_amd.ti import CtiStreamDev
     from atsc3_amd import synth
-    pipe = LdmPipeline(device=dev, msg_dtype=fmt)
+    pipe = LdmPipeline(device=dev, msg_dtype=fmt,
+                       engine=engine if fmt == "float16" else "auto")
     stream = LdmStream(pipe, 64, args.seed + 29)
     plp, ncell, nframes = LDM_FRAME_CELLS, pipe.ncell_c, args.ldm_frames
     c0, s0 = 1234, 345
@@ -333,13 +336,14 @@ def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
     rt = done * ncell / plp * FRAME_SEC / (ms / 1e3)
-    res = {"messages": fmt, "nrows": nrows, "frames_pushed": nframes, "take_every": every,
+    res = {"messages": fmt, "engine": pipe.engine, "nrows": nrows, "frames_pushed": nframes,
+           "take_every": every,
            "core_blocks_decoded": done, "enh_blocks_decoded": done * pipe.ratio,
            "esn0_db": args.ldm_esn0_cti, "core_errors": int(ec), "enh_errors": int(ee),
            "ms_total": round(ms, 2), "realtime_factor": round(rt, 2),
            "zero_filled_frames": int(cti.stats["zero_filled_frames"]),
            "overruns": int(cti.stats["buffer_overrun"])}
-    log(f"[ldm] CTI stream {fmt}: {done} core blocks over {nframes} frames in {ms:.1f} ms = "
+    log(f"[ldm] CTI stream {fmt} ({pipe.engine}): {done} core blocks over {nframes} frames in {ms:.1f} ms = "
         f"{rt:.1f}x real time; errors core {int(ec)}, enhanced {int(ee)}")
     return res
 
@@ -578,8 +582,10 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.workload == "ldm":
-        res = ldm_bench(args, dev, (args.msg,))[args.msg]
-        res["cti_stream"] = ldm_cti_bench(args, dev, args.msg)
+        fname = ("float16_resident" if args.msg == "float16" and args.engine == "resident"
+                 else args.msg)
+        res = ldm_bench(args, dev, (fname,))[fname]
+        res["cti_stream"] = ldm_cti_bench(args, dev, args.msg, engine=args.engine)
         print(json.dumps({
             "metric": "LDM frame real-time factor (core + enhanced demap, decode, SIC)",
             "value": res["realtime_factor"], "unit": "x real time", "n_gpus": 1,
