@@ -283,13 +283,16 @@ def learn_bench(args, dev, steps, eval_blocks, ebn0_points=(1.0, 1.5, 2.0, 2.5, 
             "fer_block_errors": fer}
 
 
-def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4, engine="resident"):
+def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4, engine="auto"):
     """The LDM chain fed as the upstream receives it (lab/m44_ldm.py:1187-1304):
     the combined two-layer cells go through a convolutional time interleaver
     (Nrows 1024), are cut into frames of 1,216,939 cells and pushed one frame
     at a time into the device ring; every ``every`` frames the complete FEC
     Blocks are de-interleaved on the device and decoded by the two-layer
-    chain.  Real time = decoded cells / the frame cell rate."""
+    chain.  Real time = decoded cells / the frame cell rate.  The chunks are
+    small (~150 core blocks), so the default engine is the batched one,
+    which spreads each codeword over the chip; the resident engine (one
+    codeword per compute unit) measured 74x against 123x real time here."""
     import torch
     from atsc3_amd.ldm import LdmPipeline, LdmStream
     from atsc3_amd.ti import CtiStreamDev
@@ -585,7 +588,7 @@ def main():
         fname = ("float16_resident" if args.msg == "float16" and args.engine == "resident"
                  else args.msg)
         res = ldm_bench(args, dev, (fname,))[fname]
-        res["cti_stream"] = ldm_cti_bench(args, dev, args.msg, engine=args.engine)
+        res["cti_stream"] = ldm_cti_bench(args, dev, args.msg)
         print(json.dumps({
             "metric": "LDM frame real-time factor (core + enhanced demap, decode, SIC)",
             "value": res["realtime_factor"], "unit": "x real time", "n_gpus": 1,
