@@ -84,11 +84,12 @@ _SIGS = {
     "atsc3_ldm_ws_doubles": [c_i32, c_i32],
     "atsc3_ldm_cancel_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_f32, c_vp, c_vp, c_vp, c_vp],
-    "atsc3_minsum_resident_ws_bytes": [c_i32, c_i32],
+    "atsc3_minsum_resident_ws_bytes": [c_i32, c_i32, c_i32, c_i32],
+    "atsc3_minsum_resident_lds_rows": [c_i32],
     "atsc3_minsum_resident_grid": [],
     "atsc3_minsum_resident_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
-                                  c_i32, c_i32, c_f32, c_i32, c_vp, ctypes.c_int64, c_vp, c_vp,
-                                  c_vp, c_vp, c_vp],
+                                  c_i32, c_i32, c_i32, c_f32, c_i32, c_i32, c_vp,
+                                  ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
@@ -138,7 +139,7 @@ SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ld
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
-                "atsc3_minsum_resident_grid")
+                "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows")
 
 
 def call(name, *args):

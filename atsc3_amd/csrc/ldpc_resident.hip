@@ -1,47 +1,53 @@
-// Normalized min-sum with the whole codeword resident on one compute unit:
-// FP16 check-to-variable messages, the variable posteriors held in LDS.
+// Normalized min-sum with the whole codeword resident on one compute unit.
 //
 // One workgroup of 1024 threads decodes one codeword at a time and pulls the
 // next from a device queue when it finishes (persistent, no grid barrier:
-// workgroups never wait for each other).  Its working state:
+// workgroups never wait for each other).  Two storage modes share the code:
 //
-//   tot   the posterior of every variable, FP16, in LDS (64800 x 2 bytes =
-//         129,600 bytes of the 160 KB a gfx950 workgroup may hold).  The
-//         check phase reads it with ds_read_u16 -- 64 consecutive positions
-//         of one variable group per wave instruction, conflict-free -- so
-//         the E-sized posterior gather of the flooding schedule never
-//         reaches the memory hierarchy.
-//   state one compressed record per check (min-sum messages of a check take
-//         only two magnitudes): word 0 the FP16 pair (a1, a2) = alpha *
-//         (min1, min2); word 1 the signs of edges 0..24 and the index of the
-//         edge that takes a2 (bits 25..31); a 64-bit third word the signs of
-//         edges 25..88, touched only by checks of degree > 25.  8 bytes per
-//         check in this workgroup's slot in global memory (21,600 checks =
-//         173 KB for rate 10/15), against 2 bytes per EDGE for stored FP16
-//         messages.  Each record is written twice, at positions j and j + 360
-//         of its group's 720-record row, so the variable side reads check
-//         position p + u without reducing it mod 360.
+//   FP16   every posterior in LDS as FP16 (64800 x 2 bytes = 129,600 bytes of
+//          the 160 KB a gfx950 workgroup may hold); check records hold FP16
+//          magnitudes.  The FP16 message form.
+//   FP32   the plain decoder, bit-exact with the reference: posteriors FP32,
+//          the rows of the highest-degree variable groups in LDS (up to 113
+//          rows of 360 floats = 162,720 bytes) and the remaining rows in this
+//          workgroup's slot in global memory; records hold FP32 magnitudes.
+//          The host chooses the LDS rows (for 64800 r10/15: 82% of the edge
+//          reads) and marks each row's place in the tables.
+//
+// The check phase reads posteriors with ds_read (64 consecutive positions of
+// one variable group per wave instruction, conflict-free), so most of the
+// E-sized posterior gather of the flooding schedule never leaves the CU.
+//
+// Check records: min-sum messages of a check take only two magnitudes, so
+// each check keeps one record: a1 = alpha * min1 and a2 = alpha * min2 (FP16:
+// one word a1 | a2 << 16; FP32: two words), word 1 = the signs of edges 0..24
+// and the index of the edge that takes a2 (bits 25..31), and a 64-bit word
+// with the signs of edges 25..88, touched only by checks of degree > 25.
+// Records live in this workgroup's slot in global memory (21,600 checks x 8
+// or 12 bytes for rate 10/15, against 2 or 4 bytes per EDGE for stored
+// messages); the check phase rewrites them in place (only the owning lane
+// reads a record back there) and the variable phase gathers them, 64
+// consecutive checks of one group per wave load.
 //
 // Per iteration (plain semantics, lab/ldpc_kernel.c:82-143):
 //   CHECK  lane j of a wave takes check (g, j): reads its old record and the
 //          posteriors of its edges, counts the syndrome from tot < 0, forms
 //          x = tot - m_old, min1/min2/first argmin/signs and writes the new
-//          record in place (only its own thread ever reads it back here).
+//          record.
 //   (barrier; the syndrome total decides: converged or at `iters` -> bits,
 //    conv, its, bad of the codeword, next codeword)
 //   VAR    lane p of a wave takes variable (t, p): gathers the new record of
-//          each of its checks (coalesced: 64 consecutive checks of one group
-//          per wave load), rebuilds the message, sums in FP32 in ascending
-//          check index (the reference's order: per region, the entries whose
-//          check position wrapped past 359 first, then the rest, each in
-//          table order) and stores tot = fp16(lam + s) in LDS.
+//          each of its checks, rebuilds the message, sums in FP32 in
+//          ascending check index (the reference's order: per region, the
+//          entries whose check position wrapped past 359 first, then the
+//          rest, each in table order) and stores tot = lam + s.
 //
-// Arithmetic: every message is fp16_rn(sat(alpha * min)) with the sign of
-// the plain rule, exactly as the FP16 message format stores it; every sum is
-// FP32 in the reference's order; the posterior is rounded to FP16 (saturated
-// to +-65504) where it is stored.  The checker (oracle/resident_ref.py) runs
-// the same steps in numpy; with FP32 storage in place of FP16 it reproduces
-// the plain FP32 min-sum bit for bit.
+// Arithmetic: every message is alpha * min with the sign of the plain rule
+// (FP16 mode: fp16_rn(sat(alpha * min)), exactly as the FP16 message format
+// stores it); every sum is FP32 in the reference's order; FP16 mode rounds
+// the posterior to FP16 (saturated to +-65504) where it is stored.  The
+// checker (oracle/resident_ref.py) runs the same steps in numpy; its FP32
+// mode reproduces the reference kernel's golden outputs bit for bit.
 #include <hip/hip_fp16.h>
 
 #include "ldpc_qc_common.h"
@@ -51,12 +57,10 @@
 #define RES_NMAX 64800
 #define RES_MAX_DC 89
 #define RES_MAX_DV 32
-#define RES_SIGN_LO 25         // sign bits held in the second record word
-// Records per check-group row: 720 = each record stored twice (positions j
-// and j + 360), so variable-side reads need no mod 360; 360 = stored once.
-#ifndef RES_RROW
-#define RES_RROW 360
-#endif
+#define RES_SIGN_LO 25                  // sign bits held in record word 1
+#define RES_F32_LDS_ROWS 113            // FP32 posterior rows that fit in LDS
+#define RES_F16_LDS_BYTES (RES_NMAX * 2)
+#define RES_F32_LDS_BYTES (RES_F32_LDS_ROWS * ATSC3_Z * 4)
 
 namespace atsc3 {
 namespace {
@@ -65,8 +69,10 @@ namespace {
 struct ResTab {
     int n, ncg, nvg, nbe, max_dc, max_dv, nsc, nsv;
     const QC_CONST int *grp_c;    // per check group: dc | b0 << 7 | has_missing << 18
-    const QC_CONST int *edge_c;   // per base edge: s | t << 9 | (missing j + 1) << 18
-    const QC_CONST int *grp_v;    // per variable group: dv | e0 << 6 | n0 << 17
+    const QC_CONST int *edge_c;   // per base edge: s | row << 9 | (missing j + 1) << 18
+                                  //   | global << 28
+    const QC_CONST int *grp_v;    // per variable group: dv | e0 << 6 | n0 << 17 | row << 23
+                                  //   | global << 31
     const QC_CONST int *edge_v;   // per entry, two words: u | g << 9 | k << 17; missing p + 1
     const QC_CONST int *c_steps;  // check steps: D | G << 7 | start << 10 | generic << 22
     const QC_CONST int *c_tasks;  // task: group | c6 << 8 (rows c6*64 .. c6*64+63)
@@ -82,9 +88,11 @@ struct ResArgs {
     const QC_CONST ResTab *codes;
     int nblk, iters;
     float alpha;
-    long long stride;            // records per workgroup slot (2 per check)
-    unsigned *p0, *p1;
+    long long stride;             // records per workgroup slot
+    long long tstride;            // global posterior floats per workgroup slot
+    unsigned *p0, *p0b, *p1;      // p0b: FP32 mode's a2 word
     unsigned long long *p2;
+    float *totg;
     int *queue;
     uint8_t *bits, *conv;
     int *its, *bad;
@@ -100,45 +108,91 @@ __device__ __forceinline__ float h_val(unsigned short h) {
     return __half2float(__ushort_as_half(h));
 }
 
+// Posterior rows: FP16 mode, row r of the LDS array; FP32 mode, row r of the
+// LDS array or (global) of this workgroup's global slot.
+struct Tot {
+    unsigned char *lds;
+    float *glob;
+};
+
+template <bool F32>
+__device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, unsigned p) {
+    if constexpr (!F32) {
+        return __half2float(reinterpret_cast<const __half *>(T.lds)[row * ATSC3_Z + p]);
+    } else {
+        return glob ? at(T.glob, row * ATSC3_Z + p)
+                    : reinterpret_cast<const float *>(T.lds)[row * ATSC3_Z + p];
+    }
+}
+
+template <bool F32>
+__device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, unsigned p,
+                                       float v) {
+    if constexpr (!F32) {
+        reinterpret_cast<__half *>(T.lds)[row * ATSC3_Z + p] = __ushort_as_half(h_bits_sat(v));
+    } else {
+        if (glob)
+            at(T.glob, row * ATSC3_Z + p) = v;
+        else
+            reinterpret_cast<float *>(T.lds)[row * ATSC3_Z + p] = v;
+    }
+}
+
+// Record planes of this workgroup.
+struct Rec {
+    unsigned *p0, *p0b, *p1;
+    unsigned long long *p2;
+};
+
+// Magnitudes (a1, a2) of a record from its magnitude words.
+template <bool F32>
+__device__ __forceinline__ float rec_a1(unsigned w0, unsigned) {
+    return F32 ? __uint_as_float(w0) : h_val((unsigned short)(w0 & 0xffffu));
+}
+template <bool F32>
+__device__ __forceinline__ float rec_a2(unsigned w0, unsigned w0b) {
+    return F32 ? __uint_as_float(w0b) : h_val((unsigned short)(w0 >> 16));
+}
+// Magnitude words of (a1, a2) = alpha * (m1, m2).
+template <bool F32>
+__device__ __forceinline__ void rec_put_mag(const Rec &R, unsigned r, float a1, float a2) {
+    if constexpr (F32) {
+        at(R.p0, r) = __float_as_uint(a1);
+        at(R.p0b, r) = __float_as_uint(a2);
+    } else {
+        at(R.p0, r) = (unsigned)h_bits_sat(a1) | ((unsigned)h_bits_sat(a2) << 16);
+    }
+}
+
 // Bit 31 set iff edge kk's message is negative (record words w1, w2).
 __device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2, unsigned kk) {
     return (kk < RES_SIGN_LO ? w1 << (31u - kk) : (unsigned)(w2 >> (kk - RES_SIGN_LO)) << 31) &
            0x80000000u;
 }
 
-// Message of edge kk of a check from its record words w0 (a1 | a2 << 16),
-// w1 (signs 0..24 | a2 edge << 25) and w2 (signs 25..88).  a1, a2 are
-// non-negative (alpha >= 0), so setting the sign bit negates.
-__device__ __forceinline__ float rec_msg(unsigned w0, unsigned w1, unsigned long long w2,
-                                         unsigned kk) {
-    const unsigned h = kk == (w1 >> RES_SIGN_LO) ? (w0 >> 16) : w0;
-    const float v = h_val((unsigned short)h);
+// Message of edge kk of a check from its record.  a1, a2 are non-negative
+// (alpha >= 0), so setting the sign bit negates.
+template <bool F32>
+__device__ __forceinline__ float rec_msg(unsigned w0, unsigned w0b, unsigned w1,
+                                         unsigned long long w2, unsigned kk) {
+    float v;
+    if constexpr (F32) {
+        v = __uint_as_float(kk == (w1 >> RES_SIGN_LO) ? w0b : w0);
+    } else {
+        v = h_val((unsigned short)(kk == (w1 >> RES_SIGN_LO) ? (w0 >> 16) : w0));
+    }
     return __uint_as_float(__float_as_uint(v) | rec_sign(w1, w2, kk));
-}
-
-// Record store at row position j and its copy at j + 360.
-template <typename T>
-__device__ __forceinline__ void rec_put(T *plane, unsigned r, T v) {
-    at(plane, r) = v;
-    if (RES_RROW == 2 * ATSC3_Z) at(plane, r + (unsigned)ATSC3_Z) = v;
-}
-
-// Row position of check position p + u (0 <= p, u < 360) whose wrap flag is
-// `wr` (p + u >= 360).
-__device__ __forceinline__ unsigned rec_pos(unsigned pu, bool wr) {
-    return RES_RROW == 2 * ATSC3_Z ? pu : pu - (wr ? (unsigned)ATSC3_Z : 0u);
 }
 
 // G check tasks of degree D <= 16 with every circulant full: the syndrome
 // bits of the current posteriors and, unless `update` is false, the new
 // compressed records.  Every load of the G tasks (old records, posteriors)
 // is issued before the first use.
-template <int D, int G>
-__device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const __half *tot,
-                                                unsigned *p0, unsigned *p1, int start,
-                                                unsigned lane, bool first, bool update,
-                                                float alpha) {
-    unsigned rix[G], w0[G], w1[G];
+template <bool F32, int D, int G>
+__device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const Tot &T,
+                                                const Rec &R, int start, unsigned lane,
+                                                bool first, bool update, float alpha) {
+    unsigned rix[G], w0[G], w0b[G], w1[G];
     bool valid[G];
     float tv[G][D];
 #pragma unroll
@@ -148,27 +202,29 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         const unsigned j = (unsigned)(task >> 8) * 64u + lane;
         valid[gi] = j < (unsigned)ATSC3_Z;
         const unsigned jj = valid[gi] ? j : 0u;
-        rix[gi] = (unsigned)g * RES_RROW + jj;
+        rix[gi] = (unsigned)g * ATSC3_Z + jj;
         w0[gi] = 0u;
+        w0b[gi] = 0u;
         w1[gi] = 0u;
         if (!first) {
-            w0[gi] = at(p0, rix[gi]);
-            w1[gi] = at(p1, rix[gi]);
+            w0[gi] = at(R.p0, rix[gi]);
+            if (F32) w0b[gi] = at(R.p0b, rix[gi]);
+            w1[gi] = at(R.p1, rix[gi]);
         }
         const int b0 = (ct->grp_c[g] >> 7) & 2047;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_c[b0 + k];
             const unsigned p = wrap360(jj + (ew & 511u));
-            tv[gi][k] = __half2float(tot[((ew >> 9) & 511u) * ATSC3_Z + p]);
+            tv[gi][k] = tot_ld<F32>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
     unsigned syn_total = 0u;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const float a1o = h_val((unsigned short)(w0[gi] & 0xffffu));
-        const float a2o = h_val((unsigned short)(w0[gi] >> 16));
+        const float a1o = rec_a1<F32>(w0[gi], w0b[gi]);
+        const float a2o = rec_a2<F32>(w0[gi], w0b[gi]);
         const unsigned idxo = w1[gi] >> RES_SIGN_LO;
         bool syn = false, par = false;
         unsigned idx = 0u, neg = 0u;
@@ -190,9 +246,8 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         }
         if (update && valid[gi]) {
             const unsigned sg = par ? ~neg : neg;
-            rec_put(p0, rix[gi],
-                    (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16));
-            rec_put(p1, rix[gi], (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
+            rec_put_mag<F32>(R, rix[gi], m1 * alpha, m2 * alpha);
+            at(R.p1, rix[gi]) = (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
         }
         syn_total += (valid[gi] && syn) ? 1u : 0u;
     }
@@ -201,21 +256,22 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
 
 // Any check degree up to RES_MAX_DC (records of degree > 25 use the third
 // word): the same steps as check_multi with a runtime edge loop.
-__device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, unsigned *p0,
-                                  unsigned *p1, unsigned long long *p2, int g, int b0, int dc,
-                                  bool hasmiss, unsigned j, bool valid, bool first, bool update,
-                                  float alpha) {
+template <bool F32>
+__device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const Rec &R, int g,
+                                  int b0, int dc, bool hasmiss, unsigned j, bool valid,
+                                  bool first, bool update, float alpha) {
     const unsigned jj = valid ? j : 0u;
-    const unsigned rix = (unsigned)g * RES_RROW + jj;
-    unsigned w0 = 0u, w1 = 0u;
+    const unsigned rix = (unsigned)g * ATSC3_Z + jj;
+    unsigned w0 = 0u, w0b = 0u, w1 = 0u;
     unsigned long long w2 = 0ull;
     if (!first) {
-        w0 = at(p0, rix);
-        w1 = at(p1, rix);
-        if (dc > RES_SIGN_LO) w2 = at(p2, rix);
+        w0 = at(R.p0, rix);
+        if (F32) w0b = at(R.p0b, rix);
+        w1 = at(R.p1, rix);
+        if (dc > RES_SIGN_LO) w2 = at(R.p2, rix);
     }
-    const float a1o = h_val((unsigned short)(w0 & 0xffffu));
-    const float a2o = h_val((unsigned short)(w0 >> 16));
+    const float a1o = rec_a1<F32>(w0, w0b);
+    const float a2o = rec_a2<F32>(w0, w0b);
     const unsigned idxo = w1 >> RES_SIGN_LO;
     unsigned syn = 0u, par = 0u, idx = 0u, neg_lo = 0u;
     unsigned long long neg_hi = 0ull;
@@ -224,7 +280,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
         const unsigned ew = (unsigned)ct->edge_c[b0 + k];
         if (hasmiss && ((ew >> 18) & 1023u) == jj + 1u) continue;
         const unsigned p = wrap360(jj + (ew & 511u));
-        const float tv = __half2float(tot[((ew >> 9) & 511u) * ATSC3_Z + p]);
+        const float tv = tot_ld<F32>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
         const float ao = (unsigned)k == idxo ? a2o : a1o;
         const float mo = __uint_as_float(__float_as_uint(ao) | rec_sign(w1, w2, (unsigned)k));
         const float x = tv - mo;
@@ -246,10 +302,9 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
     }
     if (update && valid) {
         const unsigned sg_lo = par ? ~neg_lo : neg_lo;
-        rec_put(p0, rix,
-                (unsigned)h_bits_sat(m1 * alpha) | ((unsigned)h_bits_sat(m2 * alpha) << 16));
-        rec_put(p1, rix, (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
-        if (dc > RES_SIGN_LO) rec_put(p2, rix, par ? ~neg_hi : neg_hi);
+        rec_put_mag<F32>(R, rix, m1 * alpha, m2 * alpha);
+        at(R.p1, rix) = (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
+        if (dc > RES_SIGN_LO) at(R.p2, rix) = par ? ~neg_hi : neg_hi;
     }
     return valid ? syn : 0u;
 }
@@ -265,12 +320,12 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const __half *tot, 
 // (Type A codes, n0 > 0 entries); else one region.  With one region of two
 // entries the order does not matter: (0 + a) + b == (0 + b) + a in IEEE
 // arithmetic, signed zeros included.
-template <int D, int G, bool TA>
-__device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot,
-                                          const float *lam, const unsigned *p0,
-                                          const unsigned *p1, int start, unsigned lane) {
-    unsigned w0[G][D], w1[G][D], dst[G];
-    bool wr[G][D];
+template <bool F32, int D, int G, bool TA>
+__device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &T,
+                                          const float *lam, const Rec &R, int start,
+                                          unsigned lane) {
+    unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G];
+    bool wr[G][D], glob[G];
     float lv[G];
     bool valid[G];
     int e0[G], n0[G];
@@ -281,19 +336,22 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
         const unsigned p = (unsigned)(task >> 8) * 64u + lane;
         valid[gi] = p < (unsigned)ATSC3_Z;
         const unsigned pp = valid[gi] ? p : 0u;
-        dst[gi] = (unsigned)t * ATSC3_Z + pp;
-        const int vw = ct->grp_v[t];
+        dst[gi] = pp;
+        const unsigned vw = (unsigned)ct->grp_v[t];
         e0[gi] = (vw >> 6) & 2047;
         n0[gi] = TA ? (vw >> 17) & 63 : 0;
-        lv[gi] = at(lam, dst[gi]);
+        row[gi] = (vw >> 23) & 255u;
+        glob[gi] = vw >> 31;
+        lv[gi] = at(lam, (unsigned)t * ATSC3_Z + pp);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u;
             wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
-            const unsigned r = g * RES_RROW + rec_pos(pp + u, wr[gi][k]);
-            w0[gi][k] = at(p0, r);
-            w1[gi][k] = at(p1, r);
+            const unsigned r = g * ATSC3_Z + pp + u - (wr[gi][k] ? (unsigned)ATSC3_Z : 0u);
+            w0[gi][k] = at(R.p0, r);
+            if (F32) w0b[gi][k] = at(R.p0b, r);
+            w1[gi][k] = at(R.p1, r);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -303,12 +361,17 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned kk = ((unsigned)ct->edge_v[2 * (e0[gi] + k)] >> 17) & 127u;
-            const unsigned h = kk == (w1[gi][k] >> RES_SIGN_LO) ? (w0[gi][k] >> 16) : w0[gi][k];
-            m[k] = __uint_as_float(__float_as_uint(h_val((unsigned short)h)) |
-                                   ((w1[gi][k] << (31u - kk)) & 0x80000000u));
+            float v;
+            if constexpr (F32) {
+                v = __uint_as_float(kk == (w1[gi][k] >> RES_SIGN_LO) ? w0b[gi][k] : w0[gi][k]);
+            } else {
+                v = h_val((unsigned short)(kk == (w1[gi][k] >> RES_SIGN_LO) ? (w0[gi][k] >> 16)
+                                                                          : w0[gi][k]));
+            }
+            m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - kk)) & 0x80000000u));
         }
         float s = 0.0f;
-        if (!TA && D == 2) {
+        if constexpr (!TA && D == 2) {
             s = (s + m[0]) + m[1];
         } else {
             if (TA) {
@@ -326,16 +389,17 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, __half *tot
             for (int k = 0; k < D; ++k)
                 if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         }
-        if (valid[gi]) tot[dst[gi]] = __ushort_as_half(h_bits_sat(lv[gi] + s));
+        if (valid[gi]) tot_st<F32>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
     }
 }
 
 // Any degree: one record load per entry, visited in the summation order
 // directly (suffix then prefix of each region).
-__device__ void var_generic(const QC_CONST ResTab *ct, __half *tot, const float *lam,
-                            const unsigned *p0, const unsigned *p1,
-                            const unsigned long long *p2, int t, int e0, int dv, int n0,
-                            unsigned p, bool valid) {
+template <bool F32>
+__device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float *lam,
+                            const Rec &R, int t, unsigned p, bool valid) {
+    const unsigned vw = (unsigned)ct->grp_v[t];
+    const int dv = vw & 63, e0 = (vw >> 6) & 2047, n0 = (vw >> 17) & 63;
     const unsigned pp = valid ? p : 0u;
     const float lv = lam[(unsigned)t * ATSC3_Z + pp];
     float s = 0.0f;
@@ -352,28 +416,35 @@ __device__ void var_generic(const QC_CONST ResTab *ct, __half *tot, const float 
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0 + k)];
             const unsigned vm = (unsigned)ct->edge_v[2 * (e0 + k) + 1];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
-            const unsigned rix = g * RES_RROW + rec_pos(pp + u, pp + u >= (unsigned)ATSC3_Z);
-            const unsigned long long w2 = kk >= RES_SIGN_LO ? at(p2, rix) : 0ull;
-            const float mv = rec_msg(at(p0, rix), at(p1, rix), w2, kk);
+            const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
+            const unsigned long long w2 = kk >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
+            const float mv = rec_msg<F32>(at(R.p0, rix), F32 ? at(R.p0b, rix) : 0u,
+                                          at(R.p1, rix), w2, kk);
             if (vm != pp + 1u) s += mv;
         }
     }
-    if (valid) tot[(unsigned)t * ATSC3_Z + p] = __ushort_as_half(h_bits_sat(lv + s));
+    if (valid) tot_st<F32>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
 }
 
 __global__ void resident_init(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
+template <bool F32>
 __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
-    __shared__ __half tot[RES_NMAX];
+    __shared__ __align__(16) unsigned char lds[F32 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
     __shared__ int s_blk;
     __shared__ int s_unsat[2];
     const unsigned tid = threadIdx.x, lane = tid & 63u;
     const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
-    unsigned *p0 = a.p0 + (size_t)blockIdx.x * a.stride;
-    unsigned *p1 = a.p1 + (size_t)blockIdx.x * a.stride;
-    unsigned long long *p2 = a.p2 + (size_t)blockIdx.x * a.stride;
+    Rec R;
+    R.p0 = a.p0 + (size_t)blockIdx.x * a.stride;
+    R.p0b = F32 ? a.p0b + (size_t)blockIdx.x * a.stride : nullptr;
+    R.p1 = a.p1 + (size_t)blockIdx.x * a.stride;
+    R.p2 = a.p2 + (size_t)blockIdx.x * a.stride;
+    Tot T;
+    T.lds = lds;
+    T.glob = F32 ? a.totg + (size_t)blockIdx.x * a.tstride : nullptr;
     for (;;) {
         if (tid == 0) {
             s_blk = atomicAdd(a.queue, 1);
@@ -384,11 +455,17 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         const int blk = __builtin_amdgcn_readfirstlane(s_blk);
         if (blk >= a.nblk) return;
         const QC_CONST ResTab *ct = &a.codes[a.blk_code[blk]];
-        const int n = ct->n;
+        const int nvg = ct->nvg;
         const long long off = a.blk_off[blk];
         const float *lam = a.llr + off;
-        for (int v = (int)tid; v < n; v += RES_WG)
-            tot[v] = __ushort_as_half(h_bits_sat(lam[v]));
+        // tot = lam, row by row (each 64-position task of a group on one wave)
+        for (int task = wave; task < nvg * 6; task += RES_WAVES) {
+            const int t = task / 6;
+            const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
+            const unsigned vw = (unsigned)ct->grp_v[t];
+            if (p < (unsigned)ATSC3_Z)
+                tot_st<F32>(T, (vw >> 23) & 255u, vw >> 31, p, lam[(unsigned)t * ATSC3_Z + p]);
+        }
         __syncthreads();
         for (int it = 0;; ++it) {
             if (tid == 0) s_unsat[(it + 1) & 1] = 0;
@@ -402,19 +479,19 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     const int g = task & 255;
                     const unsigned j = (unsigned)(task >> 8) * 64u + lane;
                     const int gw = ct->grp_c[g];
-                    unsat += check_generic(ct, tot, p0, p1, p2, g, (gw >> 7) & 2047, gw & 127,
-                                           (gw >> 18) & 1, j, j < (unsigned)ATSC3_Z, first,
-                                           update, a.alpha);
+                    unsat += check_generic<F32>(ct, T, R, g, (gw >> 7) & 2047, gw & 127,
+                                                (gw >> 18) & 1, j, j < (unsigned)ATSC3_Z, first,
+                                                update, a.alpha);
                     continue;
                 }
                 switch (G * 128 + D) {
-#define RES_C1(DD)                                                                        \
-    case 128 + DD:                                                                        \
-        unsat += check_multi<DD, 1>(ct, tot, p0, p1, start, lane, first, update, a.alpha); \
+#define RES_C1(DD)                                                                           \
+    case 128 + DD:                                                                           \
+        unsat += check_multi<F32, DD, 1>(ct, T, R, start, lane, first, update, a.alpha);     \
         break;
-#define RES_C2(DD)                                                                        \
-    case 256 + DD:                                                                        \
-        unsat += check_multi<DD, 2>(ct, tot, p0, p1, start, lane, first, update, a.alpha); \
+#define RES_C2(DD)                                                                           \
+    case 256 + DD:                                                                           \
+        unsat += check_multi<F32, DD, 2>(ct, T, R, start, lane, first, update, a.alpha);     \
         break;
                     QC_CASES16(RES_C1)
                     QC_CASES16(RES_C2)
@@ -428,8 +505,14 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             const int total = s_unsat[it & 1];
             if (total == 0 || it == a.iters) {
                 uint8_t *bout = a.bits + off;
-                for (int v = (int)tid; v < n; v += RES_WG)
-                    bout[v] = (uint8_t)(__half2float(tot[v]) < 0.0f);
+                for (int task = wave; task < nvg * 6; task += RES_WAVES) {
+                    const int t = task / 6;
+                    const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
+                    const unsigned vw = (unsigned)ct->grp_v[t];
+                    if (p < (unsigned)ATSC3_Z)
+                        bout[(unsigned)t * ATSC3_Z + p] =
+                            (uint8_t)(tot_ld<F32>(T, (vw >> 23) & 255u, vw >> 31, p) < 0.0f);
+                }
                 if (tid == 0) {
                     a.conv[blk] = (uint8_t)(total == 0);
                     a.its[blk] = total == 0 ? it : a.iters;
@@ -442,35 +525,32 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 const int D = sw & 127, G = (sw >> 7) & 7, start = (sw >> 10) & 4095;
                 if ((sw >> 22) & 1) {
                     const int task = ct->v_tasks[start];
-                    const int t = task & 255;
                     const unsigned p = (unsigned)(task >> 8) * 64u + lane;
-                    const int vw = ct->grp_v[t];
-                    var_generic(ct, tot, lam, p0, p1, p2, t, (vw >> 6) & 2047, vw & 63,
-                                (vw >> 17) & 63, p, p < (unsigned)ATSC3_Z);
+                    var_generic<F32>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
                     continue;
                 }
                 const bool ta = (sw >> 23) & 1;
                 switch ((ta ? 1024 : 0) + G * 128 + D) {
 #define RES_V1(DD)                                                      \
     case 128 + DD:                                                      \
-        var_multi<DD, 1, false>(ct, tot, lam, p0, p1, start, lane);     \
+        var_multi<F32, DD, 1, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 128 + DD:                                               \
-        var_multi<DD, 1, true>(ct, tot, lam, p0, p1, start, lane);      \
+        var_multi<F32, DD, 1, true>(ct, T, lam, R, start, lane);        \
         break;
 #define RES_V2(DD)                                                      \
     case 256 + DD:                                                      \
-        var_multi<DD, 2, false>(ct, tot, lam, p0, p1, start, lane);     \
+        var_multi<F32, DD, 2, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 256 + DD:                                               \
-        var_multi<DD, 2, true>(ct, tot, lam, p0, p1, start, lane);      \
+        var_multi<F32, DD, 2, true>(ct, T, lam, R, start, lane);        \
         break;
 #define RES_V4(DD)                                                      \
     case 512 + DD:                                                      \
-        var_multi<DD, 4, false>(ct, tot, lam, p0, p1, start, lane);     \
+        var_multi<F32, DD, 4, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 512 + DD:                                               \
-        var_multi<DD, 4, true>(ct, tot, lam, p0, p1, start, lane);      \
+        var_multi<F32, DD, 4, true>(ct, T, lam, R, start, lane);        \
         break;
                     QC_CASES16(RES_V1)
                     QC_CASES8(RES_V2)
@@ -489,13 +569,34 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
     }
 }
 
+struct WsLayout {
+    int64_t stride, tstride, p0, p0b, p1, p2, totg, bytes;
+    WsLayout(int64_t grid, int64_t max_checks, int64_t max_grows, bool f32) {
+        stride = (max_checks + 63) & ~(int64_t)63;
+        tstride = f32 ? ((max_grows * ATSC3_Z + 63) & ~(int64_t)63) : 0;
+        p0 = 256;
+        p1 = p0 + grid * stride * 4;
+        p2 = p1 + grid * stride * 4;
+        p0b = p2 + grid * stride * 8;
+        totg = p0b + (f32 ? grid * stride * 4 : 0);
+        bytes = totg + grid * tstride * 4;
+    }
+};
+
 }  // namespace
 }  // namespace atsc3
 
-extern "C" int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks) {
-    if (grid < 1 || max_checks < 1) return ATSC3_E_ARG;
-    const int64_t stride = ((int64_t)max_checks * (RES_RROW / ATSC3_Z) + 63) & ~(int64_t)63;
-    return 256 + (int64_t)grid * stride * 16;
+extern "C" int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,
+                                                  int32_t max_global_rows, int32_t msg_fmt) {
+    if (grid < 1 || max_checks < 1 || max_global_rows < 0 || (msg_fmt != 0 && msg_fmt != 1))
+        return ATSC3_E_ARG;
+    return atsc3::WsLayout(grid, max_checks, max_global_rows, msg_fmt == 0).bytes;
+}
+
+extern "C" int32_t atsc3_minsum_resident_lds_rows(int32_t msg_fmt) {
+    if (msg_fmt == 0) return RES_F32_LDS_ROWS;
+    if (msg_fmt == 1) return RES_NMAX / ATSC3_Z;
+    return ATSC3_E_ARG;
 }
 
 extern "C" int32_t atsc3_minsum_resident_grid(void) {
@@ -506,25 +607,24 @@ extern "C" int32_t atsc3_minsum_resident_grid(void) {
     return cus > 0 ? cus : ATSC3_E_HIP;
 }
 
-extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
-                                             const int64_t *blk_off, const int32_t *blk_code,
-                                             const void *codes, int32_t ncode,
-                                             int32_t max_checks, int32_t max_n, int32_t max_dc,
-                                             int32_t max_dv, int32_t iters, float alpha,
-                                             int32_t grid, void *ws, int64_t ws_bytes,
-                                             uint8_t *bits, uint8_t *conv, int32_t *its,
-                                             int32_t *bad, void *stream) {
+extern "C" int32_t atsc3_minsum_resident_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
+    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
+    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
+    int32_t *bad, void *stream) {
     if (nblk < 0 || iters < 0 || iters >= (1 << 24) || !(alpha >= 0.0f)) return ATSC3_E_ARG;
+    if (msg_fmt != 0 && msg_fmt != 1) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!llr || !blk_off || !blk_code || !codes || !ws || !bits || !conv || !its || !bad)
         return ATSC3_E_ARG;
-    if (ncode < 1 || max_checks < 1 || grid < 1) return ATSC3_E_ARG;
+    if (ncode < 1 || max_checks < 1 || grid < 1 || max_global_rows < 0) return ATSC3_E_ARG;
     if (max_n < 1 || max_n > RES_NMAX) return ATSC3_E_SIZE;
     if (max_dc < 1 || max_dc > RES_MAX_DC || max_dv < 1 || max_dv > RES_MAX_DV)
         return ATSC3_E_DEGREE;
-    const int64_t need = atsc3_minsum_resident_ws_bytes(grid, max_checks);
-    if (need < 0 || ws_bytes < need) return ATSC3_E_SIZE;
-    const int64_t stride = ((int64_t)max_checks * (RES_RROW / ATSC3_Z) + 63) & ~(int64_t)63;
+    const bool f32 = msg_fmt == 0;
+    const atsc3::WsLayout L(grid, max_checks, max_global_rows, f32);
+    if (ws_bytes < L.bytes) return ATSC3_E_SIZE;
     char *base = static_cast<char *>(ws);
     atsc3::ResArgs a{};
     a.llr = llr;
@@ -534,11 +634,14 @@ extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
     a.nblk = nblk;
     a.iters = iters;
     a.alpha = alpha;
-    a.stride = stride;
+    a.stride = L.stride;
+    a.tstride = L.tstride;
     a.queue = reinterpret_cast<int *>(base);
-    a.p0 = reinterpret_cast<unsigned *>(base + 256);
-    a.p1 = a.p0 + (size_t)grid * stride;
-    a.p2 = reinterpret_cast<unsigned long long *>(a.p1 + (size_t)grid * stride);
+    a.p0 = reinterpret_cast<unsigned *>(base + L.p0);
+    a.p1 = reinterpret_cast<unsigned *>(base + L.p1);
+    a.p2 = reinterpret_cast<unsigned long long *>(base + L.p2);
+    a.p0b = f32 ? reinterpret_cast<unsigned *>(base + L.p0b) : nullptr;
+    a.totg = f32 ? reinterpret_cast<float *>(base + L.totg) : nullptr;
     a.bits = bits;
     a.conv = conv;
     a.its = its;
@@ -546,6 +649,9 @@ extern "C" int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
     hipStream_t st = atsc3::as_stream(stream);
     hipLaunchKernelGGL(atsc3::resident_init, dim3(1), dim3(64), 0, st, a.queue);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
-    hipLaunchKernelGGL(atsc3::resident_kernel, dim3(grid), dim3(RES_WG), 0, st, a);
+    if (f32)
+        hipLaunchKernelGGL(atsc3::resident_kernel<true>, dim3(grid), dim3(RES_WG), 0, st, a);
+    else
+        hipLaunchKernelGGL(atsc3::resident_kernel<false>, dim3(grid), dim3(RES_WG), 0, st, a);
     return atsc3::launch_status();
 }

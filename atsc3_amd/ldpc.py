@@ -134,8 +134,9 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     engine: "qc" (fused 360-lifted kernel, one workgroup per codeword),
     "phase" (the same decode as per-iteration phase kernels, for large
     batches), "batch" (the persistent batched engine, atsc3_amd.batch),
-    "csr" (generic kernel; FP32 only), "resident" (FP16 only: one
-    workgroup per codeword with FP16 posteriors in LDS, atsc3_amd.resident)
+    "csr" (generic kernel; FP32 only), "resident" (FP16 or FP32: one
+    workgroup per codeword with its posteriors in LDS, atsc3_amd.resident;
+    FP32 is bit-exact with the plain decoder)
     or "auto" (the QC engines by batch size and message format when the
     code has a QC form, else csr).
     """
@@ -164,12 +165,13 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
         else:
             engine = "phase" if fmt == 0 else "batch"
     if engine == "resident":
-        if fmt != 1:
-            raise ValueError("the resident engine stores FP16 messages only")
+        if fmt == 2:
+            raise ValueError("the resident engine stores FP16 or FP32 records")
         if msg_scale is not None:
             raise ValueError("msg_scale applies to INT8 messages only")
         from .resident import decode_resident_dev
-        return decode_resident_dev(llr, code, iters, alpha)
+        return decode_resident_dev(llr, code, iters, alpha,
+                                   msg_dtype="float32" if fmt == 0 else "float16")
     if engine == "batch":
         if tabs.qc is None:
             raise ValueError("this code has no QC form with variable degree <= 32")

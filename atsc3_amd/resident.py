@@ -77,13 +77,22 @@ def _pack_tasks(deg, generic, pack):
     return tasks, np.asarray(steps, np.int64)
 
 
+F32_LDS_ROWS = 113       # FP32 posterior rows (360 floats) a workgroup holds in LDS
+
+
 @functools.lru_cache(maxsize=None)
-def _host_tables(code):
+def _host_tables(code, f32=False):
     """int32 tables of the resident kernel, from the code's QC form.
 
     grp_c[g]      dc | b0 << 7 | has_missing << 18  (check groups)
-    edge_c[b]     s | t << 9 | (missing j + 1) << 18  (base edges, check-major)
-    grp_v[t]      dv | e0 << 6 | n0 << 17             (variable groups)
+    edge_c[b]     s | row << 9 | (missing j + 1) << 18 | global << 28  (base
+                  edges, check-major; row: where the variable group's
+                  posteriors live)
+    grp_v[t]      dv | e0 << 6 | n0 << 17 | row << 23 | global << 31
+                  (variable groups)
+    FP16 (f32 False): every group's row is its index, in LDS.  FP32: the
+    F32_LDS_ROWS groups of highest degree (most edge reads) get LDS rows, the
+    others rows of the workgroup's global slot.
     edge_v[2e]    u | g << 9 | k << 17: entry e of the variable list
                   (summation order at p = 0) meets check (g, (p + u) % 360)
                   as that check's edge k
@@ -122,25 +131,40 @@ def _host_tables(code):
         first, cnt = (w >> 10) & 4095, (w >> 7) & 7
         return any(ta[v_tasks[first + i] & 255] for i in range(cnt))
     v_steps = np.array([w | (1 << 23 if _ta(w) else 0) for w in v_steps], np.int64)
-    i32 = lambda a: np.ascontiguousarray(a, np.int32)
-    return dict(n=code.n, ncg=len(dc), nvg=len(dv), nbe=nbe, max_dc=int(dc.max()),
+    nvg = len(dv)
+    row = np.arange(nvg, dtype=np.int64)
+    glob = np.zeros(nvg, np.int64)
+    if f32:
+        order = np.lexsort((np.arange(nvg), -dv))
+        nl = min(nvg, F32_LDS_ROWS)
+        row[order[:nl]] = np.arange(nl)
+        row[order[nl:]] = np.arange(nvg - nl)
+        glob[order[nl:]] = 1
+    t_of_b = (be >> 9) & 511
+    edge_c = (be & 511) | (row[t_of_b] << 9) | (miss << 18) | (glob[t_of_b] << 28)
+    grp_v = dv | (vg[:-1] << 6) | (n0 << 17) | (row << 23) | (glob << 31)
+    i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int64).astype(np.uint32).view(np.int32))
+    return dict(n=code.n, ncg=len(dc), nvg=nvg, nbe=nbe, max_dc=int(dc.max()),
+                global_rows=int(glob.sum()),
                 max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
                 c_steps=i32(c_steps), c_tasks=i32(c_tasks), v_steps=i32(v_steps),
                 v_tasks=i32(v_tasks),
                 grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
-                edge_c=i32(be & ((1 << 28) - 1)),
-                grp_v=i32(dv | (vg[:-1] << 6) | (n0 << 17)),
+                edge_c=i32(edge_c),
+                grp_v=i32(grp_v),
                 edge_v=i32(np.stack([vu | (g_of_b[vb] << 9) | (k_of_b[vb] << 17), vmiss],
                                     1).ravel()))
 
 
 class ResidentBank:
-    """Device tables of several codes plus their descriptor array."""
+    """Device tables of several codes plus their descriptor array, for one
+    storage mode (msg_fmt 1 = FP16, 0 = FP32)."""
 
-    def __init__(self, codes, device):
+    def __init__(self, codes, device, msg_fmt=1):
         self.device = torch.device(device)
         self.codes = list(codes)
-        self.host = [_host_tables(c) for c in self.codes]
+        self.fmt = int(msg_fmt)
+        self.host = [_host_tables(c, self.fmt == 0) for c in self.codes]
         self.dev = []
         arr = (_ResCode * len(self.codes))()
         for i, h in enumerate(self.host):
@@ -157,6 +181,7 @@ class ResidentBank:
         self.max_n = max(h["n"] for h in self.host)
         self.max_dc = max(h["max_dc"] for h in self.host)
         self.max_dv = max(h["max_dv"] for h in self.host)
+        self.max_global_rows = max(h["global_rows"] for h in self.host)
 
     def __len__(self):
         return len(self.codes)
@@ -177,7 +202,14 @@ class ResidentPlan:
     LLRs occupy ``llr[off[b] : off[b] + n_b]`` of the flat ``llr`` buffer,
     its bits the same range of ``bits``."""
 
-    def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None):
+    def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
+                 msg_dtype="float16"):
+        """msg_dtype "float16": the FP16 form; "float32": the plain decoder,
+        bit-exact with the reference kernel."""
+        if msg_dtype not in ("float16", "float32"):
+            raise ValueError("the resident engine stores float16 or float32 records")
+        self.msg_dtype = msg_dtype
+        self.fmt = 1 if msg_dtype == "float16" else 0
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -190,7 +222,7 @@ class ResidentPlan:
             else:
                 uniq.append(c)
                 idx.append(len(uniq) - 1)
-        self.bank = ResidentBank(uniq, self.device)
+        self.bank = ResidentBank(uniq, self.device, self.fmt)
         self.nblk = len(idx)
         self.iters, self.alpha = int(iters), float(np.float32(alpha))
         ns = np.array([uniq[i].n for i in idx], np.int64)
@@ -209,7 +241,8 @@ class ResidentPlan:
         self.conv = torch.empty(self.nblk, dtype=torch.uint8, device=dev)
         self.its = torch.empty(self.nblk, dtype=torch.int32, device=dev)
         self.bad = torch.empty(self.nblk, dtype=torch.int32, device=dev)
-        nbytes = _lib.lib().atsc3_minsum_resident_ws_bytes(self.grid, self.bank.max_checks)
+        nbytes = _lib.lib().atsc3_minsum_resident_ws_bytes(self.grid, self.bank.max_checks,
+                                                            self.bank.max_global_rows, self.fmt)
         if nbytes <= 0:
             raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
@@ -229,8 +262,9 @@ class ResidentPlan:
         b = self.bank
         _lib.call("atsc3_minsum_resident_dev", src.data_ptr(), self.nblk,
                   self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
-                  len(b), b.max_checks, b.max_n, b.max_dc, b.max_dv, self.iters, self.alpha,
-                  self.grid, self.ws.data_ptr(), self.ws_bytes, bits.data_ptr(),
+                  len(b), b.max_checks, b.max_global_rows, b.max_n, b.max_dc, b.max_dv,
+                  self.iters, self.alpha, self.fmt, self.grid, self.ws.data_ptr(),
+                  self.ws_bytes, bits.data_ptr(),
                   conv.data_ptr(), its.data_ptr(), bad.data_ptr(), _stream_ptr(self.device))
         return bits, conv, its, bad
 
@@ -242,8 +276,9 @@ _PLANS = {}            # a few recent plans (LRU): callers alternate codes, e.g.
 _PLANS_MAX = 8
 
 
-def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
+def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16"):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
+    msg_dtype "float32" is the plain decoder (bit-exact with the reference).
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -263,10 +298,10 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None):
     bad = torch.empty(nblk, dtype=torch.int32, device=dev)
     if nblk == 0:
         return bits, conv, its, bad
-    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev))
+    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype)
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
-        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype)
     _PLANS[key] = plan
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))

@@ -177,26 +177,35 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
                                int32_t *its, int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
- * Resident FP16 min-sum: the same decode calls as atsc3_minsum_batch_dev
- * replaces (lab/m9_fast.py:843-925, lab/m44_ldm.py:2497-2576 over
- * lab/ldpc_kernel.c:65-146), one workgroup per codeword with the codeword's
- * variable posteriors held in LDS as FP16 and each check's messages as one
- * compressed record (FP16 alpha*min1, alpha*min2, the edge taking min2, the
- * signs).  Messages are the FP16 message format's values and every sum is
- * FP32 in the reference's order; the posterior is rounded to FP16 where it
- * is stored.  Codewords of mixed codes (n <= 64800, check degree <= 89,
- * variable degree <= 32: every A/322 code) are taken from a device queue by `grid`
+ * Resident min-sum: the same decode calls as atsc3_minsum_batch_dev replaces
+ * (lab/m9_fast.py:843-925, lab/m44_ldm.py:2497-2576 over
+ * lab/ldpc_kernel.c:65-146), one workgroup per codeword holding the
+ * codeword's variable posteriors in LDS and each check's messages as one
+ * compressed record (alpha*min1, alpha*min2, the edge taking min2, the
+ * signs).  Two storage modes:
+ *   msg_fmt 1  FP16: posteriors in LDS as FP16, FP16 record magnitudes (the
+ *              FP16 message format's values); sums FP32 in the reference's
+ *              order, the posterior rounded to FP16 where it is stored.
+ *   msg_fmt 0  FP32, bit-exact with lab/ldpc_kernel.c (bits, conv, its,
+ *              bad): FP32 posteriors, the rows of the variable groups whose
+ *              grp_v word says so in LDS (at most
+ *              atsc3_minsum_resident_lds_rows(0) rows), the other rows in the
+ *              workgroup's global slot; FP32 record magnitudes.
+ * Codewords of mixed codes (n <= 64800, check degree <= 89, variable degree
+ * <= 32: every A/322 code) are taken from a device queue by `grid`
  * workgroups (atsc3_minsum_resident_grid() = one per compute unit; any
  * grid >= 1 is valid: workgroups never wait for each other).
  *
- *   codes      device array of ncode atsc3_res_code
+ *   codes      device array of ncode atsc3_res_code built for this msg_fmt
  *   blk_code, blk_off  as atsc3_minsum_batch_dev
  *   max_checks largest ncg * 360 over the codes
+ *   max_global_rows  largest number of posterior rows a code keeps in the
+ *              global slot (0 for msg_fmt 1)
  *   max_n, max_dc, max_dv  maxima over the codes
  *   alpha      normalization, >= 0 (the records hold magnitudes)
  *   ws         device workspace of atsc3_minsum_resident_ws_bytes(grid,
- *              max_checks) bytes (queue word + per-workgroup records, each
- *              check's record stored twice)
+ *              max_checks, max_global_rows, msg_fmt) bytes (queue word,
+ *              per-workgroup records and posterior rows)
  * Outputs bits/conv/its/bad with the plain semantics (its = iteration at
  * which the syndrome read zero, else iters; bad = unsatisfied checks).
  * Two launches on `stream` (queue reset, decode).
@@ -204,28 +213,34 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
 typedef struct atsc3_res_code {
     int32_t n, ncg, nvg, nbe, max_dc, max_dv, nsc, nsv;
     const int32_t *grp_c;   /* per check group: dc | b0 << 7 | has_missing << 18 */
-    const int32_t *edge_c;  /* per base edge: s | t << 9 | (missing j + 1) << 18   */
-    const int32_t *grp_v;   /* per variable group: dv | e0 << 6 | n0 << 17        */
+    const int32_t *edge_c;  /* per base edge: s | row << 9 | (missing j + 1) << 18
+                               | global << 28 (row: the variable group's posterior
+                               row in LDS, or in the global slot)                 */
+    const int32_t *grp_v;   /* per variable group: dv | e0 << 6 | n0 << 17 |
+                               row << 23 | global << 31                           */
     const int32_t *edge_v;  /* per entry, 2 words: u | g << 9 | k << 17;
                                missing variable position + 1                     */
     const int32_t *c_steps; /* [nsc] check steps: D | G << 7 | start << 10 |
                                generic << 22, over c_tasks                       */
     const int32_t *c_tasks; /* group | c6 << 8: rows c6*64 .. c6*64+63         */
-    const int32_t *v_steps; /* [nsv] variable steps, same packing, over v_tasks */
+    const int32_t *v_steps; /* [nsv] variable steps, same packing, over v_tasks,
+                               | Type A region << 23                             */
     const int32_t *v_tasks;
 } atsc3_res_code;
 
-int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks);
+int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,
+                                       int32_t max_global_rows, int32_t msg_fmt);
+int32_t atsc3_minsum_resident_lds_rows(int32_t msg_fmt);
 int32_t atsc3_minsum_resident_grid(void);
 int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
                                   const int64_t *blk_off, const int32_t *blk_code,
                                   const void *codes, int32_t ncode,
-                                  int32_t max_checks, int32_t max_n,
-                                  int32_t max_dc, int32_t max_dv, int32_t iters,
-                                  float alpha, int32_t grid, void *ws,
-                                  int64_t ws_bytes, uint8_t *bits,
-                                  uint8_t *conv, int32_t *its, int32_t *bad,
-                                  void *stream);
+                                  int32_t max_checks, int32_t max_global_rows,
+                                  int32_t max_n, int32_t max_dc, int32_t max_dv,
+                                  int32_t iters, float alpha, int32_t msg_fmt,
+                                  int32_t grid, void *ws, int64_t ws_bytes,
+                                  uint8_t *bits, uint8_t *conv, int32_t *its,
+                                  int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
  * Max-log NUC demapper with fused bit de-interleave.

@@ -112,6 +112,37 @@ def test_packed_tables_decode_to_the_base_graph(n, rate):
             t * 8 + c for t in range(ngrp) for c in range(6))
 
 
+@pytest.mark.parametrize("n,rate", [(64800, "10/15"), (64800, "2/15"), (64800, "7/15"),
+                                    (16200, "8/15")])
+@pytest.mark.parametrize("f32", [False, True])
+def test_posterior_rows(n, rate, f32):
+    """Each variable group's posterior row (LDS or the global slot) is the
+    same in the check-side and variable-side tables; rows are distinct within
+    their space; FP16 keeps every row in LDS at its group index; FP32 puts
+    the highest-degree groups in the 113 LDS rows."""
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.resident import F32_LDS_ROWS, _host_tables
+    code = get_code(rate, n)
+    q = code.qc()
+    h = _host_tables(code, f32)
+    gv = h["grp_v"].astype(np.int64) & 0xFFFFFFFF
+    row, glob, dv = (gv >> 23) & 255, gv >> 31, gv & 63
+    ec = h["edge_c"].astype(np.int64)
+    t_of_b = (q["be_word"].astype(np.int64) >> 9) & 511
+    assert np.array_equal((ec >> 9) & 511, row[t_of_b])
+    assert np.array_equal((ec >> 28) & 1, glob[t_of_b])
+    for g in (0, 1):
+        r = row[glob == g]
+        assert len(np.unique(r)) == len(r) and (len(r) == 0 or r.max() == len(r) - 1)
+    if not f32:
+        assert (glob == 0).all() and np.array_equal(row, np.arange(q["nvg"]))
+    else:
+        nl = int((glob == 0).sum())
+        assert nl == min(q["nvg"], F32_LDS_ROWS) and h["global_rows"] == q["nvg"] - nl
+        if (glob == 1).any():
+            assert dv[glob == 0].min() >= dv[glob == 1].max()
+
+
 # ---------------------------------------------------------------------------
 # device
 # ---------------------------------------------------------------------------
@@ -167,6 +198,60 @@ def test_kernel_equals_oracle(device, kind, key):
     out = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, alpha)
     torch.cuda.synchronize()
     _check([o.cpu().numpy() for o in out], want, f"{kind} {key}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", F.fixture_names())
+def test_fp32_kernel_equals_reference_fixtures(device, name):
+    """FP32 records and posteriors: the plain decoder, bit-exact with the
+    reference kernel's own golden outputs."""
+    import torch
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr, iters, alpha, want = F.load(name)
+    print(f"[gpu] resident fp32 fixture {name} ({code.rate} {code.n})", flush=True)
+    out = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, alpha,
+                              msg_dtype="float32")
+    torch.cuda.synchronize()
+    F.assert_same([o.cpu().numpy() for o in out], want, name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("key", [f"{r} {n} {s}" for r, n, s in SYNTH])
+def test_fp32_kernel_equals_c_oracle(device, key):
+    import torch
+    from oracle.ldpc import minsum_f32
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr, iters, alpha = _case("synth", key)
+    want = minsum_f32(llr, code, iters, alpha)
+    print(f"[gpu] resident fp32 synth {key}", flush=True)
+    out = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, alpha,
+                              msg_dtype="float32")
+    torch.cuda.synchronize()
+    _check([o.cpu().numpy() for o in out], [np.asarray(w) for w in want], f"fp32 {key}")
+
+
+@pytest.mark.gpu
+def test_fp32_mixed_codes_one_launch(device):
+    """Fixture blocks of every code in one FP32 launch on 5 workgroups."""
+    import torch
+    from atsc3_amd.resident import ResidentPlan
+    blocks = []
+    for name in F.fixture_names():
+        code, llr, iters, alpha, want = F.load(name)
+        if (iters, alpha) != (50, 0.75):
+            continue
+        for b in range(len(llr)):
+            blocks.append((code, llr[b], {k: v[b] for k, v in want.items()}, f"{name}#{b}"))
+    order = np.random.default_rng(9).permutation(len(blocks))
+    blocks = [blocks[i] for i in order]
+    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, grid=5, msg_dtype="float32")
+    llr = np.concatenate([b[1] for b in blocks]).astype(np.float32)
+    print(f"[gpu] resident fp32 mixed launch {len(blocks)} blocks", flush=True)
+    bits, conv, its, bad = (x.cpu().numpy() for x in plan.run(torch.from_numpy(llr).to(device)))
+    for i, (code, _, want, tag) in enumerate(blocks):
+        assert bool(conv[i]) == bool(want["conv"]) and its[i] == want["its"], tag
+        assert bad[i] == want["bad"], tag
+        assert np.array_equal(bits[plan.off[i]:plan.off[i + 1]], want["bits"]), tag
 
 
 @pytest.mark.gpu
