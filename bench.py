@@ -59,6 +59,11 @@ LDM_ESN0 = 18.0
 LDM_SWEEP = (18.0, 19.0, 20.0, 21.0)
 
 
+# the plain FP32 decoder of the bench (bit-exact with the reference kernel):
+# the resident engine's FP32 mode
+PLAIN_ENGINE = "resident"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -126,7 +131,7 @@ def ldm_bench(args, dev, msg_formats=("float32", "float16", "float16_resident"))
     res = {}
     for fmt in msg_formats:
         msg, engine = (("float16", "resident") if fmt == "float16_resident"
-                       else (fmt, "auto"))
+                       else (fmt, PLAIN_ENGINE if fmt == "float32" else "auto"))
         pipe = LdmPipeline(device=dev, msg_dtype=msg, engine=engine)
         if "stream" not in res:
             res["stream"] = LdmStream(pipe, 64, args.seed + 17)
@@ -161,7 +166,7 @@ def ldm_bench(args, dev, msg_formats=("float32", "float16", "float16_resident"))
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         ec, ee = errs(out, ic, ie)
         rt = args.ldm_frames * FRAME_SEC / (ms / 1e3)
-        res[fmt] = {"ms_per_step": round(ms, 3), "realtime_factor": round(rt, 2),
+        res[fmt] = {"engine": engine, "ms_per_step": round(ms, 3), "realtime_factor": round(rt, 2),
                     "blocks_per_s": round((ncore + nenh) / (ms / 1e3), 1),
                     "core_blocks": ncore, "enh_blocks": nenh, "esn0_db": args.ldm_esn0,
                     "core_errors": ec, "enh_errors": ee,
@@ -662,7 +667,7 @@ def main():
         op = None
         for es in SWEEP:
             z, w, idx, n0 = stream.batch(nblk, es)
-            out = run_window(z, w, n0, "float32")
+            out = run_window(z, w, n0, "float32", PLAIN_ENGINE)
             e = errors(idx, out)
             fer[es] = dict(blocks=nblk, errors=e,
                            its_mean=float(out[2].float().mean().item()))
@@ -681,7 +686,8 @@ def main():
             while tot_b < args.fer_blocks and tot_e == 0 and e_msg == 0:
                 z, w, idx, n0 = stream.batch(nblk, op, start=k * nblk)
                 llr_k, _ = dm.demap_dev(z, csi=w, sigma2=n0)
-                tot_e += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, "float32"))
+                tot_e += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, "float32",
+                                                     PLAIN_ENGINE))
                 if args.msg != "float32":
                     e_msg += errors(idx, ldpc.decode_dev(llr_k, code, 50, 0.75, args.msg,
                                                          engine))
@@ -696,7 +702,7 @@ def main():
             op += 1.0
             if op not in fer:
                 z, w, idx, n0 = stream.batch(nblk, op)
-                out = run_window(z, w, n0, "float32")
+                out = run_window(z, w, n0, "float32", PLAIN_ENGINE)
                 fer[op] = dict(blocks=nblk, errors=errors(idx, out),
                                its_mean=float(out[2].float().mean().item()))
     else:
@@ -788,14 +794,15 @@ def main():
     log(f"[bench] CPE over {npool} cells ({len(cpe.symbols)} symbols): {cpe_ms} ms")
     del cpe
     # plain FP32 (the bit-exact decoder) on the same resident window
-    run_window(z, w, n0, "float32")
+    run_window(z, w, n0, "float32", PLAIN_ENGINE)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out32 = run_window(z, w, n0, "float32")
+        out32 = run_window(z, w, n0, "float32", PLAIN_ENGINE)
     torch.cuda.synchronize()
     ms32 = (time.perf_counter() - t0) * 1e3 / args.steps
-    plain = {"blocks_per_s": round(nblk / (ms32 / 1e3), 1), "ms_per_step": round(ms32, 4),
+    plain = {"engine": PLAIN_ENGINE,
+             "blocks_per_s": round(nblk / (ms32 / 1e3), 1), "ms_per_step": round(ms32, 4),
              "errors": errors(idx, out32),
              "mean_its": round(float(out32[2].float().mean().item()), 3)}
     log(f"[bench] plain float32: {ms32:.3f} ms/step = {nblk / (ms32 / 1e3):.0f} blocks/s")
@@ -803,7 +810,7 @@ def main():
     # -- message-storage forms at the same operating point ----------------------
     forms = {}
     if not args.no_forms:
-        for name, fmt, engines in (("float32", "float32", ("phase", "batch")),
+        for name, fmt, engines in (("float32", "float32", ("phase", "batch", "resident")),
                                    ("float16", "float16", ("phase", "batch")),
                                    ("float16_resident", "float16", ("resident",)),
                                    ("int8", "int8", ("phase", "batch"))):
@@ -863,8 +870,11 @@ def main():
         log(f"[bench] CPU baseline: {procs} processes, sample {ns} blocks")
         secs, bits, cconv, cits, cbad, which = cpu_pool.run_pool(
             zh[:ns], wh[:ns], n0, RATE, N, MOD, 50, 0.75, procs, chunk=2)
-        g_bits = out[0][:ns].cpu().numpy()
+        # against the plain FP32 decode of the same blocks (the CPU demaps in
+        # numpy float32, so LLRs differ from the GPU's in the last bits)
+        g_bits = out32[0][:ns].cpu().numpy()
         agree = float((bits == g_bits).all(axis=1).mean())
+        its_agree = float((np.asarray(cits) == out32[2][:ns].cpu().numpy()).mean())
         t_one, *_ = cpu_pool.run_pool(zh[:2], wh[:2], n0, RATE, N, MOD, 50, 0.75, 1,
                                       chunk=1)
         cpu_model = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t") \
@@ -876,7 +886,7 @@ def main():
                           f"{procs} spawn processes over shared memory"),
                "one_core_blocks_per_s": round(2 / t_one, 2),
                "cpu_model": cpu_model,
-               "bits_agree_with_gpu": agree}
+               "bits_agree_with_gpu_fp32": agree, "its_agree_with_gpu_fp32": its_agree}
         log(f"[bench] CPU: {ns / secs:.1f} blocks/s on {procs} cores ({which}); "
             f"1 core {2 / t_one:.2f} blocks/s; GPU/CPU bit agreement {agree:.4f}")
 
