@@ -137,8 +137,9 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     "csr" (generic kernel; FP32 only), "resident" (FP16 or FP32: one
     workgroup per codeword with its posteriors in LDS, atsc3_amd.resident;
     FP32 is bit-exact with the plain decoder)
-    or "auto" (the QC engines by batch size and message format when the
-    code has a QC form, else csr).
+    or "auto" (the QC engines when the code has a QC form, else csr: the
+    fused engine below PHASE_MIN_BLOCKS blocks, above it the resident
+    engine for FP32 and the batched engine for FP16/INT8).
     """
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_dev needs a CUDA (HIP) tensor")
@@ -163,7 +164,9 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
         elif nblk < PHASE_MIN_BLOCKS:
             engine = "qc"
         else:
-            engine = "phase" if fmt == 0 else "batch"
+            # FP32: the resident engine's FP32 mode (bit-exact, ~2.4x the
+            # phased engine on 2566 blocks); FP16/INT8: the batched engine
+            engine = "resident" if fmt == 0 else "batch"
     if engine == "resident":
         if fmt == 2:
             raise ValueError("the resident engine stores FP16 or FP32 records")
