@@ -198,6 +198,9 @@ def default_grid():
 class ResidentPlan:
     """Buffers and launch parameters for decoding one batch composition.
 
+    The workspace holds the device queue word and every workgroup's records:
+    runs of one plan must not overlap (enqueue them on one stream).
+
     block_codes: one LdpcCode per codeword, in queue order; codeword b's
     LLRs occupy ``llr[off[b] : off[b] + n_b]`` of the flat ``llr`` buffer,
     its bits the same range of ``bits``."""
@@ -282,7 +285,7 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
-    shape."""
+    shape on the same stream."""
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_resident_dev needs a CUDA (HIP) tensor")
     llr = llr.contiguous()
@@ -298,7 +301,10 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     bad = torch.empty(nblk, dtype=torch.int32, device=dev)
     if nblk == 0:
         return bits, conv, its, bad
-    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype)
+    # a plan's workspace (queue word, records) serves one stream at a time:
+    # the stream is part of the key
+    key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
+           _stream_ptr(dev))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype)
