@@ -88,7 +88,7 @@ _SIGS = {
     "atsc3_minsum_resident_lds_rows": [c_i32],
     "atsc3_minsum_resident_grid": [],
     "atsc3_minsum_resident_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
-                                  c_i32, c_i32, c_i32, c_f32, c_i32, c_i32, c_vp,
+                                  c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32, c_vp,
                                   ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,

@@ -88,6 +88,7 @@ struct ResArgs {
     const QC_CONST ResTab *codes;
     int nblk, iters;
     float alpha;
+    const float *msg_scale;       // INT8 mode: step per codeword, or NULL (median rule)
     long long stride;             // records per workgroup slot
     long long tstride;            // global posterior floats per workgroup slot
     unsigned *p0, *p0b, *p1;      // p0b: FP32 mode's a2 word
@@ -115,9 +116,9 @@ struct Tot {
     float *glob;
 };
 
-template <bool F32>
+template <int M>
 __device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, unsigned p) {
-    if constexpr (!F32) {
+    if constexpr (M != 0) {
         return __half2float(reinterpret_cast<const __half *>(T.lds)[row * ATSC3_Z + p]);
     } else {
         return glob ? at(T.glob, row * ATSC3_Z + p)
@@ -125,10 +126,10 @@ __device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, u
     }
 }
 
-template <bool F32>
+template <int M>
 __device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, unsigned p,
                                        float v) {
-    if constexpr (!F32) {
+    if constexpr (M != 0) {
         reinterpret_cast<__half *>(T.lds)[row * ATSC3_Z + p] = __ushort_as_half(h_bits_sat(v));
     } else {
         if (glob)
@@ -142,25 +143,38 @@ __device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, un
 struct Rec {
     unsigned *p0, *p0b, *p1;
     unsigned long long *p2;
+    float step, inv;              // INT8 mode: the codeword's step and 1 / step
 };
 
-// Magnitudes (a1, a2) of a record from its magnitude words.
-template <bool F32>
-__device__ __forceinline__ float rec_a1(unsigned w0, unsigned) {
-    return F32 ? __uint_as_float(w0) : h_val((unsigned short)(w0 & 0xffffu));
+// Magnitudes (a1, a2) of a record from its magnitude words: FP32 words
+// (w0, w0b), an FP16 pair, or two INT8 levels times the codeword's step.
+template <int M>
+__device__ __forceinline__ float rec_a1(unsigned w0, unsigned, const Rec &R) {
+    if constexpr (M == 0) return __uint_as_float(w0);
+    else if constexpr (M == 1) return h_val((unsigned short)(w0 & 0xffffu));
+    else return (float)(w0 & 0xffu) * R.step;
 }
-template <bool F32>
-__device__ __forceinline__ float rec_a2(unsigned w0, unsigned w0b) {
-    return F32 ? __uint_as_float(w0b) : h_val((unsigned short)(w0 >> 16));
+template <int M>
+__device__ __forceinline__ float rec_a2(unsigned w0, unsigned w0b, const Rec &R) {
+    if constexpr (M == 0) return __uint_as_float(w0b);
+    else if constexpr (M == 1) return h_val((unsigned short)(w0 >> 16));
+    else return (float)((w0 >> 8) & 0xffu) * R.step;
+}
+// INT8 level of a magnitude: rint(a / step) saturated at 127 (the INT8
+// message format's codec).
+__device__ __forceinline__ unsigned i8_level(float a, float inv) {
+    return (unsigned)fminf(rintf(a * inv), 127.0f);
 }
 // Magnitude words of (a1, a2) = alpha * (m1, m2).
-template <bool F32>
+template <int M>
 __device__ __forceinline__ void rec_put_mag(const Rec &R, unsigned r, float a1, float a2) {
-    if constexpr (F32) {
+    if constexpr (M == 0) {
         at(R.p0, r) = __float_as_uint(a1);
         at(R.p0b, r) = __float_as_uint(a2);
-    } else {
+    } else if constexpr (M == 1) {
         at(R.p0, r) = (unsigned)h_bits_sat(a1) | ((unsigned)h_bits_sat(a2) << 16);
+    } else {
+        at(R.p0, r) = i8_level(a1, R.inv) | (i8_level(a2, R.inv) << 8);
     }
 }
 
@@ -172,15 +186,14 @@ __device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2,
 
 // Message of edge kk of a check from its record.  a1, a2 are non-negative
 // (alpha >= 0), so setting the sign bit negates.
-template <bool F32>
+template <int M>
 __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w0b, unsigned w1,
-                                         unsigned long long w2, unsigned kk) {
+                                         unsigned long long w2, unsigned kk, const Rec &R) {
+    const bool a2 = kk == (w1 >> RES_SIGN_LO);
     float v;
-    if constexpr (F32) {
-        v = __uint_as_float(kk == (w1 >> RES_SIGN_LO) ? w0b : w0);
-    } else {
-        v = h_val((unsigned short)(kk == (w1 >> RES_SIGN_LO) ? (w0 >> 16) : w0));
-    }
+    if constexpr (M == 0) v = __uint_as_float(a2 ? w0b : w0);
+    else if constexpr (M == 1) v = h_val((unsigned short)(a2 ? (w0 >> 16) : w0));
+    else v = (float)((a2 ? (w0 >> 8) : w0) & 0xffu) * R.step;
     return __uint_as_float(__float_as_uint(v) | rec_sign(w1, w2, kk));
 }
 
@@ -188,7 +201,7 @@ __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w0b, unsigned w1,
 // bits of the current posteriors and, unless `update` is false, the new
 // compressed records.  Every load of the G tasks (old records, posteriors)
 // is issued before the first use.
-template <bool F32, int D, int G>
+template <int M, int D, int G>
 __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const Tot &T,
                                                 const Rec &R, int start, unsigned lane,
                                                 bool first, bool update, float alpha) {
@@ -208,7 +221,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         w1[gi] = 0u;
         if (!first) {
             w0[gi] = at(R.p0, rix[gi]);
-            if (F32) w0b[gi] = at(R.p0b, rix[gi]);
+            if (M == 0) w0b[gi] = at(R.p0b, rix[gi]);
             w1[gi] = at(R.p1, rix[gi]);
         }
         const int b0 = (ct->grp_c[g] >> 7) & 2047;
@@ -216,15 +229,15 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_c[b0 + k];
             const unsigned p = wrap360(jj + (ew & 511u));
-            tv[gi][k] = tot_ld<F32>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
+            tv[gi][k] = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
     unsigned syn_total = 0u;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const float a1o = rec_a1<F32>(w0[gi], w0b[gi]);
-        const float a2o = rec_a2<F32>(w0[gi], w0b[gi]);
+        const float a1o = rec_a1<M>(w0[gi], w0b[gi], R);
+        const float a2o = rec_a2<M>(w0[gi], w0b[gi], R);
         const unsigned idxo = w1[gi] >> RES_SIGN_LO;
         bool syn = false, par = false;
         unsigned idx = 0u, neg = 0u;
@@ -246,7 +259,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         }
         if (update && valid[gi]) {
             const unsigned sg = par ? ~neg : neg;
-            rec_put_mag<F32>(R, rix[gi], m1 * alpha, m2 * alpha);
+            rec_put_mag<M>(R, rix[gi], m1 * alpha, m2 * alpha);
             at(R.p1, rix[gi]) = (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
         }
         syn_total += (valid[gi] && syn) ? 1u : 0u;
@@ -256,7 +269,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
 
 // Any check degree up to RES_MAX_DC (records of degree > 25 use the third
 // word): the same steps as check_multi with a runtime edge loop.
-template <bool F32>
+template <int M>
 __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const Rec &R, int g,
                                   int b0, int dc, bool hasmiss, unsigned j, bool valid,
                                   bool first, bool update, float alpha) {
@@ -266,12 +279,12 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
     unsigned long long w2 = 0ull;
     if (!first) {
         w0 = at(R.p0, rix);
-        if (F32) w0b = at(R.p0b, rix);
+        if (M == 0) w0b = at(R.p0b, rix);
         w1 = at(R.p1, rix);
         if (dc > RES_SIGN_LO) w2 = at(R.p2, rix);
     }
-    const float a1o = rec_a1<F32>(w0, w0b);
-    const float a2o = rec_a2<F32>(w0, w0b);
+    const float a1o = rec_a1<M>(w0, w0b, R);
+    const float a2o = rec_a2<M>(w0, w0b, R);
     const unsigned idxo = w1 >> RES_SIGN_LO;
     unsigned syn = 0u, par = 0u, idx = 0u, neg_lo = 0u;
     unsigned long long neg_hi = 0ull;
@@ -280,7 +293,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
         const unsigned ew = (unsigned)ct->edge_c[b0 + k];
         if (hasmiss && ((ew >> 18) & 1023u) == jj + 1u) continue;
         const unsigned p = wrap360(jj + (ew & 511u));
-        const float tv = tot_ld<F32>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
+        const float tv = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
         const float ao = (unsigned)k == idxo ? a2o : a1o;
         const float mo = __uint_as_float(__float_as_uint(ao) | rec_sign(w1, w2, (unsigned)k));
         const float x = tv - mo;
@@ -302,7 +315,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
     }
     if (update && valid) {
         const unsigned sg_lo = par ? ~neg_lo : neg_lo;
-        rec_put_mag<F32>(R, rix, m1 * alpha, m2 * alpha);
+        rec_put_mag<M>(R, rix, m1 * alpha, m2 * alpha);
         at(R.p1, rix) = (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
         if (dc > RES_SIGN_LO) at(R.p2, rix) = par ? ~neg_hi : neg_hi;
     }
@@ -320,7 +333,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
 // (Type A codes, n0 > 0 entries); else one region.  With one region of two
 // entries the order does not matter: (0 + a) + b == (0 + b) + a in IEEE
 // arithmetic, signed zeros included.
-template <bool F32, int D, int G, bool TA>
+template <int M, int D, int G, bool TA>
 __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &T,
                                           const float *lam, const Rec &R, int start,
                                           unsigned lane) {
@@ -350,7 +363,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
             wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
             const unsigned r = g * ATSC3_Z + pp + u - (wr[gi][k] ? (unsigned)ATSC3_Z : 0u);
             w0[gi][k] = at(R.p0, r);
-            if (F32) w0b[gi][k] = at(R.p0b, r);
+            if (M == 0) w0b[gi][k] = at(R.p0b, r);
             w1[gi][k] = at(R.p1, r);
         }
     }
@@ -361,12 +374,14 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned kk = ((unsigned)ct->edge_v[2 * (e0[gi] + k)] >> 17) & 127u;
+            const bool a2 = kk == (w1[gi][k] >> RES_SIGN_LO);
             float v;
-            if constexpr (F32) {
-                v = __uint_as_float(kk == (w1[gi][k] >> RES_SIGN_LO) ? w0b[gi][k] : w0[gi][k]);
+            if constexpr (M == 0) {
+                v = __uint_as_float(a2 ? w0b[gi][k] : w0[gi][k]);
+            } else if constexpr (M == 1) {
+                v = h_val((unsigned short)(a2 ? (w0[gi][k] >> 16) : w0[gi][k]));
             } else {
-                v = h_val((unsigned short)(kk == (w1[gi][k] >> RES_SIGN_LO) ? (w0[gi][k] >> 16)
-                                                                          : w0[gi][k]));
+                v = (float)((a2 ? (w0[gi][k] >> 8) : w0[gi][k]) & 0xffu) * R.step;
             }
             m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - kk)) & 0x80000000u));
         }
@@ -389,13 +404,13 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
             for (int k = 0; k < D; ++k)
                 if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         }
-        if (valid[gi]) tot_st<F32>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
+        if (valid[gi]) tot_st<M>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
     }
 }
 
 // Any degree: one record load per entry, visited in the summation order
 // directly (suffix then prefix of each region).
-template <bool F32>
+template <int M>
 __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float *lam,
                             const Rec &R, int t, unsigned p, bool valid) {
     const unsigned vw = (unsigned)ct->grp_v[t];
@@ -418,33 +433,34 @@ __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
             const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
             const unsigned long long w2 = kk >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
-            const float mv = rec_msg<F32>(at(R.p0, rix), F32 ? at(R.p0b, rix) : 0u,
-                                          at(R.p1, rix), w2, kk);
+            const float mv = rec_msg<M>(at(R.p0, rix), M == 0 ? at(R.p0b, rix) : 0u,
+                                        at(R.p1, rix), w2, kk, R);
             if (vm != pp + 1u) s += mv;
         }
     }
-    if (valid) tot_st<F32>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
+    if (valid) tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
 }
 
 __global__ void resident_init(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
-template <bool F32>
+template <int M>
 __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
-    __shared__ __align__(16) unsigned char lds[F32 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
+    __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
     __shared__ int s_blk;
     __shared__ int s_unsat[2];
+    __shared__ int s_hist[M == 2 ? 256 : 1];
     const unsigned tid = threadIdx.x, lane = tid & 63u;
     const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
     Rec R;
     R.p0 = a.p0 + (size_t)blockIdx.x * a.stride;
-    R.p0b = F32 ? a.p0b + (size_t)blockIdx.x * a.stride : nullptr;
+    R.p0b = M == 0 ? a.p0b + (size_t)blockIdx.x * a.stride : nullptr;
     R.p1 = a.p1 + (size_t)blockIdx.x * a.stride;
     R.p2 = a.p2 + (size_t)blockIdx.x * a.stride;
     Tot T;
     T.lds = lds;
-    T.glob = F32 ? a.totg + (size_t)blockIdx.x * a.tstride : nullptr;
+    T.glob = M == 0 ? a.totg + (size_t)blockIdx.x * a.tstride : nullptr;
     for (;;) {
         if (tid == 0) {
             s_blk = atomicAdd(a.queue, 1);
@@ -458,13 +474,18 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         const int nvg = ct->nvg;
         const long long off = a.blk_off[blk];
         const float *lam = a.llr + off;
+        if constexpr (M == 2) {
+            // the codeword's INT8 step: the caller's, or 6 * median|llr| / 127
+            R.step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, ct->n, s_hist);
+            R.inv = 1.0f / R.step;
+        }
         // tot = lam, row by row (each 64-position task of a group on one wave)
         for (int task = wave; task < nvg * 6; task += RES_WAVES) {
             const int t = task / 6;
             const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
             const unsigned vw = (unsigned)ct->grp_v[t];
             if (p < (unsigned)ATSC3_Z)
-                tot_st<F32>(T, (vw >> 23) & 255u, vw >> 31, p, lam[(unsigned)t * ATSC3_Z + p]);
+                tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, lam[(unsigned)t * ATSC3_Z + p]);
         }
         __syncthreads();
         for (int it = 0;; ++it) {
@@ -479,7 +500,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     const int g = task & 255;
                     const unsigned j = (unsigned)(task >> 8) * 64u + lane;
                     const int gw = ct->grp_c[g];
-                    unsat += check_generic<F32>(ct, T, R, g, (gw >> 7) & 2047, gw & 127,
+                    unsat += check_generic<M>(ct, T, R, g, (gw >> 7) & 2047, gw & 127,
                                                 (gw >> 18) & 1, j, j < (unsigned)ATSC3_Z, first,
                                                 update, a.alpha);
                     continue;
@@ -487,11 +508,11 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 switch (G * 128 + D) {
 #define RES_C1(DD)                                                                           \
     case 128 + DD:                                                                           \
-        unsat += check_multi<F32, DD, 1>(ct, T, R, start, lane, first, update, a.alpha);     \
+        unsat += check_multi<M, DD, 1>(ct, T, R, start, lane, first, update, a.alpha);     \
         break;
 #define RES_C2(DD)                                                                           \
     case 256 + DD:                                                                           \
-        unsat += check_multi<F32, DD, 2>(ct, T, R, start, lane, first, update, a.alpha);     \
+        unsat += check_multi<M, DD, 2>(ct, T, R, start, lane, first, update, a.alpha);     \
         break;
                     QC_CASES16(RES_C1)
                     QC_CASES16(RES_C2)
@@ -511,7 +532,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     const unsigned vw = (unsigned)ct->grp_v[t];
                     if (p < (unsigned)ATSC3_Z)
                         bout[(unsigned)t * ATSC3_Z + p] =
-                            (uint8_t)(tot_ld<F32>(T, (vw >> 23) & 255u, vw >> 31, p) < 0.0f);
+                            (uint8_t)(tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p) < 0.0f);
                 }
                 if (tid == 0) {
                     a.conv[blk] = (uint8_t)(total == 0);
@@ -526,31 +547,31 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 if ((sw >> 22) & 1) {
                     const int task = ct->v_tasks[start];
                     const unsigned p = (unsigned)(task >> 8) * 64u + lane;
-                    var_generic<F32>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
+                    var_generic<M>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
                     continue;
                 }
                 const bool ta = (sw >> 23) & 1;
                 switch ((ta ? 1024 : 0) + G * 128 + D) {
 #define RES_V1(DD)                                                      \
     case 128 + DD:                                                      \
-        var_multi<F32, DD, 1, false>(ct, T, lam, R, start, lane);       \
+        var_multi<M, DD, 1, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 128 + DD:                                               \
-        var_multi<F32, DD, 1, true>(ct, T, lam, R, start, lane);        \
+        var_multi<M, DD, 1, true>(ct, T, lam, R, start, lane);        \
         break;
 #define RES_V2(DD)                                                      \
     case 256 + DD:                                                      \
-        var_multi<F32, DD, 2, false>(ct, T, lam, R, start, lane);       \
+        var_multi<M, DD, 2, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 256 + DD:                                               \
-        var_multi<F32, DD, 2, true>(ct, T, lam, R, start, lane);        \
+        var_multi<M, DD, 2, true>(ct, T, lam, R, start, lane);        \
         break;
 #define RES_V4(DD)                                                      \
     case 512 + DD:                                                      \
-        var_multi<F32, DD, 4, false>(ct, T, lam, R, start, lane);       \
+        var_multi<M, DD, 4, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
     case 1024 + 512 + DD:                                               \
-        var_multi<F32, DD, 4, true>(ct, T, lam, R, start, lane);        \
+        var_multi<M, DD, 4, true>(ct, T, lam, R, start, lane);        \
         break;
                     QC_CASES16(RES_V1)
                     QC_CASES8(RES_V2)
@@ -588,14 +609,14 @@ struct WsLayout {
 
 extern "C" int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,
                                                   int32_t max_global_rows, int32_t msg_fmt) {
-    if (grid < 1 || max_checks < 1 || max_global_rows < 0 || (msg_fmt != 0 && msg_fmt != 1))
+    if (grid < 1 || max_checks < 1 || max_global_rows < 0 || msg_fmt < 0 || msg_fmt > 2)
         return ATSC3_E_ARG;
     return atsc3::WsLayout(grid, max_checks, max_global_rows, msg_fmt == 0).bytes;
 }
 
 extern "C" int32_t atsc3_minsum_resident_lds_rows(int32_t msg_fmt) {
     if (msg_fmt == 0) return RES_F32_LDS_ROWS;
-    if (msg_fmt == 1) return RES_NMAX / ATSC3_Z;
+    if (msg_fmt == 1 || msg_fmt == 2) return RES_NMAX / ATSC3_Z;
     return ATSC3_E_ARG;
 }
 
@@ -611,10 +632,11 @@ extern "C" int32_t atsc3_minsum_resident_dev(
     const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
     const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
     int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
-    int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
+    const float *msg_scale, int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits,
+    uint8_t *conv, int32_t *its,
     int32_t *bad, void *stream) {
     if (nblk < 0 || iters < 0 || iters >= (1 << 24) || !(alpha >= 0.0f)) return ATSC3_E_ARG;
-    if (msg_fmt != 0 && msg_fmt != 1) return ATSC3_E_ARG;
+    if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!llr || !blk_off || !blk_code || !codes || !ws || !bits || !conv || !its || !bad)
         return ATSC3_E_ARG;
@@ -649,9 +671,12 @@ extern "C" int32_t atsc3_minsum_resident_dev(
     hipStream_t st = atsc3::as_stream(stream);
     hipLaunchKernelGGL(atsc3::resident_init, dim3(1), dim3(64), 0, st, a.queue);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    a.msg_scale = msg_fmt == 2 ? msg_scale : nullptr;
     if (f32)
-        hipLaunchKernelGGL(atsc3::resident_kernel<true>, dim3(grid), dim3(RES_WG), 0, st, a);
+        hipLaunchKernelGGL(atsc3::resident_kernel<0>, dim3(grid), dim3(RES_WG), 0, st, a);
+    else if (msg_fmt == 1)
+        hipLaunchKernelGGL(atsc3::resident_kernel<1>, dim3(grid), dim3(RES_WG), 0, st, a);
     else
-        hipLaunchKernelGGL(atsc3::resident_kernel<false>, dim3(grid), dim3(RES_WG), 0, st, a);
+        hipLaunchKernelGGL(atsc3::resident_kernel<2>, dim3(grid), dim3(RES_WG), 0, st, a);
     return atsc3::launch_status();
 }

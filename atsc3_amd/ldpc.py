@@ -134,7 +134,7 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     engine: "qc" (fused 360-lifted kernel, one workgroup per codeword),
     "phase" (the same decode as per-iteration phase kernels, for large
     batches), "batch" (the persistent batched engine, atsc3_amd.batch),
-    "csr" (generic kernel; FP32 only), "resident" (FP16 or FP32: one
+    "csr" (generic kernel; FP32 only), "resident" (any format: one
     workgroup per codeword with its posteriors in LDS, atsc3_amd.resident;
     FP32 is bit-exact with the plain decoder)
     or "auto" (the QC engines when the code has a QC form, else csr: the
@@ -168,13 +168,12 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
             # phased engine on 2566 blocks); FP16/INT8: the batched engine
             engine = "resident" if fmt == 0 else "batch"
     if engine == "resident":
-        if fmt == 2:
-            raise ValueError("the resident engine stores FP16 or FP32 records")
-        if msg_scale is not None:
+        if msg_scale is not None and fmt != 2:
             raise ValueError("msg_scale applies to INT8 messages only")
         from .resident import decode_resident_dev
         return decode_resident_dev(llr, code, iters, alpha,
-                                   msg_dtype="float32" if fmt == 0 else "float16")
+                                   msg_dtype={0: "float32", 1: "float16", 2: "int8"}[fmt],
+                                   msg_scale=msg_scale)
     if engine == "batch":
         if tabs.qc is None:
             raise ValueError("this code has no QC form with variable degree <= 32")

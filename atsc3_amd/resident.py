@@ -206,13 +206,16 @@ class ResidentPlan:
     its bits the same range of ``bits``."""
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
-                 msg_dtype="float16"):
-        """msg_dtype "float16": the FP16 form; "float32": the plain decoder,
+                 msg_dtype="float16", msg_scale=None):
+        """msg_dtype "float16": the FP16 form; "int8": INT8 record magnitudes
+        with a per-codeword step (msg_scale, one per codeword, or None: 6 *
+        median|llr| / 127 on the device); "float32": the plain decoder,
         bit-exact with the reference kernel."""
-        if msg_dtype not in ("float16", "float32"):
-            raise ValueError("the resident engine stores float16 or float32 records")
+        fmts = {"float32": 0, "float16": 1, "int8": 2}
+        if msg_dtype not in fmts:
+            raise ValueError("the resident engine stores float32, float16 or int8 records")
         self.msg_dtype = msg_dtype
-        self.fmt = 1 if msg_dtype == "float16" else 0
+        self.fmt = fmts[msg_dtype]
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -250,6 +253,14 @@ class ResidentPlan:
             raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
         self.ws_bytes = int(nbytes)
+        self.msg_scale = None
+        if msg_scale is not None:
+            if self.fmt != 2:
+                raise ValueError("msg_scale applies to INT8 records only")
+            self.msg_scale = torch.as_tensor(msg_scale, dtype=torch.float32,
+                                             device=dev).contiguous()
+            if self.msg_scale.numel() != self.nblk:
+                raise ValueError("msg_scale needs one value per codeword")
 
     def run(self, llr=None, out=None):
         """Enqueue the decode of ``llr`` (flat, or None: self.llr as filled)
@@ -266,7 +277,9 @@ class ResidentPlan:
         _lib.call("atsc3_minsum_resident_dev", src.data_ptr(), self.nblk,
                   self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
                   len(b), b.max_checks, b.max_global_rows, b.max_n, b.max_dc, b.max_dv,
-                  self.iters, self.alpha, self.fmt, self.grid, self.ws.data_ptr(),
+                  self.iters, self.alpha, self.fmt,
+                  None if self.msg_scale is None else self.msg_scale.data_ptr(),
+                  self.grid, self.ws.data_ptr(),
                   self.ws_bytes, bits.data_ptr(),
                   conv.data_ptr(), its.data_ptr(), bad.data_ptr(), _stream_ptr(self.device))
         return bits, conv, its, bad
@@ -279,9 +292,11 @@ _PLANS = {}            # a few recent plans (LRU): callers alternate codes, e.g.
 _PLANS_MAX = 8
 
 
-def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16"):
+def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16",
+                        msg_scale=None):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
-    msg_dtype "float32" is the plain decoder (bit-exact with the reference).
+    msg_dtype "float32" is the plain decoder (bit-exact with the reference);
+    "int8" takes msg_scale (one step per codeword) or derives it.
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -305,6 +320,10 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     # the stream is part of the key
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
            _stream_ptr(dev))
+    if msg_scale is not None:
+        # caller-given steps: a plan of its own (not cached)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale)
+        return plan.run(llr.reshape(-1), (bits, conv, its, bad))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype)

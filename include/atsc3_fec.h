@@ -191,6 +191,12 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
  *              grp_v word says so in LDS (at most
  *              atsc3_minsum_resident_lds_rows(0) rows), the other rows in the
  *              workgroup's global slot; FP32 record magnitudes.
+ *   msg_fmt 2  INT8: as FP16 with the record magnitudes stored as INT8
+ *              levels of the codeword's step (rint(a / step) saturated at
+ *              127, the INT8 message format's codec); msg_scale[b] = the
+ *              step of codeword b, or NULL: 6 * median|llr| / 127 computed on
+ *              the device (the median to 8 mantissa bits), as the other
+ *              engines' INT8 format.
  * Codewords of mixed codes (n <= 64800, check degree <= 89, variable degree
  * <= 32: every A/322 code) are taken from a device queue by `grid`
  * workgroups (atsc3_minsum_resident_grid() = one per compute unit; any
@@ -238,7 +244,8 @@ int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
                                   int32_t max_checks, int32_t max_global_rows,
                                   int32_t max_n, int32_t max_dc, int32_t max_dv,
                                   int32_t iters, float alpha, int32_t msg_fmt,
-                                  int32_t grid, void *ws, int64_t ws_bytes,
+                                  const float *msg_scale, int32_t grid, void *ws,
+                                  int64_t ws_bytes,
                                   uint8_t *bits, uint8_t *conv, int32_t *its,
                                   int32_t *bad, void *stream);
 

@@ -47,8 +47,29 @@ def _tables(code):
                 k_of_b=np.arange(nbe) - cg_ptr[g_of_b])
 
 
-def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant=None):
+def int8_step(llr_block):
+    """The INT8 step the device derives for one codeword when the caller gives
+    none: 6 * median|llr| / 127 in float32, with the lower median (the
+    ((n-1)/2)-th smallest |llr|) kept to its exponent and top 8 mantissa bits
+    and placed in the middle of that bucket.  (The device finds the bucket
+    with two integer histograms; for non-negative floats the bit pattern
+    orders as the value, so it is the bucket of the sorted rank-th value.)"""
+    a = np.sort(np.abs(np.asarray(llr_block, np.float32)))
+    u = np.array([a[(len(a) - 1) // 2]], np.float32).view(np.uint32)[0]
+    med = np.array([(u & np.uint32(0xFFFF8000)) | np.uint32(1 << 14)], np.uint32).view(
+        np.float32)[0]
+    return np.float32(max(np.float32(np.float32(6.0) * med) / np.float32(127.0),
+                          np.float32(1e-30)))
+
+
+def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant=None,
+                    steps=None):
     """(nblk, n) LLRs -> (bits u8 (nblk, n), conv bool, its i32, bad i32).
+
+    storage "int8": posteriors as for "float16"; each message magnitude is an
+    INT8 level of the codeword's step, min(rint(a / step), 127) * step (the
+    INT8 message format's codec), with ``steps`` one per codeword or None
+    (``int8_step`` of each codeword's LLRs).
 
     ``_plant="list_order"`` is a negative control for the tests: each
     variable's messages are summed in table order, ignoring the wrap
@@ -59,15 +80,30 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
     ncg, nvg = T["ncg"], T["nvg"]
     llr = llr.reshape(B, nvg, Z)
     f32 = np.float32
-    if storage == "float16":
+    step = inv = None
+    if storage == "int8":
+        step = (np.array([int8_step(row.reshape(-1)) for row in llr], f32) if steps is None
+                else np.asarray(steps, f32).reshape(-1))
+        inv = (f32(1.0) / step).astype(f32)
+        storage_tot = "float16"
+    else:
+        storage_tot = storage
+    if storage_tot == "float16":
         def rnd(x):
             return np.clip(x, -65504.0, 65504.0).astype(np.float16).astype(f32)
-    elif storage == "float32":
+    elif storage_tot == "float32":
         def rnd(x):
             return np.array(x, f32)             # a copy: tot must not alias lam
     else:
         raise ValueError(storage)
     alpha = f32(alpha)
+    if storage == "int8":
+        def rnd_msg(a):
+            # a: (B, Z) magnitudes; level per codeword's step
+            q = np.minimum(np.rint((a * inv[:, None]).astype(f32)), f32(127.0))
+            return (q * step[:, None]).astype(f32)
+    else:
+        rnd_msg = rnd
     cg = T["cg_ptr"]
     dcmax = int(np.diff(cg).max())
     tot = rnd(llr)
@@ -105,8 +141,8 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
                 m1 = mag.min(axis=1)
                 m2 = np.partition(mag, 1, axis=1)[:, 1] if dc > 1 else np.full_like(m1, np.inf)
                 unsat += syn.sum(axis=1)
-                n1[:, g] = rnd(m1 * alpha)
-                n2[:, g] = rnd(m2 * alpha)
+                n1[:, g] = rnd_msg(m1 * alpha)
+                n2[:, g] = rnd_msg(m2 * alpha)
                 nidx[:, g] = mag.argmin(axis=1)
                 nsg[:, g, :dc] = par[:, None, :] ^ neg
             fin = ~done & ((unsat == 0) | (it == iters))

@@ -68,6 +68,20 @@ def test_oracle_detects_a_planted_order_change():
     assert missed == 2
 
 
+def test_int8_step_closed_forms():
+    """6 * median|llr| / 127 with the lower median kept to its exponent and
+    top 8 mantissa bits, placed in the middle of that bucket."""
+    from oracle.resident_ref import int8_step
+    f = np.float32
+    mid = lambda v: np.array([np.array([v], f).view(np.uint32)[0] & 0xFFFF8000 | (1 << 14)],
+                             np.uint32).view(f)[0]
+    assert int8_step([-3.0, 1.0, 2.0]) == f(f(6) * mid(2.0)) / f(127)
+    assert int8_step([4.0, -1.0, 3.0, 2.0]) == f(f(6) * mid(2.0)) / f(127)     # lower median
+    assert mid(2.0) == f(2.0 + 2.0 ** -8)          # bucket [2, 2 + 2^-7): its middle
+    assert int8_step([1.234567, -1.234567]) == f(f(6) * mid(1.234567)) / f(127)
+    assert int8_step(np.zeros(7)) == f(1e-30)
+
+
 @pytest.mark.parametrize("n,rate", [(64800, "10/15"), (64800, "2/15"), (64800, "7/15"),
                                     (64800, "13/15"), (16200, "8/15"), (16200, "4/15")])
 def test_packed_tables_decode_to_the_base_graph(n, rate):
@@ -252,6 +266,53 @@ def test_fp32_mixed_codes_one_launch(device):
         assert bool(conv[i]) == bool(want["conv"]) and its[i] == want["its"], tag
         assert bad[i] == want["bad"], tag
         assert np.array_equal(bits[plan.off[i]:plan.off[i + 1]], want["bits"]), tag
+
+
+_ORACLE8 = {}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,key", [("fixture", "g16k_r4"), ("fixture", "s16k_r6_ties"),
+                                      ("fixture", "s16k_r3_infnan"), ("fixture", "g64k_r10_nuc20"),
+                                      ("synth", "13/15 64800 5.5"), ("synth", "7/15 64800 0.8"),
+                                      ("synth", "6/15 64800 1.5")])
+def test_int8_kernel_equals_oracle(device, kind, key):
+    """INT8 record magnitudes with the oracle's per-codeword steps given
+    explicitly: bit-exact."""
+    import torch
+    from oracle.resident_ref import int8_step
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr, iters, alpha = _case(kind, key)
+    steps = np.array([int8_step(r) for r in llr], np.float32)
+    want = resident_minsum(llr, code, iters, alpha, "int8", steps=steps)
+    print(f"[gpu] resident int8 {kind} {key}", flush=True)
+    out = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, alpha,
+                              msg_dtype="int8", msg_scale=steps)
+    torch.cuda.synchronize()
+    _check([o.cpu().numpy() for o in out], want, f"int8 {kind} {key}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["resident", "qc"])
+def test_device_int8_step_is_the_oracle_rule(device, engine):
+    """With no step given the engines derive 6 * median|llr| / 127 on the
+    device; decoding with the oracle's int8_step given explicitly yields the
+    same outputs (the resident engine and the fused engine, whose INT8 step
+    code every QC engine shares)."""
+    import torch
+    from oracle.resident_ref import int8_step
+    from atsc3_amd import ldpc
+    code, llr = _synth_llr("10/15", 64800, 3.3, nblk=6, seed=31)
+    llr = llr * np.float32(np.linspace(0.3, 3.0, 6))[:, None]      # different medians
+    steps = np.array([int8_step(r) for r in llr], np.float32)
+    t = torch.from_numpy(llr).to(device)
+    print(f"[gpu] device int8 step ({engine})", flush=True)
+    a = ldpc.decode_dev(t, code, 50, 0.75, "int8", engine)
+    b = ldpc.decode_dev(t, code, 50, 0.75, "int8", engine,
+                        msg_scale=torch.from_numpy(steps).to(device))
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
 
 
 @pytest.mark.gpu
