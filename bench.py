@@ -813,7 +813,8 @@ def main():
         for name, fmt, engines in (("float32", "float32", ("phase", "batch", "resident")),
                                    ("float16", "float16", ("phase", "batch")),
                                    ("float16_resident", "float16", ("resident",)),
-                                   ("int8", "int8", ("phase", "batch"))):
+                                   ("int8", "int8", ("phase", "batch")),
+                                   ("int8_resident", "int8", ("resident",))):
             eng_ms = {}
             for eng in engines:
                 run_window(z, w, n0, fmt, eng)
@@ -850,7 +851,7 @@ def main():
                 f"{', '.join(f'{k} {v:.2f} ms' for k, v in eng_ms.items())}), "
                 f"{errs}/{blocks} block errors; curve {[(c[0], c[1]) for c in curve]}")
         if "float32" in forms:
-            for name in ("float16", "float16_resident", "int8"):
+            for name in ("float16", "float16_resident", "int8", "int8_resident"):
                 forms[name]["gap_db_at_fer_1e-2"] = fer_gap_db(forms["float32"]["fer_curve"],
                                                                forms[name]["fer_curve"], 1e-2)
 
