@@ -12,7 +12,11 @@ Two knobs set how the persistent kernel maps onto the device:
 builds the pipeline for a (slots, grid) pair; each candidate is captured in a
 HIP graph and replayed), keeps the fastest, and caches the choice in a JSON
 file keyed by device name and workload signature (``ATSC3_TUNE_CACHE``, else
-~/.cache/atsc3_amd/batch_tune.json).
+~/.cache/atsc3_amd/batch_tune.json).  With ``resident=True`` the resident
+engine (one workgroup per codeword, one per compute unit) is a candidate
+too, built as slots = "resident": it wins when a launch carries many
+codewords, the batched engine when a few slow codewords would leave most
+compute units idle.
 """
 from __future__ import annotations
 
@@ -71,25 +75,29 @@ def time_graph(g, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True, log=None):
+def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True, log=None,
+         resident=False):
     """build(slots, grid) -> object with .capture() and .plan; prepare(obj)
-    loads the tuning input.  Returns (slots, grid, table of ms per candidate)."""
+    loads the tuning input.  Returns (slots, grid, table of ms per candidate);
+    slots == "resident" (grid None: one workgroup per compute unit) when the
+    resident engine won."""
     dev = torch.cuda.get_device_name()
-    key = f"{dev}|{signature}|{msg_dtype}"
+    key = f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}"
     cache = _load() if use_cache else {}
     if key in cache:
         c = cache[key]
         return c["slots"], c["grid"], c.get("table", {})
     table = {}
     best = None
-    for slots, grid in candidates(nblk, msg_dtype):
+    cands = candidates(nblk, msg_dtype) + ([("resident", None)] if resident else [])
+    for slots, grid in cands:
         obj = build(slots, grid)
         if prepare is not None:
             prepare(obj)
         g = obj.capture()
         ms = time_graph(g, reps)
         obj.plan.check()
-        table[f"{slots}x{grid}"] = round(ms, 4)
+        table["resident" if slots == "resident" else f"{slots}x{grid}"] = round(ms, 4)
         if log:
             log(f"[tune] {signature} {msg_dtype}: slots {slots} grid {grid}: {ms:.3f} ms")
         if best is None or ms < best[0]:

@@ -66,7 +66,10 @@ class MixedPipeline:
     """Demap + batched decode of frames with a fixed block layout."""
 
     def __init__(self, specs, frames=1, device="cuda", iters=50, alpha=0.75,
-                 msg_dtype="float32", slots=None, grid=None):
+                 msg_dtype="float32", slots=None, grid=None, engine="batch"):
+        """engine "batch": the persistent batched engine (slots, grid);
+        "resident": one workgroup per codeword (grid = workgroups, slots
+        unused)."""
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -79,7 +82,12 @@ class MixedPipeline:
             else:
                 self.groups.append((s, b, 1))
         codes = [get_code(r, n) for n, r, m in self.specs]
-        self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid)
+        if engine == "resident":
+            from .resident import ResidentPlan
+            self.plan = ResidentPlan(codes, self.device, iters, alpha, grid, msg_dtype)
+        else:
+            self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid)
+        self.engine = engine
         self.demappers = {}
         self.cells, self.sigma2 = [], []
         for (n, r, m), b0, cnt in self.groups:

@@ -284,6 +284,43 @@ class ResidentPlan:
                   conv.data_ptr(), its.data_ptr(), bad.data_ptr(), _stream_ptr(self.device))
         return bits, conv, its, bad
 
+    def check(self):
+        """Interface of BatchPlan.check: the resident kernel has no grid
+        barrier and always completes every codeword, so nothing can abort."""
+        return None
+
+    def rounds(self):
+        """Interface of BatchPlan.rounds: no grid-wide rounds here."""
+        return 0
+
+    def llr_view(self, b):
+        return self.llr[int(self.off[b]):int(self.off[b + 1])]
+
+    def block_results(self, b):
+        """(bits u8 [n_b], conv, its, bad) of codeword b as device tensors."""
+        lo, hi = int(self.off[b]), int(self.off[b + 1])
+        return self.bits[lo:hi], self.conv[b], self.its[b], self.bad[b]
+
+    def capture(self, pre=None):
+        """A torch.cuda.CUDAGraph replaying ``pre()`` (e.g. the demaps that
+        fill ``self.llr``) followed by the decode of ``self.llr`` into the
+        plan's buffers (the queue reset is a kernel, not a memset)."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            if pre is not None:
+                pre()
+            self.run()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g, stream=s):
+            if pre is not None:
+                pre()
+            self.run()
+        torch.cuda.synchronize(self.device)
+        return g
+
     def block_bits(self, b):
         return self.bits[int(self.off[b]):int(self.off[b + 1])]
 

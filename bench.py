@@ -201,7 +201,9 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     nblk = len(specs) * F
 
     def build(fmt):
-        return lambda slots, grid: MixedPipeline(specs, F, dev, 50, 0.75, fmt, slots, grid)
+        return lambda slots, grid: MixedPipeline(
+            specs, F, dev, 50, 0.75, fmt, None if slots == "resident" else slots, grid,
+            engine="resident" if slots == "resident" else "batch")
 
     def prepare(p):
         src = MixedSource(p, 8, args.seed + 5)
@@ -211,7 +213,7 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     ref_bits = None
     for fmt in formats:
         slots, grid, table = autotune.tune(f"mixed{len(specs)}x{F}", nblk, fmt, build(fmt),
-                                           prepare, use_cache=False, log=log)
+                                           prepare, use_cache=False, log=log, resident=True)
         pipe = build(fmt)(slots, grid)
         src = MixedSource(pipe, 8, args.seed + 5)
         errs, blocks, its, disagree = 0, 0, [], 0
@@ -244,7 +246,7 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
         ms_eager = (time.perf_counter() - t0) * 1e3 / max(args.steps, 5)
         kbits = sum(__import__("atsc3_amd.codes", fromlist=["get_code"]).get_code(r, n).K
                     for n, r, m in pipe.specs)
-        res[fmt] = {"slots": slots, "grid": grid, "tune_ms": table,
+        res[fmt] = {"engine": pipe.engine, "slots": slots, "grid": grid, "tune_ms": table,
                     "ms_per_launch_graph": round(ms_graph, 4),
                     "ms_per_launch_eager": round(ms_eager, 4),
                     "blocks_per_s": round(nblk / (ms_graph / 1e3), 1),
@@ -253,7 +255,7 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
                     "its_min": int(its.min().item()), "its_max": int(its.max().item()),
                     "its_mean": round(float(its.mean().item()), 2),
                     "bit_disagreement_vs_fp32": disagree, "rounds": pipe.plan.rounds()}
-        log(f"[mixed] {fmt}: slots {slots} grid {grid}: graph {ms_graph:.3f} ms "
+        log(f"[mixed] {fmt}: {pipe.engine} slots {slots} grid {grid}: graph {ms_graph:.3f} ms "
             f"(eager {ms_eager:.3f}) per {nblk} blocks = {nblk / (ms_graph / 1e3):.0f} blocks/s; "
             f"{errs}/{blocks} block errors, its {its.min().item():.0f}..{its.max().item():.0f}")
         del pipe, g

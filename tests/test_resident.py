@@ -400,3 +400,28 @@ def test_graph_replay_equals_eager(device):
         torch.cuda.synchronize()
         for a, b in zip((plan.bits, plan.conv, plan.its, plan.bad), eager):
             assert torch.equal(a, b), f"replay {r}"
+
+
+@pytest.mark.gpu
+def test_mixed_pipeline_engines_agree_fp32(device):
+    """Two uneven frames (every code and constellation of the mixed
+    workload) demapped and decoded in one captured graph: the resident
+    engine's FP32 mode equals the batched engine (both bit-exact)."""
+    import torch
+    from atsc3_amd.mixed import MixedPipeline, MixedSource, block_esn0, frame_mix
+    specs = frame_mix(74, 3)
+    outs = []
+    for eng in ("batch", "resident"):
+        print(f"[gpu] mixed pipeline fp32 {eng}", flush=True)
+        p = MixedPipeline(specs, 2, device, 50, 0.75, "float32", engine=eng)
+        src = MixedSource(p, 8, 21)
+        cells, s2, _ = src.frame(block_esn0(p.specs, 23))
+        p.load(cells, s2)
+        g = p.capture()
+        g.replay()
+        torch.cuda.synchronize()
+        p.plan.check()
+        outs.append([x.clone() for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad)])
+        del g, p
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
