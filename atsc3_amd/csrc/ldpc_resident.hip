@@ -236,8 +236,11 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
     unsigned syn_total = 0u;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const float a1o = rec_a1<M>(w0[gi], w0b[gi], R);
-        const float a2o = rec_a2<M>(w0[gi], w0b[gi], R);
+        float a1o = rec_a1<M>(w0[gi], w0b[gi], R);
+        float a2o = rec_a2<M>(w0[gi], w0b[gi], R);
+        // converted once per check: keeps the compiler from converting the
+        // selected FP16/INT8 magnitude again for every edge (no instruction)
+        asm volatile("" : "+v"(a1o), "+v"(a2o));
         const unsigned idxo = w1[gi] >> RES_SIGN_LO;
         bool syn = false, par = false;
         unsigned idx = 0u, neg = 0u;
@@ -251,7 +254,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
             syn ^= tv[gi][k] < 0.0f;
             const bool ng = x < 0.0f;
             par ^= ng;
-            neg |= ng ? 1u << k : 0u;
+            neg |= (unsigned)ng << k;
             const float mag = clamp_mag(x);
             idx = mag < m1 ? (unsigned)k : idx;
             m2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
