@@ -4,9 +4,10 @@
 // next from a device queue when it finishes (persistent, no grid barrier:
 // workgroups never wait for each other).  Two storage modes share the code:
 //
-//   FP16   every posterior in LDS as FP16 (64800 x 2 bytes = 129,600 bytes of
-//          the 160 KB a gfx950 workgroup may hold); check records hold FP16
-//          magnitudes.  The FP16 message form.
+//   FP16   every posterior in LDS as FP16 (180 rows of 424 halves, the last
+//          64 a copy of the first 64: 152,640 bytes of the 160 KB a gfx950
+//          workgroup may hold); check records hold FP16 magnitudes.  The
+//          FP16 message form.
 //   FP32   the plain decoder, bit-exact with the reference: posteriors FP32,
 //          the rows of the highest-degree variable groups in LDS (up to 113
 //          rows of 360 floats = 162,720 bytes) and the remaining rows in this
@@ -59,7 +60,10 @@
 #define RES_MAX_DV 32
 #define RES_SIGN_LO 25                  // sign bits held in record word 1
 #define RES_F32_LDS_ROWS 113            // FP32 posterior rows that fit in LDS
-#define RES_F16_LDS_BYTES (RES_NMAX * 2)
+// FP16 posterior rows in LDS are padded: positions 0..359, then 0..63 again,
+// so the 64 positions a wave reads for one edge are contiguous (no mod 360).
+#define RES_PROW 424
+#define RES_F16_LDS_BYTES ((RES_NMAX / ATSC3_Z) * RES_PROW * 2)
 #define RES_F32_LDS_BYTES (RES_F32_LDS_ROWS * ATSC3_Z * 4)
 
 namespace atsc3 {
@@ -78,8 +82,11 @@ struct ResTab {
     const QC_CONST int *c_tasks;  // task: group | c6 << 8 (rows c6*64 .. c6*64+63)
     const QC_CONST int *v_steps;  // variable steps, same packing (+ Type A region << 23)
     const QC_CONST int *v_tasks;
+    // FP16 / INT8 modes: at b * 6 + c6, the byte offset in LDS of the padded
+    // posterior row position that check (g, c6*64) meets on base edge b
+    const QC_CONST int *c_off;
 };
-static_assert(sizeof(ResTab) == 96, "resident code descriptor layout");
+static_assert(sizeof(ResTab) == 104, "resident code descriptor layout");
 
 struct ResArgs {
     const float *llr;
@@ -119,7 +126,7 @@ struct Tot {
 template <int M>
 __device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, unsigned p) {
     if constexpr (M != 0) {
-        return __half2float(reinterpret_cast<const __half *>(T.lds)[row * ATSC3_Z + p]);
+        return __half2float(reinterpret_cast<const __half *>(T.lds)[row * RES_PROW + p]);
     } else {
         return glob ? at(T.glob, row * ATSC3_Z + p)
                     : reinterpret_cast<const float *>(T.lds)[row * ATSC3_Z + p];
@@ -130,7 +137,10 @@ template <int M>
 __device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, unsigned p,
                                        float v) {
     if constexpr (M != 0) {
-        reinterpret_cast<__half *>(T.lds)[row * ATSC3_Z + p] = __ushort_as_half(h_bits_sat(v));
+        __half *h = reinterpret_cast<__half *>(T.lds) + row * RES_PROW;
+        const __half hv = __ushort_as_half(h_bits_sat(v));
+        h[p] = hv;
+        if (p < 64u) h[p + ATSC3_Z] = hv;
     } else {
         if (glob)
             at(T.glob, row * ATSC3_Z + p) = v;
@@ -227,9 +237,15 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
         const int b0 = (ct->grp_c[g] >> 7) & 2047;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const unsigned ew = (unsigned)ct->edge_c[b0 + k];
-            const unsigned p = wrap360(jj + (ew & 511u));
-            tv[gi][k] = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
+            if constexpr (M == 0) {
+                const unsigned ew = (unsigned)ct->edge_c[b0 + k];
+                const unsigned p = wrap360(jj + (ew & 511u));
+                tv[gi][k] = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
+            } else {
+                const unsigned off = (unsigned)ct->c_off[(b0 + k) * 6 + (task >> 8)];
+                tv[gi][k] = __half2float(
+                    *reinterpret_cast<const __half *>(T.lds + off + 2u * lane));
+            }
         }
     }
     __builtin_amdgcn_sched_barrier(0);

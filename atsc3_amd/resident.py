@@ -30,7 +30,9 @@ MAX_DC = 89
 MAX_DV = 32
 
 
-_TABLES = ("grp_c", "edge_c", "grp_v", "edge_v", "c_steps", "c_tasks", "v_steps", "v_tasks")
+_TABLES = ("grp_c", "edge_c", "grp_v", "edge_v", "c_steps", "c_tasks", "v_steps", "v_tasks",
+           "c_off")
+PROW = 424               # padded FP16 posterior row in LDS: 0..359, then 0..63 again
 
 
 class _ResCode(ctypes.Structure):
@@ -40,7 +42,7 @@ class _ResCode(ctypes.Structure):
                 + [(f, ctypes.c_void_p) for f in _TABLES])
 
 
-assert ctypes.sizeof(_ResCode) == 96
+assert ctypes.sizeof(_ResCode) == 104
 
 # tasks packed per wave step by degree: check steps of degree <= 16 take 2
 # tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
@@ -143,11 +145,16 @@ def _host_tables(code, f32=False):
     t_of_b = (be >> 9) & 511
     edge_c = (be & 511) | (row[t_of_b] << 9) | (miss << 18) | (glob[t_of_b] << 28)
     grp_v = dv | (vg[:-1] << 6) | (n0 << 17) | (row << 23) | (glob << 31)
+    # FP16 / INT8: byte offset of the first of a wave's 64 posteriors on each
+    # (base edge, 64-row block) in the padded LDS rows
+    c64 = np.arange(6, dtype=np.int64) * 64
+    c_off = 2 * (row[t_of_b][:, None] * PROW + (c64[None, :] + (be & 511)[:, None]) % Z)
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int64).astype(np.uint32).view(np.int32))
     return dict(n=code.n, ncg=len(dc), nvg=nvg, nbe=nbe, max_dc=int(dc.max()),
                 global_rows=int(glob.sum()),
                 max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
                 c_steps=i32(c_steps), c_tasks=i32(c_tasks), v_steps=i32(v_steps),
+                c_off=i32(c_off.ravel()),
                 v_tasks=i32(v_tasks),
                 grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
                 edge_c=i32(edge_c),

@@ -232,6 +232,10 @@ typedef struct atsc3_res_code {
     const int32_t *v_steps; /* [nsv] variable steps, same packing, over v_tasks,
                                | Type A region << 23                             */
     const int32_t *v_tasks;
+    const int32_t *c_off;   /* FP16/INT8: [nbe * 6] at b*6 + c6, the byte offset
+                               in LDS of the posterior that check (g, c6*64)
+                               meets on base edge b; rows of 424 halves
+                               (positions 0..359, then 0..63 again)           */
 } atsc3_res_code;
 
 int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,

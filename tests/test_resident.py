@@ -150,6 +150,19 @@ def test_posterior_rows(n, rate, f32):
         assert len(np.unique(r)) == len(r) and (len(r) == 0 or r.max() == len(r) - 1)
     if not f32:
         assert (glob == 0).all() and np.array_equal(row, np.arange(q["nvg"]))
+        # lane l of the wave on checks (g, c6*64 .. c6*64+63) reads the half
+        # at c_off / 2 + l of rows of 424 halves, positions 360..423 being
+        # copies of 0..63: it must be posterior (t, (j + s) mod 360), j = c6*64 + l
+        from atsc3_amd.resident import PROW
+        off = h["c_off"].astype(np.int64).reshape(-1, 6) // 2
+        s = q["be_word"].astype(np.int64) & 511
+        lane = np.arange(64)
+        for c6 in range(6):
+            a = off[:, c6, None] + lane[None, :]
+            r, pos = a // PROW, a % PROW
+            assert np.array_equal(r, np.broadcast_to(row[t_of_b][:, None], r.shape))
+            unpadded = np.where(pos >= 360, pos - 360, pos)
+            assert np.array_equal(unpadded, (c6 * 64 + lane[None, :] + s[:, None]) % 360)
     else:
         nl = int((glob == 0).sum())
         assert nl == min(q["nvg"], F32_LDS_ROWS) and h["global_rows"] == q["nvg"] - nl
