@@ -139,6 +139,13 @@ __device__ __forceinline__ T &at(T *base, unsigned idx) {
                                   (uint64_t)(idx * (unsigned)sizeof(T)));
 }
 
+// The same with a byte offset (the caller has scaled the index).
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, unsigned, unsigned byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                        (uint64_t)byte_off);
+}
+
 __device__ __forceinline__ float clamp_mag(float x) { return fminf(fabsf(x), INFINITY); }
 
 // G checks (g_0..g_{G-1}, j), all of degree D: every load of the G checks

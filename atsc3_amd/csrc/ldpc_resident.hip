@@ -38,14 +38,16 @@
 //   (barrier; the syndrome total decides: converged or at `iters` -> bits,
 //    conv, its, bad of the codeword, next codeword)
 //   VAR    lane p of a wave takes variable (t, p): gathers the new record of
-//          each of its checks, rebuilds the message, sums in FP32 in
-//          ascending check index (the reference's order: per region, the
-//          entries whose check position wrapped past 359 first, then the
-//          rest, each in table order) and stores tot = lam + s.
+//          each of its checks, rebuilds the message, sums in FP32 and
+//          stores tot = lam + s.  FP32 mode sums in ascending check index
+//          (the reference's order: per region, the entries whose check
+//          position wrapped past 359 first, then the rest, each in table
+//          order); FP16 and INT8 modes sum in table order (no per-lane
+//          order selects: they are not bound to the reference's rounding).
 //
 // Arithmetic: every message is alpha * min with the sign of the plain rule
 // (FP16 mode: fp16_rn(sat(alpha * min)), exactly as the FP16 message format
-// stores it); every sum is FP32 in the reference's order; FP16 mode rounds
+// stores it); every sum is FP32 in the order above; FP16 mode rounds
 // the posterior to FP16 (saturated to +-65504) where it is stored.  The
 // checker (oracle/resident_ref.py) runs the same steps in numpy; its FP32
 // mode reproduces the reference kernel's golden outputs bit for bit.
@@ -343,8 +345,8 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
 
 // G variable tasks of degree D <= 16 whose entries are all full circulants
 // and all name check edges below 25 (record word 1 holds their signs): every
-// record and LLR load of the G tasks first, then the sums in ascending check
-// index.  Within a region the entries are sorted by u, so the ones whose
+// record and LLR load of the G tasks first, then the sums: FP16 / INT8 in
+// table order; FP32 in ascending check index.  Within a region the entries are sorted by u, so the ones whose
 // check position wrapped past 359 (p + u >= 360) are a suffix of the
 // region's list; the order is that suffix, then the prefix.  A skipped
 // message is added as +0.0, which leaves the sum unchanged (it starts at
@@ -356,7 +358,7 @@ template <int M, int D, int G, bool TA>
 __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &T,
                                           const float *lam, const Rec &R, int start,
                                           unsigned lane) {
-    unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G];
+    unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G], pp4[G];
     bool wr[G][D], glob[G];
     float lv[G];
     bool valid[G];
@@ -369,6 +371,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
         valid[gi] = p < (unsigned)ATSC3_Z;
         const unsigned pp = valid[gi] ? p : 0u;
         dst[gi] = pp;
+        pp4[gi] = 4u * pp;
         const unsigned vw = (unsigned)ct->grp_v[t];
         e0[gi] = (vw >> 6) & 2047;
         n0[gi] = TA ? (vw >> 17) & 63 : 0;
@@ -379,11 +382,20 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u;
-            wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
-            const unsigned r = g * ATSC3_Z + pp + u - (wr[gi][k] ? (unsigned)ATSC3_Z : 0u);
-            w0[gi][k] = at(R.p0, r);
-            if (M == 0) w0b[gi][k] = at(R.p0b, r);
-            w1[gi][k] = at(R.p1, r);
+            if constexpr (M == 0) {
+                wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
+                const unsigned r = g * ATSC3_Z + pp + u - (wr[gi][k] ? (unsigned)ATSC3_Z : 0u);
+                w0[gi][k] = at(R.p0, r);
+                w0b[gi][k] = at(R.p0b, r);
+                w1[gi][k] = at(R.p1, r);
+            } else {
+                // byte offset within group g's records, wrapped (min(q, q - 1440)),
+                // on a wave-uniform base: add, subtract, min per entry
+                const unsigned q = pp4[gi] + 4u * u;
+                const unsigned ob = min(q, q - 4u * ATSC3_Z);
+                w0[gi][k] = at(R.p0 + g * ATSC3_Z, 0u, ob);
+                w1[gi][k] = at(R.p1 + g * ATSC3_Z, 0u, ob);
+            }
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -405,8 +417,9 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
             m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - kk)) & 0x80000000u));
         }
         float s = 0.0f;
-        if constexpr (!TA && D == 2) {
-            s = (s + m[0]) + m[1];
+        if constexpr (M != 0 || (!TA && D == 2)) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += m[k];
         } else {
             if (TA) {
 #pragma unroll
@@ -428,7 +441,21 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
 }
 
 // Any degree: one record load per entry, visited in the summation order
-// directly (suffix then prefix of each region).
+// directly (FP32: suffix then prefix of each region; FP16 / INT8: table
+// order).
+template <int M>
+__device__ float var_term(const QC_CONST ResTab *ct, const Rec &R, int e, unsigned pp) {
+    const unsigned ew = (unsigned)ct->edge_v[2 * e];
+    const unsigned vm = (unsigned)ct->edge_v[2 * e + 1];
+    const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
+    const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
+    const unsigned long long w2 = kk >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
+    const float mv = rec_msg<M>(at(R.p0, rix), M == 0 ? at(R.p0b, rix) : 0u, at(R.p1, rix),
+                                w2, kk, R);
+    // a missing circulant entry adds +0.0: the sum (started at +0.0) is unchanged
+    return vm != pp + 1u ? mv : 0.0f;
+}
+
 template <int M>
 __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float *lam,
                             const Rec &R, int t, unsigned p, bool valid) {
@@ -437,24 +464,21 @@ __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float
     const unsigned pp = valid ? p : 0u;
     const float lv = lam[(unsigned)t * ATSC3_Z + pp];
     float s = 0.0f;
-    for (int r = 0; r < 2; ++r) {
-        const int lo = r == 0 ? 0 : n0, hi = r == 0 ? n0 : dv;
-        // first wrapped entry of the region (u ascending: a suffix)
-        int tr = hi;
-        while (tr > lo &&
-               pp + ((unsigned)ct->edge_v[2 * (e0 + tr - 1)] & 511u) >= (unsigned)ATSC3_Z)
-            --tr;
-        for (int q = 0; q < hi - lo; ++q) {
-            int k = tr + q;
-            if (k >= hi) k -= hi - lo;
-            const unsigned ew = (unsigned)ct->edge_v[2 * (e0 + k)];
-            const unsigned vm = (unsigned)ct->edge_v[2 * (e0 + k) + 1];
-            const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
-            const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
-            const unsigned long long w2 = kk >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
-            const float mv = rec_msg<M>(at(R.p0, rix), M == 0 ? at(R.p0b, rix) : 0u,
-                                        at(R.p1, rix), w2, kk, R);
-            if (vm != pp + 1u) s += mv;
+    if constexpr (M != 0) {
+        for (int k = 0; k < dv; ++k) s += var_term<M>(ct, R, e0 + k, pp);
+    } else {
+        for (int r = 0; r < 2; ++r) {
+            const int lo = r == 0 ? 0 : n0, hi = r == 0 ? n0 : dv;
+            // first wrapped entry of the region (u ascending: a suffix)
+            int tr = hi;
+            while (tr > lo &&
+                   pp + ((unsigned)ct->edge_v[2 * (e0 + tr - 1)] & 511u) >= (unsigned)ATSC3_Z)
+                --tr;
+            for (int q = 0; q < hi - lo; ++q) {
+                int k = tr + q;
+                if (k >= hi) k -= hi - lo;
+                s += var_term<M>(ct, R, e0 + k, pp);
+            }
         }
     }
     if (valid) tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
@@ -569,7 +593,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     var_generic<M>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
                     continue;
                 }
-                const bool ta = (sw >> 23) & 1;
+                const bool ta = M == 0 && ((sw >> 23) & 1);
                 switch ((ta ? 1024 : 0) + G * 128 + D) {
 #define RES_V1(DD)                                                      \
     case 128 + DD:                                                      \

@@ -4,7 +4,8 @@
 codes with FP16 check-to-variable messages (each check's messages kept as
 one compressed record: FP16 alpha*min1 and alpha*min2, the edge taking min2,
 the signs) and the codeword's variable posteriors held in the workgroup's
-LDS as FP16.  The sums are FP32 in the reference's order; the posterior is
+LDS as FP16.  The sums are FP32 in the order of each variable group's entry
+list (FP32 mode: the reference's ascending check order); the posterior is
 rounded to FP16 where it is stored.  It stands where the upstream decodes a
 frame's blocks with per-code sub-batched min-sum calls (lab/m9_fast.py:
 843-925, lab/m44_ldm.py:2497-2576 over lab/ldpc_kernel.c:65-146), as the

@@ -184,8 +184,10 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
  * compressed record (alpha*min1, alpha*min2, the edge taking min2, the
  * signs).  Two storage modes:
  *   msg_fmt 1  FP16: posteriors in LDS as FP16, FP16 record magnitudes (the
- *              FP16 message format's values); sums FP32 in the reference's
- *              order, the posterior rounded to FP16 where it is stored.
+ *              FP16 message format's values); each variable's messages
+ *              summed in FP32 in the order of its entry list (the
+ *              atsc3_res_code edge_v order), the posterior rounded to FP16
+ *              where it is stored.
  *   msg_fmt 0  FP32, bit-exact with lab/ldpc_kernel.c (bits, conv, its,
  *              bad): FP32 posteriors, the rows of the variable groups whose
  *              grp_v word says so in LDS (at most

@@ -14,9 +14,14 @@ two storage roundings when ``storage="float16"``:
     in LDS as FP16), including the initial tot = lam.
 
 Messages are kept per check as the compressed record (a1, a2, index of the
-edge taking a2, per-edge signs); the variable sum visits, per region, the
-entries whose check position wrapped past 359 first, then the others, each
-in table order -- the ascending-check order of the reference.
+edge taking a2, per-edge signs).  With FP32 storage the variable sum visits,
+per region, the entries whose check position wrapped past 359 first, then
+the others, each in table order -- the ascending-check order of the
+reference (lab/ldpc_kernel.c:137-142).  With FP16 or INT8 storage it visits
+the entries in table order (every entry of the variable group's list, in
+list order, regions included): these reduced-precision forms are not bound
+to the reference's rounding, and the table order costs the kernel no
+per-lane selects.
 
 Pinning (tests/test_resident.py): with ``storage="float32"`` every rounding
 is the identity and the function must reproduce the upstream kernel's own
@@ -71,9 +76,10 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
     INT8 message format's codec), with ``steps`` one per codeword or None
     (``int8_step`` of each codeword's LLRs).
 
-    ``_plant="list_order"`` is a negative control for the tests: each
-    variable's messages are summed in table order, ignoring the wrap
-    rotation -- a plausible mistake the golden fixtures must catch."""
+    ``_plant="list_order"`` is a negative control for the tests: with FP32
+    storage each variable's messages are summed in table order (the FP16 /
+    INT8 order), ignoring the wrap rotation -- a plausible mistake the golden
+    fixtures must catch."""
     T = _tables(code)
     llr = np.asarray(llr2d, np.float32)
     B = llr.shape[0]
@@ -88,6 +94,7 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
         storage_tot = "float16"
     else:
         storage_tot = storage
+    table_order = _plant == "list_order" or storage != "float32"
     if storage_tot == "float16":
         def rnd(x):
             return np.clip(x, -65504.0, 65504.0).astype(np.float16).astype(f32)
@@ -158,9 +165,9 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
             for tg in range(nvg):
                 ents = np.arange(T["vg_ptr"][tg], T["vg_ptr"][tg + 1])
                 s = np.zeros((B, Z), f32)
-                for r in (0, 1):
-                    er = ents[T["vreg"][ents] == r]
-                    for wrapped in ((None,) if _plant == "list_order" else (True, False)):
+                for r in ((None,) if table_order else (0, 1)):
+                    er = ents if r is None else ents[T["vreg"][ents] == r]
+                    for wrapped in ((None,) if table_order else (True, False)):
                         for e in er:
                             b = T["vb"][e]
                             g, k, u, vm = T["g_of_b"][b], T["k_of_b"][b], T["vu"][e], T["vmiss"][e]
