@@ -24,6 +24,9 @@
 // one word a1 | a2 << 16; FP32: two words), word 1 = the signs of edges 0..24
 // and the index of the edge that takes a2 (bits 25..31), and a 64-bit word
 // with the signs of edges 25..88, touched only by checks of degree > 25.
+// Checks of degree <= 16 without missing entries (check_multi) build their
+// sign word by shifting in one bit per edge, so edge k's sign sits at bit
+// D - 1 - k; the tables give the variable side each entry's bit.
 // Records live in this workgroup's slot in global memory (21,600 checks x 8
 // or 12 bytes for rate 10/15, against 2 or 4 bytes per EDGE for stored
 // messages); the check phase rewrites them in place (only the owning lane
@@ -79,7 +82,8 @@ struct ResTab {
                                   //   | global << 28
     const QC_CONST int *grp_v;    // per variable group: dv | e0 << 6 | n0 << 17 | row << 23
                                   //   | global << 31
-    const QC_CONST int *edge_v;   // per entry, two words: u | g << 9 | k << 17; missing p + 1
+    const QC_CONST int *edge_v;   // per entry, two words: u | g << 9 | k << 17 | sign bit << 24;
+                                  //   missing p + 1
     const QC_CONST int *c_steps;  // check steps: D | G << 7 | start << 10 | generic << 22
     const QC_CONST int *c_tasks;  // task: group | c6 << 8 (rows c6*64 .. c6*64+63)
     const QC_CONST int *v_steps;  // variable steps, same packing (+ Type A region << 23)
@@ -190,23 +194,36 @@ __device__ __forceinline__ void rec_put_mag(const Rec &R, unsigned r, float a1, 
     }
 }
 
-// Bit 31 set iff edge kk's message is negative (record words w1, w2).
-__device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2, unsigned kk) {
-    return (kk < RES_SIGN_LO ? w1 << (31u - kk) : (unsigned)(w2 >> (kk - RES_SIGN_LO)) << 31) &
+// neg * 2 + this lane's `bit`, as one v_addc_co_u32 whose carry-in is the
+// wave's lane mask of `bit` (the compiler otherwise spends a select, a shift
+// and an or per bit).
+__device__ __forceinline__ unsigned shl1_add_lane_bit(unsigned neg, bool bit) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(bit);
+    unsigned long long co;
+    unsigned r;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(co) : "v"(neg), "s"(m));
+    return r;
+}
+
+// Bit 31 set iff the message whose sign sits at bit sp is negative (record
+// words w1, w2; bits 0..24 of w1, then w2).
+__device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2, unsigned sp) {
+    return (sp < RES_SIGN_LO ? w1 << (31u - sp) : (unsigned)(w2 >> (sp - RES_SIGN_LO)) << 31) &
            0x80000000u;
 }
 
-// Message of edge kk of a check from its record.  a1, a2 are non-negative
-// (alpha >= 0), so setting the sign bit negates.
+// Message of edge kk of a check from its record, its sign at bit sp.  a1,
+// a2 are non-negative (alpha >= 0), so setting the sign bit negates.
 template <int M>
 __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w0b, unsigned w1,
-                                         unsigned long long w2, unsigned kk, const Rec &R) {
+                                         unsigned long long w2, unsigned kk, unsigned sp,
+                                         const Rec &R) {
     const bool a2 = kk == (w1 >> RES_SIGN_LO);
     float v;
     if constexpr (M == 0) v = __uint_as_float(a2 ? w0b : w0);
     else if constexpr (M == 1) v = h_val((unsigned short)(a2 ? (w0 >> 16) : w0));
     else v = (float)((a2 ? (w0 >> 8) : w0) & 0xffu) * R.step;
-    return __uint_as_float(__float_as_uint(v) | rec_sign(w1, w2, kk));
+    return __uint_as_float(__float_as_uint(v) | rec_sign(w1, w2, sp));
 }
 
 // G check tasks of degree D <= 16 with every circulant full: the syndrome
@@ -220,6 +237,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
     unsigned rix[G], w0[G], w0b[G], w1[G];
     bool valid[G];
     float tv[G][D];
+    __half th[G][D];                // FP16 / INT8: the posteriors as loaded
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
         const int task = ct->c_tasks[start + gi];
@@ -245,12 +263,17 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
                 tv[gi][k] = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
             } else {
                 const unsigned off = (unsigned)ct->c_off[(b0 + k) * 6 + (task >> 8)];
-                tv[gi][k] = __half2float(
-                    *reinterpret_cast<const __half *>(T.lds + off + 2u * lane));
+                th[gi][k] = *reinterpret_cast<const __half *>(T.lds + off + 2u * lane);
             }
         }
     }
     __builtin_amdgcn_sched_barrier(0);
+    // FP16 / INT8: x = tot - m as fma(tot, 1, -m), which is exactly the FP32
+    // difference (tot * 1 is exact; one rounding), taking tot as FP16: the
+    // compiler then emits one mixed-precision fma per edge instead of a
+    // conversion and a subtraction (the 1.0 is hidden from constant folding)
+    float one = 1.0f, ninf = -INFINITY;     // -inf hidden too: med3(a, b, -inf)
+    asm volatile("" : "+s"(one), "+s"(ninf)); // would fold back to a quieting fminf
     unsigned syn_total = 0u;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
@@ -266,17 +289,29 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const float ao = (unsigned)k == idxo ? a2o : a1o;
-            const float mo =
-                __uint_as_float(__float_as_uint(ao) | ((w1[gi] << (31 - k)) & 0x80000000u));
-            const float x = tv[gi][k] - mo;
-            syn ^= tv[gi][k] < 0.0f;
+            // fast checks keep edge k's sign at bit D - 1 - k
+            const float mo = __uint_as_float(__float_as_uint(ao) |
+                                             ((w1[gi] << (32 - D + k)) & 0x80000000u));
+            float x;
+            if constexpr (M == 0) {
+                x = tv[gi][k] - mo;
+                syn ^= tv[gi][k] < 0.0f;
+            } else {
+                x = __builtin_fmaf(__half2float(th[gi][k]), one, -mo);
+                syn ^= __half2float(th[gi][k]) < 0.0f;
+            }
             const bool ng = x < 0.0f;
             par ^= ng;
-            neg |= (unsigned)ng << k;
+            neg = shl1_add_lane_bit(neg, ng);       // edge k ends at bit D - 1 - k
+            // mag is never NaN (clamp_mag takes NaN to +inf), so min1 is the
+            // median of (m1, mag, -inf): med3 needs no NaN-quieting max on
+            // its operands, which fminf gets once the compiler sinks this
+            // chain into the record-update branch
             const float mag = clamp_mag(x);
+            asm volatile("" ::"v"(mag));           // clamp beside x, where x is known canonical
             idx = mag < m1 ? (unsigned)k : idx;
             m2 = __builtin_amdgcn_fmed3f(m1, m2, mag);
-            m1 = fminf(m1, mag);
+            m1 = __builtin_amdgcn_fmed3f(m1, mag, ninf);
         }
         if (update && valid[gi]) {
             const unsigned sg = par ? ~neg : neg;
@@ -404,7 +439,8 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
         float m[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const unsigned kk = ((unsigned)ct->edge_v[2 * (e0[gi] + k)] >> 17) & 127u;
+            const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
+            const unsigned kk = (ew >> 17) & 127u, sp = ew >> 24;
             const bool a2 = kk == (w1[gi][k] >> RES_SIGN_LO);
             float v;
             if constexpr (M == 0) {
@@ -414,7 +450,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
             } else {
                 v = (float)((a2 ? (w0[gi][k] >> 8) : w0[gi][k]) & 0xffu) * R.step;
             }
-            m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - kk)) & 0x80000000u));
+            m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - sp)) & 0x80000000u));
         }
         float s = 0.0f;
         if constexpr (M != 0 || (!TA && D == 2)) {
@@ -447,11 +483,11 @@ template <int M>
 __device__ float var_term(const QC_CONST ResTab *ct, const Rec &R, int e, unsigned pp) {
     const unsigned ew = (unsigned)ct->edge_v[2 * e];
     const unsigned vm = (unsigned)ct->edge_v[2 * e + 1];
-    const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u;
+    const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u, sp = ew >> 24;
     const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
-    const unsigned long long w2 = kk >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
+    const unsigned long long w2 = sp >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
     const float mv = rec_msg<M>(at(R.p0, rix), M == 0 ? at(R.p0b, rix) : 0u, at(R.p1, rix),
-                                w2, kk, R);
+                                w2, kk, sp, R);
     // a missing circulant entry adds +0.0: the sum (started at +0.0) is unchanged
     return vm != pp + 1u ? mv : 0.0f;
 }

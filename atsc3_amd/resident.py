@@ -96,9 +96,11 @@ def _host_tables(code, f32=False):
     FP16 (f32 False): every group's row is its index, in LDS.  FP32: the
     F32_LDS_ROWS groups of highest degree (most edge reads) get LDS rows, the
     others rows of the workgroup's global slot.
-    edge_v[2e]    u | g << 9 | k << 17: entry e of the variable list
-                  (summation order at p = 0) meets check (g, (p + u) % 360)
-                  as that check's edge k
+    edge_v[2e]    u | g << 9 | k << 17 | sp << 24: entry e of the variable
+                  list (summation order at p = 0) meets check (g, (p + u) %
+                  360) as that check's edge k, whose sign the check's record
+                  keeps at bit sp (dc - 1 - k for checks of the fast steps,
+                  which build the sign word by shifting, else k)
     edge_v[2e+1]  missing variable position + 1 (0 = full circulant)
     c_steps, c_tasks, v_steps, v_tasks  the work of each phase (_pack_tasks);
                   generic: a missing entry, degree > 16, or (variables) an
@@ -124,7 +126,8 @@ def _host_tables(code, f32=False):
         raise ValueError(f"code {code.rate} {code.n} exceeds the resident kernel's tables")
     vkmax = np.array([int(k_of_b[vb[vg[t]:vg[t + 1]]].max()) for t in range(len(dv))])
     vmissg = np.array([int((vmiss[vg[t]:vg[t + 1]] > 0).any()) for t in range(len(dv))])
-    c_tasks, c_steps = _pack_tasks(dc, (hasmiss > 0) | (dc > FAST_MAX), CHECK_PACK)
+    c_generic = (hasmiss > 0) | (dc > FAST_MAX)
+    c_tasks, c_steps = _pack_tasks(dc, c_generic, CHECK_PACK)
     v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25), VAR_PACK)
     # a variable step with any group that has a first region (Type A: n0 > 0
     # entries summed before the rest) carries bit 23
@@ -150,6 +153,8 @@ def _host_tables(code, f32=False):
     # (base edge, 64-row block) in the padded LDS rows
     c64 = np.arange(6, dtype=np.int64) * 64
     c_off = 2 * (row[t_of_b][:, None] * PROW + (c64[None, :] + (be & 511)[:, None]) % Z)
+    gb, kb = g_of_b[vb], k_of_b[vb]
+    sp = np.where(c_generic[gb], kb, dc[gb] - 1 - kb)
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int64).astype(np.uint32).view(np.int32))
     return dict(n=code.n, ncg=len(dc), nvg=nvg, nbe=nbe, max_dc=int(dc.max()),
                 global_rows=int(glob.sum()),
@@ -160,7 +165,8 @@ def _host_tables(code, f32=False):
                 grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
                 edge_c=i32(edge_c),
                 grp_v=i32(grp_v),
-                edge_v=i32(np.stack([vu | (g_of_b[vb] << 9) | (k_of_b[vb] << 17), vmiss],
+                edge_v=i32(np.stack([vu | (g_of_b[vb] << 9) | (k_of_b[vb] << 17) | (sp << 24),
+                                     vmiss],
                                     1).ravel()))
 
 

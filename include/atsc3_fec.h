@@ -226,8 +226,11 @@ typedef struct atsc3_res_code {
                                row in LDS, or in the global slot)                 */
     const int32_t *grp_v;   /* per variable group: dv | e0 << 6 | n0 << 17 |
                                row << 23 | global << 31                           */
-    const int32_t *edge_v;  /* per entry, 2 words: u | g << 9 | k << 17;
-                               missing variable position + 1                     */
+    const int32_t *edge_v;  /* per entry, 2 words: u | g << 9 | k << 17 |
+                               sp << 24 (sp: the bit of the check's record that
+                               holds edge k's sign: dc - 1 - k for checks of
+                               the non-generic steps, else k); missing variable
+                               position + 1                                      */
     const int32_t *c_steps; /* [nsc] check steps: D | G << 7 | start << 10 |
                                generic << 22, over c_tasks                       */
     const int32_t *c_tasks; /* group | c6 << 8: rows c6*64 .. c6*64+63         */

@@ -105,6 +105,11 @@ def test_packed_tables_decode_to_the_base_graph(n, rate):
             s, tb, miss = be[b] & 511, (be[b] >> 9) & 511, (be[b] >> 18) & 1023
             assert tb == t and (s + u) % 360 == 0
             assert ev[e, 1] == (0 if miss == 0 else (miss - 1 + s) % 360 + 1)
+            # sign bit of the check's record: shifted-in order (bit dc-1-k)
+            # for checks without missing entries of degree <= 16, else k
+            dcg = cg[g + 1] - cg[g]
+            fast = dcg <= 16 and not ((be[cg[g]:cg[g + 1]] >> 18) & 1023).any()
+            assert ev[e, 0] >> 24 == (dcg - 1 - k if fast else k)
             seen[b] += 1
     assert (seen == 1).all()
     # every 64-row task of both phases is covered once, in steps of G tasks
