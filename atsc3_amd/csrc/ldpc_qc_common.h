@@ -91,8 +91,17 @@ __device__ inline float int8_step_of_block(const float *lam, int n, int *hist) {
     const int rank = (n - 1) / 2;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    for (int v = threadIdx.x; v < n; v += blockDim.x)
-        atomicAdd(&hist[(__float_as_uint(fabsf(lam[v])) >> 23) & 255], 1);
+    // 8 loads in flight per thread before the histogram updates
+    const int step = blockDim.x;
+    for (int v0 = threadIdx.x; v0 < n; v0 += 8 * step) {
+        unsigned u[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            u[q] = v0 + q * step < n ? __float_as_uint(fabsf(lam[v0 + q * step])) : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (u[q] != 0xffffffffu) atomicAdd(&hist[(u[q] >> 23) & 255], 1);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int c = 0, e = 0;
@@ -105,9 +114,15 @@ __device__ inline float int8_step_of_block(const float *lam, int n, int *hist) {
     const int r2 = sel[1];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    for (int v = threadIdx.x; v < n; v += blockDim.x) {
-        const unsigned u = __float_as_uint(fabsf(lam[v]));
-        if (((u >> 23) & 255) == e) atomicAdd(&hist[(u >> 15) & 255], 1);
+    for (int v0 = threadIdx.x; v0 < n; v0 += 8 * step) {
+        unsigned u[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            u[q] = v0 + q * step < n ? __float_as_uint(fabsf(lam[v0 + q * step])) : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (u[q] != 0xffffffffu && ((u[q] >> 23) & 255) == e)
+                atomicAdd(&hist[(u[q] >> 15) & 255], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {

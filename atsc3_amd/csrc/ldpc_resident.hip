@@ -558,13 +558,26 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             R.step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, ct->n, s_hist);
             R.inv = 1.0f / R.step;
         }
-        // tot = lam, row by row (each 64-position task of a group on one wave)
-        for (int task = wave; task < nvg * 6; task += RES_WAVES) {
-            const int t = task / 6;
-            const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
-            const unsigned vw = (unsigned)ct->grp_v[t];
-            if (p < (unsigned)ATSC3_Z)
-                tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, lam[(unsigned)t * ATSC3_Z + p]);
+        // tot = lam, row by row (each 64-position task of a group on one
+        // wave), the loads of 8 tasks in flight together
+        for (int t8 = wave; t8 < nvg * 6; t8 += RES_WAVES * 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int task = t8 + q * RES_WAVES, t = task / 6;
+                const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
+                v[q] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)t * ATSC3_Z + p]
+                                                               : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int task = t8 + q * RES_WAVES, t = task / 6;
+                const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
+                if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
+                    const unsigned vw = (unsigned)ct->grp_v[t];
+                    tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, v[q]);
+                }
+            }
         }
         __syncthreads();
         for (int it = 0;; ++it) {
@@ -638,12 +651,16 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
     case 1024 + 128 + DD:                                               \
         var_multi<M, DD, 1, true>(ct, T, lam, R, start, lane);        \
         break;
+// pairs of degree 9..16 tasks: FP16 / INT8 only (FP32 keeps them single:
+// its two record words and ordered sums would not fit the register budget)
 #define RES_V2(DD)                                                      \
     case 256 + DD:                                                      \
-        var_multi<M, DD, 2, false>(ct, T, lam, R, start, lane);       \
+        if constexpr (M != 0 || DD <= 8)                                \
+            var_multi<M, DD, 2, false>(ct, T, lam, R, start, lane);   \
         break;                                                          \
     case 1024 + 256 + DD:                                               \
-        var_multi<M, DD, 2, true>(ct, T, lam, R, start, lane);        \
+        if constexpr (M == 0 && DD <= 8)                                \
+            var_multi<M, DD, 2, true>(ct, T, lam, R, start, lane);    \
         break;
 #define RES_V4(DD)                                                      \
     case 512 + DD:                                                      \
@@ -653,7 +670,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         var_multi<M, DD, 4, true>(ct, T, lam, R, start, lane);        \
         break;
                     QC_CASES16(RES_V1)
-                    QC_CASES8(RES_V2)
+                    QC_CASES16(RES_V2)
                     QC_CASES4(RES_V4)
 #undef RES_V1
 #undef RES_V2

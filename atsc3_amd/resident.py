@@ -48,7 +48,8 @@ assert ctypes.sizeof(_ResCode) == 104
 # tasks packed per wave step by degree: check steps of degree <= 16 take 2
 # tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
 CHECK_PACK = ((16, 2),)
-VAR_PACK = ((4, 4), (8, 2), (16, 1))
+VAR_PACK = ((4, 4), (16, 2))             # FP16 / INT8: more record loads in flight
+VAR_PACK_F32 = ((4, 4), (8, 2), (16, 1))  # FP32: two record words, ordered sums
 FAST_MAX = 16
 
 
@@ -128,7 +129,8 @@ def _host_tables(code, f32=False):
     vmissg = np.array([int((vmiss[vg[t]:vg[t + 1]] > 0).any()) for t in range(len(dv))])
     c_generic = (hasmiss > 0) | (dc > FAST_MAX)
     c_tasks, c_steps = _pack_tasks(dc, c_generic, CHECK_PACK)
-    v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25), VAR_PACK)
+    v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25),
+                                   VAR_PACK_F32 if f32 else VAR_PACK)
     # a variable step with any group that has a first region (Type A: n0 > 0
     # entries summed before the rest) carries bit 23
     ta = n0 > 0
