@@ -84,7 +84,7 @@ struct ResTab {
                                   //   | global << 31
     const QC_CONST int *edge_v;   // per entry, two words: u | g << 9 | k << 17 | sign bit << 24;
                                   //   missing p + 1
-    const QC_CONST int *c_steps;  // check steps: D | G << 7 | start << 10 | generic << 22
+    const QC_CONST int *c_steps;  // check steps: D | log2 G << 7 | start << 10 | generic << 22
     const QC_CONST int *c_tasks;  // task: group | c6 << 8 (rows c6*64 .. c6*64+63)
     const QC_CONST int *v_steps;  // variable steps, same packing (+ Type A region << 23)
     const QC_CONST int *v_tasks;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             unsigned unsat = 0u;
             for (int si = wave; si < ct->nsc; si += RES_WAVES) {
                 const int sw = ct->c_steps[si];
-                const int D = sw & 127, G = (sw >> 7) & 7, start = (sw >> 10) & 4095;
+                const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
                 if ((sw >> 22) & 1) {
                     const int task = ct->c_tasks[start];
                     const int g = task & 255;
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             }
             for (int si = wave; si < ct->nsv; si += RES_WAVES) {
                 const int sw = ct->v_steps[si];
-                const int D = sw & 127, G = (sw >> 7) & 7, start = (sw >> 10) & 4095;
+                const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
                 if ((sw >> 22) & 1) {
                     const int task = ct->v_tasks[start];
                     const unsigned p = (unsigned)(task >> 8) * 64u + lane;
@@ -643,12 +643,12 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     continue;
                 }
                 const bool ta = M == 0 && ((sw >> 23) & 1);
-                switch ((ta ? 1024 : 0) + G * 128 + D) {
+                switch ((ta ? 2048 : 0) + G * 128 + D) {
 #define RES_V1(DD)                                                      \
     case 128 + DD:                                                      \
         var_multi<M, DD, 1, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
-    case 1024 + 128 + DD:                                               \
+    case 2048 + 128 + DD:                                               \
         var_multi<M, DD, 1, true>(ct, T, lam, R, start, lane);        \
         break;
 // pairs of degree 9..16 tasks: FP16 / INT8 only (FP32 keeps them single:
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         if constexpr (M != 0 || DD <= 8)                                \
             var_multi<M, DD, 2, false>(ct, T, lam, R, start, lane);   \
         break;                                                          \
-    case 1024 + 256 + DD:                                               \
+    case 2048 + 256 + DD:                                               \
         if constexpr (M == 0 && DD <= 8)                                \
             var_multi<M, DD, 2, true>(ct, T, lam, R, start, lane);    \
         break;
@@ -666,15 +666,23 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
     case 512 + DD:                                                      \
         var_multi<M, DD, 4, false>(ct, T, lam, R, start, lane);       \
         break;                                                          \
-    case 1024 + 512 + DD:                                               \
+    case 2048 + 512 + DD:                                               \
         var_multi<M, DD, 4, true>(ct, T, lam, R, start, lane);        \
+        break;
+// eight tasks of degree <= 3: FP16 / INT8 only
+#define RES_V8(DD)                                                      \
+    case 1024 + DD:                                                     \
+        if constexpr (M != 0)                                           \
+            var_multi<M, DD, 8, false>(ct, T, lam, R, start, lane);   \
         break;
                     QC_CASES16(RES_V1)
                     QC_CASES16(RES_V2)
                     QC_CASES4(RES_V4)
+                    RES_V8(1) RES_V8(2) RES_V8(3)
 #undef RES_V1
 #undef RES_V2
 #undef RES_V4
+#undef RES_V8
                     default: break;
                 }
             }

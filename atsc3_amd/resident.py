@@ -48,7 +48,7 @@ assert ctypes.sizeof(_ResCode) == 104
 # tasks packed per wave step by degree: check steps of degree <= 16 take 2
 # tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
 CHECK_PACK = ((16, 2),)
-VAR_PACK = ((4, 4), (16, 2))             # FP16 / INT8: more record loads in flight
+VAR_PACK = ((3, 8), (4, 4), (16, 2))     # FP16 / INT8: more record loads in flight
 VAR_PACK_F32 = ((4, 4), (8, 2), (16, 1))  # FP32: two record words, ordered sums
 FAST_MAX = 16
 
@@ -56,8 +56,8 @@ FAST_MAX = 16
 def _pack_tasks(deg, generic, pack):
     """Tasks (group | c6 << 8: rows c6*64 .. c6*64+63 of a 360-row group) in
     the order groups sorted by (generic, -degree), cut into steps
-    D | G << 7 | start << 10 | generic << 22 of G equal-degree tasks
-    (G in {1, 2, 4}); a generic step holds one task."""
+    D | log2(G) << 7 | start << 10 | generic << 22 of G equal-degree tasks
+    (G in {1, 2, 4, 8}); a generic step holds one task."""
     deg = np.asarray(deg, np.int64)
     generic = np.asarray(generic, bool)
     order = np.lexsort((np.arange(len(deg)), -deg, generic))
@@ -67,7 +67,7 @@ def _pack_tasks(deg, generic, pack):
         g = tasks[i] & 255
         d = int(deg[g])
         if generic[g]:
-            steps.append(min(d, 127) | (1 << 7) | (i << 10) | (1 << 22))
+            steps.append(min(d, 127) | (i << 10) | (1 << 22))
             i += 1
             continue
         gmax = next(gg for lim, gg in pack if d <= lim)
@@ -76,7 +76,7 @@ def _pack_tasks(deg, generic, pack):
                and deg[tasks[i + run] & 255] == d):
             run += 1
         k = 1 << (run.bit_length() - 1)
-        steps.append(d | (k << 7) | (i << 10))
+        steps.append(d | ((k.bit_length() - 1) << 7) | (i << 10))
         i += k
     return tasks, np.asarray(steps, np.int64)
 
@@ -136,7 +136,7 @@ def _host_tables(code, f32=False):
     ta = n0 > 0
 
     def _ta(w):
-        first, cnt = (w >> 10) & 4095, (w >> 7) & 7
+        first, cnt = (w >> 10) & 4095, 1 << ((w >> 7) & 7)
         return any(ta[v_tasks[first + i] & 255] for i in range(cnt))
     v_steps = np.array([w | (1 << 23 if _ta(w) else 0) for w in v_steps], np.int64)
     nvg = len(dv)

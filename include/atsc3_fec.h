@@ -231,7 +231,7 @@ typedef struct atsc3_res_code {
                                holds edge k's sign: dc - 1 - k for checks of
                                the non-generic steps, else k); missing variable
                                position + 1                                      */
-    const int32_t *c_steps; /* [nsc] check steps: D | G << 7 | start << 10 |
+    const int32_t *c_steps; /* [nsc] check steps: D | log2(G) << 7 | start << 10 |
                                generic << 22, over c_tasks                       */
     const int32_t *c_tasks; /* group | c6 << 8: rows c6*64 .. c6*64+63         */
     const int32_t *v_steps; /* [nsv] variable steps, same packing, over v_tasks,

@@ -120,8 +120,8 @@ def test_packed_tables_decode_to_the_base_graph(n, rate):
         tk = h[tasks].astype(np.int64)
         cover = np.zeros(len(tk), int)
         for w in st:
-            d, g, start, generic = w & 127, (w >> 7) & 7, (w >> 10) & 4095, (w >> 22) & 1
-            assert g in (1, 2, 4) and (not generic or g == 1)
+            d, g, start, generic = w & 127, 1 << ((w >> 7) & 7), (w >> 10) & 4095, (w >> 22) & 1
+            assert g in (1, 2, 4, 8) and (not generic or g == 1)
             cover[start:start + g] += 1
             grp = tk[start:start + g] & 255
             if not generic:
