@@ -91,8 +91,14 @@ struct ResTab {
     // FP16 / INT8 modes: at b * 6 + c6, the byte offset in LDS of the padded
     // posterior row position that check (g, c6*64) meets on base edge b
     const QC_CONST int *c_off;
+    // per variable step: offset into v_sdata (fast steps), where each of the
+    // step's G tasks has a head word t | c6 << 8 | n0 << 11 | row << 17 |
+    // global << 25 followed by its D entry words (edge_v word 0): the whole
+    // step in one scalar fetch after the step word
+    const QC_CONST int *v_soff;
+    const QC_CONST int *v_sdata;
 };
-static_assert(sizeof(ResTab) == 104, "resident code descriptor layout");
+static_assert(sizeof(ResTab) == 120, "resident code descriptor layout");
 
 struct ResArgs {
     const float *llr;
@@ -390,32 +396,29 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
 // entries the order does not matter: (0 + a) + b == (0 + b) + a in IEEE
 // arithmetic, signed zeros included.
 template <int M, int D, int G, bool TA>
-__device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &T,
-                                          const float *lam, const Rec &R, int start,
-                                          unsigned lane) {
+__device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
+                                          const float *lam, const Rec &R, unsigned lane) {
     unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G], pp4[G];
     bool wr[G][D], glob[G];
     float lv[G];
     bool valid[G];
-    int e0[G], n0[G];
+    int n0[G];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const int task = ct->v_tasks[start + gi];
-        const int t = task & 255;
-        const unsigned p = (unsigned)(task >> 8) * 64u + lane;
+        const unsigned hw = (unsigned)sd[gi * (D + 1)];
+        const int t = hw & 255;
+        const unsigned p = ((hw >> 8) & 7u) * 64u + lane;
         valid[gi] = p < (unsigned)ATSC3_Z;
         const unsigned pp = valid[gi] ? p : 0u;
         dst[gi] = pp;
         pp4[gi] = 4u * pp;
-        const unsigned vw = (unsigned)ct->grp_v[t];
-        e0[gi] = (vw >> 6) & 2047;
-        n0[gi] = TA ? (vw >> 17) & 63 : 0;
-        row[gi] = (vw >> 23) & 255u;
-        glob[gi] = vw >> 31;
+        n0[gi] = TA ? (hw >> 11) & 63 : 0;
+        row[gi] = (hw >> 17) & 255u;
+        glob[gi] = (hw >> 25) & 1u;
         lv[gi] = at(lam, (unsigned)t * ATSC3_Z + pp);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
+            const unsigned ew = (unsigned)sd[gi * (D + 1) + 1 + k];
             const unsigned u = ew & 511u, g = (ew >> 9) & 255u;
             if constexpr (M == 0) {
                 wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
@@ -439,7 +442,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST ResTab *ct, const Tot &
         float m[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const unsigned ew = (unsigned)ct->edge_v[2 * (e0[gi] + k)];
+            const unsigned ew = (unsigned)sd[gi * (D + 1) + 1 + k];
             const unsigned kk = (ew >> 17) & 127u, sp = ew >> 24;
             const bool a2 = kk == (w1[gi][k] >> RES_SIGN_LO);
             float v;
@@ -643,37 +646,38 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     continue;
                 }
                 const bool ta = M == 0 && ((sw >> 23) & 1);
+                const QC_CONST int *sd = ct->v_sdata + ct->v_soff[si];
                 switch ((ta ? 2048 : 0) + G * 128 + D) {
 #define RES_V1(DD)                                                      \
     case 128 + DD:                                                      \
-        var_multi<M, DD, 1, false>(ct, T, lam, R, start, lane);       \
+        var_multi<M, DD, 1, false>(sd, T, lam, R, lane);       \
         break;                                                          \
     case 2048 + 128 + DD:                                               \
-        var_multi<M, DD, 1, true>(ct, T, lam, R, start, lane);        \
+        var_multi<M, DD, 1, true>(sd, T, lam, R, lane);        \
         break;
 // pairs of degree 9..16 tasks: FP16 / INT8 only (FP32 keeps them single:
 // its two record words and ordered sums would not fit the register budget)
 #define RES_V2(DD)                                                      \
     case 256 + DD:                                                      \
         if constexpr (M != 0 || DD <= 8)                                \
-            var_multi<M, DD, 2, false>(ct, T, lam, R, start, lane);   \
+            var_multi<M, DD, 2, false>(sd, T, lam, R, lane);   \
         break;                                                          \
     case 2048 + 256 + DD:                                               \
         if constexpr (M == 0 && DD <= 8)                                \
-            var_multi<M, DD, 2, true>(ct, T, lam, R, start, lane);    \
+            var_multi<M, DD, 2, true>(sd, T, lam, R, lane);    \
         break;
 #define RES_V4(DD)                                                      \
     case 512 + DD:                                                      \
-        var_multi<M, DD, 4, false>(ct, T, lam, R, start, lane);       \
+        var_multi<M, DD, 4, false>(sd, T, lam, R, lane);       \
         break;                                                          \
     case 2048 + 512 + DD:                                               \
-        var_multi<M, DD, 4, true>(ct, T, lam, R, start, lane);        \
+        var_multi<M, DD, 4, true>(sd, T, lam, R, lane);        \
         break;
 // eight tasks of degree <= 3: FP16 / INT8 only
 #define RES_V8(DD)                                                      \
     case 1024 + DD:                                                     \
         if constexpr (M != 0)                                           \
-            var_multi<M, DD, 8, false>(ct, T, lam, R, start, lane);   \
+            var_multi<M, DD, 8, false>(sd, T, lam, R, lane);   \
         break;
                     QC_CASES16(RES_V1)
                     QC_CASES16(RES_V2)

@@ -32,7 +32,7 @@ MAX_DV = 32
 
 
 _TABLES = ("grp_c", "edge_c", "grp_v", "edge_v", "c_steps", "c_tasks", "v_steps", "v_tasks",
-           "c_off")
+           "c_off", "v_soff", "v_sdata")
 PROW = 424               # padded FP16 posterior row in LDS: 0..359, then 0..63 again
 
 
@@ -43,7 +43,7 @@ class _ResCode(ctypes.Structure):
                 + [(f, ctypes.c_void_p) for f in _TABLES])
 
 
-assert ctypes.sizeof(_ResCode) == 104
+assert ctypes.sizeof(_ResCode) == 120
 
 # tasks packed per wave step by degree: check steps of degree <= 16 take 2
 # tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
@@ -157,18 +157,34 @@ def _host_tables(code, f32=False):
     c_off = 2 * (row[t_of_b][:, None] * PROW + (c64[None, :] + (be & 511)[:, None]) % Z)
     gb, kb = g_of_b[vb], k_of_b[vb]
     sp = np.where(c_generic[gb], kb, dc[gb] - 1 - kb)
+    # each fast variable step flattened: per task a head word t | c6 << 8 |
+    # n0 << 11 | row << 17 | global << 25, then the D entry words
+    ev0 = vu | (gb << 9) | (kb << 17) | (sp << 24)
+    v_soff, v_sdata = [], []
+    for w in v_steps:
+        v_soff.append(len(v_sdata))
+        if (w >> 22) & 1:
+            continue
+        d, cnt, first = w & 127, 1 << ((w >> 7) & 7), (w >> 10) & 4095
+        for q in range(cnt):
+            task = int(v_tasks[first + q])
+            t, c6 = task & 255, task >> 8
+            v_sdata.append(t | (c6 << 8) | (int(n0[t]) << 11) | (int(row[t]) << 17)
+                           | (int(glob[t]) << 25))
+            v_sdata.extend(int(x) for x in ev0[vg[t]:vg[t] + d])
+    if not v_sdata:
+        v_sdata.append(0)
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int64).astype(np.uint32).view(np.int32))
     return dict(n=code.n, ncg=len(dc), nvg=nvg, nbe=nbe, max_dc=int(dc.max()),
                 global_rows=int(glob.sum()),
                 max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
                 c_steps=i32(c_steps), c_tasks=i32(c_tasks), v_steps=i32(v_steps),
-                c_off=i32(c_off.ravel()),
+                c_off=i32(c_off.ravel()), v_soff=i32(v_soff), v_sdata=i32(v_sdata),
                 v_tasks=i32(v_tasks),
                 grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
                 edge_c=i32(edge_c),
                 grp_v=i32(grp_v),
-                edge_v=i32(np.stack([vu | (g_of_b[vb] << 9) | (k_of_b[vb] << 17) | (sp << 24),
-                                     vmiss],
+                edge_v=i32(np.stack([ev0, vmiss],
                                     1).ravel()))
 
 

@@ -241,6 +241,12 @@ typedef struct atsc3_res_code {
                                in LDS of the posterior that check (g, c6*64)
                                meets on base edge b; rows of 424 halves
                                (positions 0..359, then 0..63 again)           */
+    const int32_t *v_soff;  /* [nsv] offset into v_sdata of each variable step
+                               (generic steps: unused)                         */
+    const int32_t *v_sdata; /* per non-generic variable step, for each of its G
+                               tasks: t | c6 << 8 | n0 << 11 | row << 17 |
+                               global << 25, then the group's D edge_v word-0
+                               entries                                          */
 } atsc3_res_code;
 
 int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,

@@ -129,6 +129,23 @@ def test_packed_tables_decode_to_the_base_graph(n, rate):
         assert (cover == 1).all()
         assert sorted((tk & 255) * 8 + (tk >> 8)) == sorted(
             t * 8 + c for t in range(ngrp) for c in range(6))
+    # flattened variable steps: each task's head word names its group, row
+    # block and first-region size, and its entries are the group's edge_v
+    # words in list order
+    sd, so = h["v_sdata"].astype(np.int64), h["v_soff"].astype(np.int64)
+    vt = h["v_tasks"].astype(np.int64)
+    for si, w in enumerate(h["v_steps"].astype(np.int64)):
+        if (w >> 22) & 1:
+            continue
+        d, g, start = w & 127, 1 << ((w >> 7) & 7), (w >> 10) & 4095
+        for q in range(g):
+            hw = sd[so[si] + q * (d + 1)]
+            t, c6 = vt[start + q] & 255, vt[start + q] >> 8
+            assert (hw & 255, (hw >> 8) & 7) == (t, c6)
+            assert (hw >> 11) & 63 == (gv[t] >> 17) & 63 and (hw >> 17) & 255 == (gv[t] >> 23) & 255
+            e0 = (gv[t] >> 6) & 2047
+            assert np.array_equal(sd[so[si] + q * (d + 1) + 1:so[si] + (q + 1) * (d + 1)],
+                                  ev[e0:e0 + d, 0])
 
 
 @pytest.mark.parametrize("n,rate", [(64800, "10/15"), (64800, "2/15"), (64800, "7/15"),
