@@ -97,8 +97,12 @@ struct ResTab {
     // step in one scalar fetch after the step word
     const QC_CONST int *v_soff;
     const QC_CONST int *v_sdata;
+    // the same for check steps: per task g | c6 << 8, then its D words
+    // (FP32: edge_c; FP16 / INT8: the c_off byte offsets for this c6)
+    const QC_CONST int *c_soff;
+    const QC_CONST int *c_sdata;
 };
-static_assert(sizeof(ResTab) == 120, "resident code descriptor layout");
+static_assert(sizeof(ResTab) == 136, "resident code descriptor layout");
 
 struct ResArgs {
     const float *llr;
@@ -237,16 +241,16 @@ __device__ __forceinline__ float rec_msg(unsigned w0, unsigned w0b, unsigned w1,
 // compressed records.  Every load of the G tasks (old records, posteriors)
 // is issued before the first use.
 template <int M, int D, int G>
-__device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const Tot &T,
-                                                const Rec &R, int start, unsigned lane,
-                                                bool first, bool update, float alpha) {
+__device__ __forceinline__ unsigned check_multi(const QC_CONST int *sd, const Tot &T,
+                                                const Rec &R, unsigned lane, bool first,
+                                                bool update, float alpha) {
     unsigned rix[G], w0[G], w0b[G], w1[G];
     bool valid[G];
     float tv[G][D];
     __half th[G][D];                // FP16 / INT8: the posteriors as loaded
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        const int task = ct->c_tasks[start + gi];
+        const int task = sd[gi * (D + 1)];
         const int g = task & 255;
         const unsigned j = (unsigned)(task >> 8) * 64u + lane;
         valid[gi] = j < (unsigned)ATSC3_Z;
@@ -260,16 +264,15 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST ResTab *ct, const
             if (M == 0) w0b[gi] = at(R.p0b, rix[gi]);
             w1[gi] = at(R.p1, rix[gi]);
         }
-        const int b0 = (ct->grp_c[g] >> 7) & 2047;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
+            // FP32: the edge_c word; FP16 / INT8: the c_off byte offset
+            const unsigned ew = (unsigned)sd[gi * (D + 1) + 1 + k];
             if constexpr (M == 0) {
-                const unsigned ew = (unsigned)ct->edge_c[b0 + k];
                 const unsigned p = wrap360(jj + (ew & 511u));
                 tv[gi][k] = tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p);
             } else {
-                const unsigned off = (unsigned)ct->c_off[(b0 + k) * 6 + (task >> 8)];
-                th[gi][k] = *reinterpret_cast<const __half *>(T.lds + off + 2u * lane);
+                th[gi][k] = *reinterpret_cast<const __half *>(T.lds + ew + 2u * lane);
             }
         }
     }
@@ -600,14 +603,15 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                                                 update, a.alpha);
                     continue;
                 }
+                const QC_CONST int *sd = ct->c_sdata + ct->c_soff[si];
                 switch (G * 128 + D) {
 #define RES_C1(DD)                                                                           \
     case 128 + DD:                                                                           \
-        unsat += check_multi<M, DD, 1>(ct, T, R, start, lane, first, update, a.alpha);     \
+        unsat += check_multi<M, DD, 1>(sd, T, R, lane, first, update, a.alpha);           \
         break;
 #define RES_C2(DD)                                                                           \
     case 256 + DD:                                                                           \
-        unsat += check_multi<M, DD, 2>(ct, T, R, start, lane, first, update, a.alpha);     \
+        unsat += check_multi<M, DD, 2>(sd, T, R, lane, first, update, a.alpha);           \
         break;
                     QC_CASES16(RES_C1)
                     QC_CASES16(RES_C2)

@@ -32,7 +32,7 @@ MAX_DV = 32
 
 
 _TABLES = ("grp_c", "edge_c", "grp_v", "edge_v", "c_steps", "c_tasks", "v_steps", "v_tasks",
-           "c_off", "v_soff", "v_sdata")
+           "c_off", "v_soff", "v_sdata", "c_soff", "c_sdata")
 PROW = 424               # padded FP16 posterior row in LDS: 0..359, then 0..63 again
 
 
@@ -43,7 +43,7 @@ class _ResCode(ctypes.Structure):
                 + [(f, ctypes.c_void_p) for f in _TABLES])
 
 
-assert ctypes.sizeof(_ResCode) == 120
+assert ctypes.sizeof(_ResCode) == 136
 
 # tasks packed per wave step by degree: check steps of degree <= 16 take 2
 # tasks; variable steps 4 (degree <= 4), 2 (<= 8) or 1 (<= 16)
@@ -174,12 +174,29 @@ def _host_tables(code, f32=False):
             v_sdata.extend(int(x) for x in ev0[vg[t]:vg[t] + d])
     if not v_sdata:
         v_sdata.append(0)
+    # check steps flattened the same way: per task g | c6 << 8, then the D
+    # words the check step reads (FP32: edge_c; FP16 / INT8: c_off for c6)
+    c_soff, c_sdata = [], []
+    for w in c_steps:
+        c_soff.append(len(c_sdata))
+        if (w >> 22) & 1:
+            continue
+        d, cnt, first = w & 127, 1 << ((w >> 7) & 7), (w >> 10) & 4095
+        for q in range(cnt):
+            task = int(c_tasks[first + q])
+            g, c6 = task & 255, task >> 8
+            c_sdata.append(task)
+            c_sdata.extend(int(x) for x in (edge_c[cg[g]:cg[g] + d] if f32
+                                            else c_off[cg[g]:cg[g] + d, c6]))
+    if not c_sdata:
+        c_sdata.append(0)
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int64).astype(np.uint32).view(np.int32))
     return dict(n=code.n, ncg=len(dc), nvg=nvg, nbe=nbe, max_dc=int(dc.max()),
                 global_rows=int(glob.sum()),
                 max_dv=int(dv.max()), nsc=len(c_steps), nsv=len(v_steps),
                 c_steps=i32(c_steps), c_tasks=i32(c_tasks), v_steps=i32(v_steps),
                 c_off=i32(c_off.ravel()), v_soff=i32(v_soff), v_sdata=i32(v_sdata),
+                c_soff=i32(c_soff), c_sdata=i32(c_sdata),
                 v_tasks=i32(v_tasks),
                 grp_c=i32(dc | (cg[:-1] << 7) | (hasmiss << 18)),
                 edge_c=i32(edge_c),

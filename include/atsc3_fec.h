@@ -247,6 +247,11 @@ typedef struct atsc3_res_code {
                                tasks: t | c6 << 8 | n0 << 11 | row << 17 |
                                global << 25, then the group's D edge_v word-0
                                entries                                          */
+    const int32_t *c_soff;  /* [nsc] offset into c_sdata of each check step    */
+    const int32_t *c_sdata; /* per non-generic check step, for each of its G
+                               tasks: g | c6 << 8, then D words: msg_fmt 0 the
+                               group's edge_c words, else its c_off offsets for
+                               c6                                               */
 } atsc3_res_code;
 
 int64_t atsc3_minsum_resident_ws_bytes(int32_t grid, int32_t max_checks,

@@ -146,6 +146,20 @@ def test_packed_tables_decode_to_the_base_graph(n, rate):
             e0 = (gv[t] >> 6) & 2047
             assert np.array_equal(sd[so[si] + q * (d + 1) + 1:so[si] + (q + 1) * (d + 1)],
                                   ev[e0:e0 + d, 0])
+    # flattened check steps: task word, then the group's c_off offsets for
+    # the task's row block (FP16 / INT8 tables)
+    sd, so = h["c_sdata"].astype(np.int64), h["c_soff"].astype(np.int64)
+    ct_, off = h["c_tasks"].astype(np.int64), h["c_off"].astype(np.int64).reshape(-1, 6)
+    for si, w in enumerate(h["c_steps"].astype(np.int64)):
+        if (w >> 22) & 1:
+            continue
+        d, g, start = w & 127, 1 << ((w >> 7) & 7), (w >> 10) & 4095
+        for q in range(g):
+            task = ct_[start + q]
+            grp, c6 = task & 255, task >> 8
+            base = so[si] + q * (d + 1)
+            assert sd[base] == task
+            assert np.array_equal(sd[base + 1:base + 1 + d], off[cg[grp]:cg[grp] + d, c6])
 
 
 @pytest.mark.parametrize("n,rate", [(64800, "10/15"), (64800, "2/15"), (64800, "7/15"),
