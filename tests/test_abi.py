@@ -1,6 +1,7 @@
 """The C-ABI library: it loads without a GPU, exports every function that
 include/atsc3_fec.h declares, the ctypes layer binds exactly those, and the
 pure size queries answer (no compute call is made)."""
+import ctypes
 import os
 import re
 
@@ -104,3 +105,35 @@ def test_size_queries():
     assert L.atsc3_minsum_batch_ctl_words(-1, 10) < 0
     assert L.atsc3_soft_demap_bwd_grid(3, 256) == 3
     assert L.atsc3_minsum_batch_status_word() >= 0
+
+
+def test_resident_entry_rejects_bad_arguments():
+    """atsc3_minsum_resident_dev validates its arguments before any device
+    call: negative sizes, a negative or NaN alpha (records hold magnitudes),
+    an unknown message format, NULL buffers, sizes beyond the kernel's
+    capacities and a short workspace are refused with the header's codes; an
+    empty batch is a no-op that needs no buffers."""
+    L = _lib().lib()
+    E_DEGREE, E_ARG, E_SIZE, OK = -1, -2, -4, 0
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    ws_need = L.atsc3_minsum_resident_ws_bytes(4, 21600, 0, 1)
+    assert ws_need > 4 * 21600 * 8
+
+    def call(nblk=1, iters=50, alpha=0.75, fmt=1, ptr=p, max_n=64800, max_dc=14, max_dv=13,
+             ws_bytes=None):
+        return L.atsc3_minsum_resident_dev(ptr, nblk, ptr, ptr, ptr, 1, 21600, 0, max_n,
+                                           max_dc, max_dv, iters, alpha, fmt, None, 4, ptr,
+                                           ws_need if ws_bytes is None else ws_bytes,
+                                           ptr, ptr, ptr, ptr, None)
+    assert call(nblk=-1) == E_ARG
+    assert call(iters=-1) == E_ARG
+    assert call(alpha=-0.5) == E_ARG
+    assert call(alpha=float("nan")) == E_ARG
+    assert call(fmt=3) == E_ARG
+    assert call(nblk=0, ptr=None) == OK
+    assert call(ptr=None) == E_ARG
+    assert call(max_n=64801) == E_SIZE
+    assert call(max_dc=90) == E_DEGREE
+    assert call(max_dv=33) == E_DEGREE
+    assert call(ws_bytes=ws_need - 1) == E_SIZE

@@ -474,3 +474,30 @@ def test_mixed_pipeline_engines_agree_fp32(device):
         del g, p
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float16", "float32", "int8"])
+def test_edge_cases_small_batches(device, msg):
+    """An empty batch returns empty outputs; one codeword on a full grid
+    (255 workgroups find the queue empty at once); iters = 0 (syndrome of
+    the LLRs only: bits are their hard decisions) -- each equal to the
+    oracle in the same storage mode."""
+    import torch
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.resident import decode_resident_dev, default_grid
+    code = get_code("8/15", 16200)
+    out = decode_resident_dev(torch.empty((0, code.n), device=device), code, 50, 0.75,
+                              msg_dtype=msg)
+    assert all(o.shape[0] == 0 for o in out)
+    code, llr = _synth_llr("8/15", 16200, 2.0, nblk=1, seed=5)
+    for iters in (50, 0):
+        print(f"[gpu] resident edge {msg} 1 block iters {iters} grid {default_grid()}",
+              flush=True)
+        want = resident_minsum(llr, code, iters, 0.75, msg)
+        got = decode_resident_dev(torch.from_numpy(llr).to(device), code, iters, 0.75,
+                                  grid=default_grid(), msg_dtype=msg)
+        torch.cuda.synchronize()
+        _check([o.cpu().numpy() for o in got], want, f"{msg} iters {iters}")
+        if iters == 0:
+            assert np.array_equal(want[0], (llr < 0).astype(np.uint8))
