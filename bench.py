@@ -930,7 +930,7 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        del z, w, llr
+        del z, w, llr_s, out_s
         torch.cuda.empty_cache()
         mixed = mixed_bench(args, dev)
     learn = None
