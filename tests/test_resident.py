@@ -501,3 +501,32 @@ def test_edge_cases_small_batches(device, msg):
         _check([o.cpu().numpy() for o in got], want, f"{msg} iters {iters}")
         if iters == 0:
             assert np.array_equal(want[0], (llr < 0).astype(np.uint8))
+
+
+@pytest.mark.gpu
+def test_two_stream_pipeline_equals_one_stream(device):
+    """The receiver pipeline of the bench headline: LLRs produced on one
+    stream, decoded on another behind an event (one resident plan per
+    stream), three windows in flight -- every window's outputs equal the
+    one-stream decode of the same LLRs."""
+    import torch
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr = _synth_llr("8/15", 16200, 2.0, nblk=24, seed=9)
+    src = torch.from_numpy(llr).to(device)
+    want = [o.cpu().numpy() for o in decode_resident_dev(src, code, 50, 0.75)]
+    torch.cuda.synchronize()
+    s_prod, s_dec = torch.cuda.Stream(device), torch.cuda.Stream(device)
+    outs = []
+    print("[gpu] resident two-stream pipeline, 3 windows of 24 blocks", flush=True)
+    for _ in range(3):
+        with torch.cuda.stream(s_prod):
+            win = src * 1.0                     # the window's LLRs, made on s_prod
+            done = torch.cuda.Event()
+            done.record(s_prod)
+        with torch.cuda.stream(s_dec):
+            s_dec.wait_event(done)
+            win.record_stream(s_dec)
+            outs.append(decode_resident_dev(win, code, 50, 0.75))
+    torch.cuda.synchronize()
+    for k, out in enumerate(outs):
+        _check([o.cpu().numpy() for o in out], want, f"window {k}")
