@@ -15,6 +15,8 @@ Sources (paths relative to the upstream tree):
   lab/spec_ldpc64k.py:21    LDPC_64800   (Annex A.1 rows + parameters)
   lab/spec_bitint.py:57-240 RATES, MOD_BITS, TYPE_A, TYPE_B, DEMUX, BI_TYPE,
                             GROUPWISE, GROUPWISE_HDR
+  lab/spec_bitint.py:380-391 GROUPWISE_LABELS (the printed rate label of
+                            each Annex B block, for the table gate H8)
   lab/spec_bank/nuc_a322.npz  Annex C.1.2-C.1.7 position vectors ``w``
 
 The fingerprints the upstream tree records for the LDPC payload
@@ -60,7 +62,7 @@ def main(root):
                     ["LDPC_64800", "A1_INT_COUNT", "A1_INT_SUM"])
     bi = _literals(os.path.join(lab, "spec_bitint.py"),
                    ["RATES", "MOD_BITS", "TYPE_A", "TYPE_B", "DEMUX",
-                    "BI_TYPE", "GROUPWISE", "GROUPWISE_HDR"])
+                    "BI_TYPE", "GROUPWISE", "GROUPWISE_HDR", "GROUPWISE_LABELS"])
 
     # -- LDPC fingerprints, exactly as the upstream records define them ------
     flat16 = [x for r in sl["LDPC_16200"] for row in sl["LDPC_16200"][r]["rows"]
@@ -107,7 +109,9 @@ def main(root):
         demux=[[n, m, *v] for (n, m), v in bi["DEMUX"].items()],
         bi_type=[[n, nb, r, t] for (n, nb, r), t in bi["BI_TYPE"].items()],
         groupwise=[[n, m, r, list(p)] for (n, m, r), p in bi["GROUPWISE"].items()],
-        groupwise_hdr=[[n, m, list(h)] for (n, m), h in bi["GROUPWISE_HDR"].items()])
+        groupwise_hdr=[[n, m, list(h)] for (n, m), h in bi["GROUPWISE_HDR"].items()],
+        groupwise_labels=[[n, m, [list(x) for x in labs]]
+                          for (n, m), labs in bi["GROUPWISE_LABELS"].items()])
     with open(os.path.join(HERE, "bitint_tables.json"), "w") as f:
         json.dump(bitint, f, separators=(",", ":"))
 

@@ -10,9 +10,10 @@ input cells in ``shared_memory``, work items on a Queue, results written back
 into shared memory, the parent re-orders by item index.  Each worker runs
 the CPU demap -- the float32 broadcast form of lab/m44_ldm.py:2431-2494
 (``demap_f32_cpu``) -- then the bit de-interleave ``lam[:, lam_of_q] = q``,
-then the FP32 min-sum C kernel: the upstream kernel built from its own
-source (``_ref/ldpc_kernel.so``) where present, else this repository's C
-restatement of it.
+then the FP32 min-sum C kernel: this repository's C restatement of the
+upstream kernel (``liboracle_minsum.so``, equal to the upstream kernel built
+from its own source on every golden fixture), so no binary built from the
+upstream tree is needed where the baseline runs.
 """
 from __future__ import annotations
 
@@ -67,6 +68,7 @@ def _worker(shm_names, shapes, cfg, in_q, out_q):
         sys.path.insert(0, cfg["root"])
         from atsc3_amd import bicm
         from atsc3_amd.codes import get_code
+        from oracle import bicm as OB
         from oracle import ldpc as OL
         shms = [shared_memory.SharedMemory(name=n) for n in shm_names]
         cells = np.ndarray(shapes["cells"], np.complex64, buffer=shms[0].buf)
@@ -75,7 +77,7 @@ def _worker(shm_names, shapes, cfg, in_q, out_q):
         meta = np.ndarray(shapes["meta"], np.int32, buffer=shms[3].buf)
         code = get_code(cfg["rate"], cfg["n"])
         pts = bicm.points(cfg["mod"], cfg["rate"])
-        lq = bicm.bicm_map(cfg["n"], cfg["mod"], cfg["rate"])
+        lq = OB.lam_of_q(cfg["n"], cfg["mod"], cfg["rate"])
         which = cfg["which"]
 
         def run(lo, hi):
@@ -113,7 +115,7 @@ def run_pool(cells, csi, sigma2, rate, n, mod, iters, alpha, procs, chunk=4,
     from multiprocessing import shared_memory
     from . import ldpc as OL
     if which is None:
-        which = "ref" if OL.available("ref") else "oracle"
+        which = "oracle"
     root = root or os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     cells = np.ascontiguousarray(cells, np.complex64)
     csi = np.ascontiguousarray(csi, np.float32)

@@ -43,6 +43,20 @@ def load(name):
     return code, llr, meta["iters"], meta["alpha"], want
 
 
+def ref_lam_of_q(n, mod, rate):
+    """The upstream's de-interleave map lam_of_q for (n, mod, rate): stored
+    whole in tests/golden/ref_bicm.npz at the demap tests' operating points,
+    else the label-simulating restatement (oracle/bicm.py) after checking
+    its sha256 against the upstream's (same fixture file)."""
+    from oracle import bicm as OB
+    ref = np.load(os.path.join(GOLDEN, "ref_bicm.npz"), allow_pickle=False)
+    key = f"map_{n}_{mod}_{rate.replace('/', '_')}"
+    sha = dict(zip(map(str, ref["keys"]), map(str, ref["sha256"])))[f"{n}|{mod}|{rate}"]
+    m = ref[key].astype(np.int64) if key in ref.files else OB.lam_of_q(n, mod, rate)
+    assert OB.map_sha256(m) == sha, f"lam_of_q {n} {mod} {rate} is not the upstream's"
+    return m
+
+
 def regenerate(spec):
     from atsc3_amd import bicm, synth
     if spec["gen"] == "bpsk":
@@ -51,7 +65,7 @@ def regenerate(spec):
     from oracle import demap as OD
     n, rate, mod = spec["n"], spec["rate"], spec["mod"]
     _, y, n0 = synth.awgn_blocks(rate, n, mod, spec["nblk"], spec["snr"], spec["seed"])
-    lam, _ = OD.demap(y, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate), sigma2=n0)
+    lam, _ = OD.demap(y, bicm.points(mod, rate), ref_lam_of_q(n, mod, rate), sigma2=n0)
     return lam.astype(np.float32)
 
 

@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 from atsc3_amd import bicm, synth                      # noqa: E402
 from atsc3_amd.codes import get_code                   # noqa: E402
+from oracle import bicm as OB                          # noqa: E402
 from oracle import demap as OD                         # noqa: E402
 from oracle import ldpc as OL                          # noqa: E402
 
@@ -45,7 +46,7 @@ def seeded_llr(spec):
         n, rate, mod = spec["n"], spec["rate"], spec["mod"]
         _, y, n0 = synth.awgn_blocks(rate, n, mod, spec["nblk"], spec["snr"],
                                      spec["seed"])
-        lam, _ = OD.demap(y, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate),
+        lam, _ = OD.demap(y, bicm.points(mod, rate), OB.lam_of_q(n, mod, rate),
                           sigma2=n0)
         return lam.astype(np.float32)
     raise ValueError(kind)

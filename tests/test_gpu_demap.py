@@ -6,6 +6,7 @@ import pytest
 
 from oracle import demap as OD
 from atsc3_amd import bicm, synth
+from tests._fixtures import ref_lam_of_q
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -24,7 +25,7 @@ def test_demap_vs_fp64_oracle(device, n, mod, rate, esn0, given_sigma):
     print(f"[gpu] demap {n} {mod} {rate} sigma_given={given_sigma}", flush=True)
     cw, y, n0 = synth.awgn_blocks(rate, n, mod, 3, esn0, 31)
     y = y.astype(np.complex64).astype(np.complex128)      # the kernel's input precision
-    want, s2w = OD.demap(y, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate),
+    want, s2w = OD.demap(y, bicm.points(mod, rate), ref_lam_of_q(n, mod, rate),
                          sigma2=n0 if given_sigma else None)
     dm = Demapper(n, mod, rate, device)
     z = torch.from_numpy(y.astype(np.complex64)).to(device)
@@ -44,7 +45,7 @@ def test_demap_with_csi_rayleigh(device):
     z = z.astype(np.complex64).astype(np.complex128)
     w = w.astype(np.float32).astype(np.float64)
     for s2 in (n0, None):
-        want, s2w = OD.demap(z, bicm.points(mod, rate), bicm.bicm_map(n, mod, rate), csi=w, sigma2=s2)
+        want, s2w = OD.demap(z, bicm.points(mod, rate), ref_lam_of_q(n, mod, rate), csi=w, sigma2=s2)
         dm = Demapper(n, mod, rate, device)
         llr, s2g = dm.demap_dev(torch.from_numpy(z.astype(np.complex64)).to(device),
                                 csi=torch.from_numpy(w.astype(np.float32)).to(device), sigma2=s2)

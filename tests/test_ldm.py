@@ -24,7 +24,7 @@ from oracle import ldpc as OP
 from atsc3_amd import bicm, synth
 from atsc3_amd.codes import get_code
 
-from tests._fixtures import ensure_oracle_built
+from tests._fixtures import ensure_oracle_built, ref_lam_of_q
 
 CORE = (64800, "QPSK", "4/15")
 ENH = (64800, "64QAM", "10/15")
@@ -35,7 +35,7 @@ ESN0_DEC = 20.0
 
 def _lq(cfg):
     n, mod, rate = cfg
-    return bicm.bicm_map(n, mod, rate)
+    return ref_lam_of_q(n, mod, rate)
 
 
 def _superpose(ncore, seed, n0):
