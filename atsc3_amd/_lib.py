@@ -91,12 +91,19 @@ _SIGS = {
                                   c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32, c_vp,
                                   ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
+    "atsc3_demap_raw_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
+                            c_vp],
+    "atsc3_minsum_utc_ws_bytes": [c_i32, c_i32, c_i32, c_i32],
+    "atsc3_minsum_utc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
+                             c_i32, c_i32, c_f64, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp, c_vp,
+                             c_vp],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
 }
 
 # functions whose C return type is not int32_t
-_RESTYPES = {"atsc3_minsum_resident_ws_bytes": ctypes.c_int64}
+_RESTYPES = {"atsc3_minsum_resident_ws_bytes": ctypes.c_int64,
+             "atsc3_minsum_utc_ws_bytes": ctypes.c_int64}
 
 
 def restype_of(name):
@@ -139,7 +146,8 @@ SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ld
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
-                "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows")
+                "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows",
+                "atsc3_minsum_utc_ws_bytes")
 
 
 def call(name, *args):

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libatsc3fec.so")
 SOURCES = ["ldpc_qc.hip", "ldpc_qc_phase.hip", "ldpc_batch.hip", "ldpc_batch_f32.hip",
            "ldpc_batch_f16.hip", "ldpc_batch_i8.hip", "ldpc_csr.hip", "demap.hip", "ldm.hip",
-           "unrolled.hip", "baseband.hip", "sum_product.hip", "cpe.hip", "ldpc_resident.hip"]
+           "unrolled.hip", "baseband.hip", "sum_product.hip", "cpe.hip", "ldpc_resident.hip", "ldpc_utc.hip"]
 ARCH = os.environ.get("ATSC3_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -267,6 +267,27 @@ extern "C" int32_t atsc3_demap_dev(const float *cells, const float *csi,
                                                 atsc3::as_stream(stream));
 }
 
+extern "C" int32_t atsc3_demap_raw_dev(const float *cells, const float *csi, int32_t nblk,
+                                       int32_t ncell, const float *points, int32_t nbits,
+                                       const int32_t *lam_of_q, float *llr_out, float *d2_out,
+                                       double *ws, void *stream) {
+    if (nblk < 0 || ncell < 0) return ATSC3_E_ARG;
+    if (nblk == 0 || ncell == 0) return ATSC3_OK;
+    if (!cells || !points || !lam_of_q || !llr_out || !d2_out || !ws) return ATSC3_E_ARG;
+    const atsc3::Gather g{nullptr, 0, 0, 0, 0};
+    hipStream_t st = atsc3::as_stream(stream);
+    // sigma2 NULL: the kernel writes the unscaled differences, d2min per cell
+    // and its FP64 partial sums (unused here); no scale pass follows.
+    switch (nbits) {
+        case 2: atsc3::launch_demap<2, atsc3::GATHER_NONE>(cells, csi, nullptr, nblk, ncell, points, lam_of_q, llr_out, ws, g, st, d2_out); break;
+        case 4: atsc3::launch_demap<4, atsc3::GATHER_NONE>(cells, csi, nullptr, nblk, ncell, points, lam_of_q, llr_out, ws, g, st, d2_out); break;
+        case 6: atsc3::launch_demap<6, atsc3::GATHER_NONE>(cells, csi, nullptr, nblk, ncell, points, lam_of_q, llr_out, ws, g, st, d2_out); break;
+        case 8: atsc3::launch_demap<8, atsc3::GATHER_NONE>(cells, csi, nullptr, nblk, ncell, points, lam_of_q, llr_out, ws, g, st, d2_out); break;
+        default: return ATSC3_E_ARG;
+    }
+    return atsc3::launch_status();
+}
+
 extern "C" int32_t atsc3_demap_gather_dev(const float *pool, const float *pool_csi,
                                           const atsc3_gather *gather, const double *sigma2,
                                           int32_t nblk, int32_t ncell, const float *points,

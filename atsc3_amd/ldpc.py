@@ -1,23 +1,39 @@
 """GPU min-sum decoding behind the upstream call signatures.
 
+Two decode contracts, as the upstream has them:
+
+* the compiled kernel's (lab/ldpc_kernel.c:82-143): syndrome checked before
+  each update (a clean block reports its = 0), tot = llr + (0 + m_1 + ...).
+  Served by ``min_sum_decode_batch(..., dtype=float32)`` -- lab/m3_ldpc.py:509
+  routes float32 to that kernel -- bit-exact on every field (bits, conv, its,
+  bad), and by ``decode_dev`` / ``ladder_dev`` with the FP16 / INT8 message
+  forms beside it.
+* the update-then-check one of the torch decoder ``GpuMinSum.decode``
+  (lab/m9_gpu.py:138-211) and of the float64 decoders ``min_sum_decode``
+  (lab/m3_ldpc.py:245-286) and ``min_sum_decode_batch(dtype=float64)``:
+  messages updated first (its >= 1), tot = ((llr + m_1) + m_2) + ..., rows
+  padded with 1e9 entries.  Served by ``decode_utc_dev`` in FP32 or FP64,
+  bit-exact with those functions (``GpuMinSum(dtype="float32")``,
+  ``GpuMinSum(dtype="float64")`` -- the upstream default -- and the float64
+  functions).
+
 Mirrors (same arguments, same return tuples):
+  * ``min_sum_decode(llr, checks, n, iters=50, alpha=0.75)``
   * ``min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype)``
-    -- lab/m3_ldpc.py:509-630.  float32 is the path the upstream serves with
-    its compiled kernel (lab/ldpc_kernel.c); here it runs on the GPU and is
-    bit-exact with that kernel on every field (bits, conv, its, bad).
   * ``GpuMinSum(checks, n, iters, alpha, device, dtype).decode(llr2d, iters,
-    alpha)`` -- lab/m9_gpu.py:84-220; accepts a numpy array or a device
-    tensor and returns numpy ``(bits u8, conv bool, its i32, bad i32)``.
+    alpha)``: a numpy array or a device tensor in, numpy
+    ``(bits u8, conv bool, its i32, bad i32)`` out.
   * ``ladder_decode`` -- the alpha ladder of lab/m9_fast.py:843-891 /
     lab/m6_bicm.py:341-355 (rungs 1.0, 0.85, 0.75 on still-failing rows; a
     rung's result is taken when it converged or has fewer unsatisfied checks).
 
 ``checks`` may be an ``LdpcCode`` or the upstream list-of-lists; a list is
 matched to its A/322 code (N, rate from N - len(checks)) and verified equal,
-so the QC kernel can serve it; an unrecognised matrix runs on the CSR kernel.
+so the QC kernels can serve it; an unrecognised matrix runs on the CSR kernels.
 
-Message storage (``msg_dtype``): "float32" (plain, bit-exact), "float16",
-"int8" (per-block scale).  Variable sums accumulate in FP32 in every mode.
+Message storage (``msg_dtype``, kernel contract): "float32" (plain,
+bit-exact), "float16", "int8" (per-block scale).  Variable sums accumulate
+in FP32 in every mode.
 """
 from __future__ import annotations
 
@@ -102,6 +118,7 @@ class DeviceTables:
         self.nc = len(cptr) - 1
         self.nedges = len(cidx)
         self.max_check_degree = int(np.diff(cptr).max(initial=0))
+        self.dmax = max(1, self.max_check_degree)     # row padding of the upstream
         try:
             q = code.kernel_tables()
         except (ValueError, AttributeError):
@@ -232,53 +249,140 @@ def _to_numpy(bits, conv, its, bad):
     return (bits.cpu().numpy(), conv.cpu().numpy().astype(bool), its_h, bad.cpu().numpy())
 
 
-def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float32,
+UTC_DTYPES = {"float32": 0, "float64": 1}
+
+
+def decode_utc_dev(llr, code, iters=50, alpha=0.75):
+    """Update-then-check decode of device LLRs (nblk, n), float32 or float64
+    (the arithmetic follows the tensor's dtype); no host sync.
+
+    Contract of lab/m9_gpu.py:138-211 (float32) and lab/m3_ldpc.py:245-286
+    (float64): its >= 1 for iters >= 1, tot = ((llr + m_1) + m_2) + ... in
+    ascending check order, rows padded to the largest check degree with
+    1e9 entries; iters == 0 gives bits = llr < 0, conv 0, its 0, bad -1.
+    Returns device tensors (bits u8 (nblk, n), conv u8, its i32, bad i32).
+    """
+    if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
+        raise TypeError("decode_utc_dev needs a CUDA (HIP) tensor")
+    if llr.dtype not in (torch.float32, torch.float64):
+        raise TypeError("update-then-check decoding runs in float32 or float64")
+    if iters < 0:
+        raise ValueError("iters must be >= 0")
+    llr = llr.contiguous()
+    dev = llr.device
+    tabs = device_tables(code, dev)
+    nblk, n = llr.shape
+    if n != tabs.n:
+        raise ValueError(f"LLR width {n} != code length {tabs.n}")
+    dt = 1 if llr.dtype == torch.float64 else 0
+    bits = torch.empty((nblk, n), dtype=torch.uint8, device=dev)
+    conv = torch.empty(nblk, dtype=torch.uint8, device=dev)
+    its = torch.empty(nblk, dtype=torch.int32, device=dev)
+    bad = torch.empty(nblk, dtype=torch.int32, device=dev)
+    for lo in range(0, nblk, 65535):
+        hi = min(nblk, lo + 65535)
+        nb = hi - lo
+        wsb = int(_lib.lib().atsc3_minsum_utc_ws_bytes(nb, n, tabs.nedges, dt))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        _lib.call("atsc3_minsum_utc_dev", llr[lo:hi].data_ptr(), dt, nb, n, tabs.nc,
+                  tabs.nedges, tabs.cptr.data_ptr(), tabs.cidx.data_ptr(),
+                  tabs.vptr.data_ptr(), tabs.vedge.data_ptr(), tabs.dmax, int(iters),
+                  float(alpha), ws.data_ptr(), wsb, bits[lo:hi].data_ptr(),
+                  conv[lo:hi].data_ptr(), its[lo:hi].data_ptr(), bad[lo:hi].data_ptr(),
+                  _stream_ptr(dev))
+    return bits, conv, its, bad
+
+
+def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float64,
                          msg_dtype="float32", device="cuda", engine="auto"):
     """lab/m3_ldpc.py:509: (nblk, n) LLRs -> (bits, conv, its, bad) numpy.
 
-    dtype must be float32 (the upstream's compiled-kernel path); the float64
-    update-then-check variant is not served on the GPU.
+    dtype float32: the compiled-kernel contract (lab/ldpc_kernel.c:65-146,
+    the upstream's float32 route), bit-exact, on the fast engines (msg_dtype
+    selects the FP16 / INT8 message forms).  dtype float64 (the upstream's
+    default): the update-then-check float64 decoder, row-exact with
+    lab/m3_ldpc.py:245-286, run in FP64 on the GPU.
     """
-    if np.dtype(dtype) != np.float32:
-        raise NotImplementedError("GPU min-sum serves the float32 (compiled-kernel) semantics")
+    dt = np.dtype(dtype)
+    if dt not in (np.float32, np.float64):
+        raise ValueError("dtype must be float32 or float64")
     code = resolve_code(checks, n)
     lam = np.asarray(lam2d) if not isinstance(lam2d, torch.Tensor) else lam2d
     nblk = int(lam.shape[0])
     if nblk == 0:
         return (np.zeros((0, n), np.uint8), np.zeros(0, bool),
                 np.full(0, iters, np.int32), np.full(0, -1, np.int32))
-    t = torch.as_tensor(np.ascontiguousarray(lam, np.float32)) if not isinstance(lam, torch.Tensor) else lam
-    t = t.to(device=device, dtype=torch.float32)
+    tdt = torch.float32 if dt == np.float32 else torch.float64
+    if isinstance(lam, torch.Tensor):
+        t = lam.to(device=device, dtype=tdt)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(lam, dt)).to(device)
+    if dt == np.float64:
+        return _to_numpy(*decode_utc_dev(t, code, iters, alpha))
     return _to_numpy(*decode_dev(t, code, iters, alpha, msg_dtype, engine))
 
 
+def min_sum_decode(llr, checks, n, iters=50, alpha=0.75, device="cuda"):
+    """lab/m3_ldpc.py:245: one block, float64 update-then-check ->
+    (bits u8 (n,), converged bool, n_iter int, n_unsatisfied int)."""
+    b, c, i, d = min_sum_decode_batch(np.asarray(llr, np.float64).reshape(1, -1), checks, n,
+                                      iters, alpha, dtype=np.float64, device=device)
+    return b[0], bool(c[0]), int(i[0]), int(d[0])
+
+
 class GpuMinSum:
-    """lab/m9_gpu.py:84: batched min-sum for ONE code, built once, reused."""
+    """lab/m9_gpu.py:84: batched min-sum for ONE code, built once, reused.
+
+    dtype "float32" / "float64" (the upstream default): the upstream's
+    update-then-check contract in that precision, bit-exact with
+    GpuMinSum.decode (lab/m9_gpu.py:138-211).  contract="kernel" instead
+    serves the compiled kernel's contract (lab/ldpc_kernel.c:82-143) on the
+    fast engines, where msg_dtype selects the FP16 / INT8 message forms.
+    """
 
     def __init__(self, checks, n=None, iters=50, alpha=0.75, device="cuda",
-                 dtype="float32", msg_dtype="float32", engine="auto"):
-        if dtype not in ("float32",):
-            raise NotImplementedError("GPU min-sum serves float32 semantics")
+                 dtype="float64", contract="upstream", msg_dtype="float32", engine="auto"):
+        if contract not in ("upstream", "kernel"):
+            raise ValueError("contract is 'upstream' or 'kernel'")
+        if contract == "upstream" and dtype not in UTC_DTYPES:
+            raise NotImplementedError(
+                f"dtype {dtype!r}: the update-then-check decoder runs in float32 or "
+                "float64 (reduced-precision messages: contract='kernel', msg_dtype)")
+        if contract == "kernel" and dtype != "float32":
+            raise ValueError("the kernel contract computes in float32")
         self.code = resolve_code(checks, n if n is not None else getattr(checks, "n", None))
         self.n = self.code.n
         self.iters, self.alpha = iters, alpha
         self.device = torch.device(device)
+        self.dtype_name, self.contract = dtype, contract
+        self.dt = torch.float64 if dtype == "float64" else torch.float32
         self.msg_dtype, self.engine = msg_dtype, engine
         self.tables = device_tables(self.code, self.device)
         self.stats = dict(calls=0, iters=0, blocks=0)
 
     def decode(self, llr2d, iters=None, alpha=None):
-        iters = self.iters if iters is None else iters
+        """(nblk, n) LLRs (ndarray or device tensor) -> numpy
+        (bits u8 (nblk, n), converged bool, iters i32, unsatisfied i32).
+        iters None or 0 means the constructor's value (lab/m9_gpu.py:145)."""
+        iters = iters or self.iters
+        if iters < 1:
+            raise ValueError("iters must be >= 1")
         alpha = self.alpha if alpha is None else alpha
         if isinstance(llr2d, torch.Tensor):
-            t = llr2d.to(self.device, dtype=torch.float32)
+            t = llr2d.to(self.device, dtype=self.dt)
         else:
-            t = torch.from_numpy(np.ascontiguousarray(llr2d, np.float32)).to(self.device)
-        assert t.shape[1] == self.n
-        out = _to_numpy(*decode_dev(t, self.code, iters, alpha, self.msg_dtype, self.engine))
+            a = np.ascontiguousarray(np.asarray(llr2d, np.float64))
+            t = torch.from_numpy(a).to(self.device, dtype=self.dt)
+        if t.dim() != 2 or t.shape[1] != self.n:
+            raise ValueError(f"LLRs must be (nblk, {self.n})")
+        if self.contract == "upstream":
+            out = _to_numpy(*decode_utc_dev(t, self.code, iters, alpha))
+        else:
+            out = _to_numpy(*decode_dev(t, self.code, iters, alpha, self.msg_dtype,
+                                        self.engine))
         self.stats["calls"] += 1
         self.stats["blocks"] += int(t.shape[0])
-        self.stats["iters"] += int(out[2].sum())
+        self.stats["iters"] += int(out[2].max(initial=0))
         return out
 
 

@@ -61,6 +61,35 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
                                  int32_t *its, int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Update-then-check min-sum in FP32 (dtype 0) or FP64 (dtype 1), the
+ * contract of lab/m9_gpu.py:138-211 GpuMinSum.decode (float32 / float64) and
+ * lab/m3_ldpc.py:245-286 min_sum_decode, lab/m3_ldpc.py:509-630
+ * min_sum_decode_batch(dtype=float64): messages updated first, the syndrome
+ * of tot < 0 checked after each update (its >= 1 when iters >= 1);
+ * tot[v] = ((llr[v] + m_e1) + m_e2) + ... in ascending check order; each
+ * check's two-minimum also sees (dmax - degree) pad entries of magnitude
+ * 1e9 (the upstream's rows padded to the largest degree dmax); alpha is
+ * rounded to T.  iters == 0: bits = llr < 0, conv 0, its 0, bad -1.
+ * Finite LLRs whose sums stay finite (the upstream's own implementations
+ * treat NaN differently from one another).
+ *   llr      [nblk][n] float (dtype 0) or double (dtype 1), device
+ *   cptr..vedge  CSR tables as ldpc_minsum_f32 (any degrees)
+ *   dmax     the largest check degree of the code (the row padding width)
+ *   ws       atsc3_minsum_utc_ws_bytes(nblk, n, nedges, dtype) bytes
+ * 2 * (iters + 1) + 1 launches on `stream`; nblk <= 65535.
+ * ------------------------------------------------------------------------ */
+int64_t atsc3_minsum_utc_ws_bytes(int32_t nblk, int32_t n, int32_t nedges,
+                                  int32_t dtype);
+int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t nblk,
+                             int32_t n, int32_t nc, int32_t nedges,
+                             const int32_t *cptr, const int32_t *cidx,
+                             const int32_t *vptr, const int32_t *vedge,
+                             int32_t dmax, int32_t iters, double alpha,
+                             void *ws, int64_t ws_bytes, uint8_t *bits,
+                             uint8_t *conv, int32_t *its, int32_t *bad,
+                             void *stream);
+
+/* ------------------------------------------------------------------------
  * Device-pointer min-sum over the 360-lifted QC base graph.
  * The fast path behind the upstream min-sum calls (lab/m3_ldpc.py:509-630
  * min_sum_decode_batch at float32 -> lab/ldpc_kernel.c:65; lab/m9_gpu.py:138
@@ -294,6 +323,17 @@ int32_t atsc3_demap_dev(const float *cells, const float *csi,
                         const float *points, int32_t nbits,
                         const int32_t *lam_of_q, float *llr_out,
                         double *sigma2_out, double *ws, void *stream);
+
+/* Raw max-log differences and nearest-point distances, no noise scaling:
+ * replaces lab/m9_fast.py:756-822 FrameDecoder.demap_batch called with
+ * d2min_out (sigma^2 switched off).  llr_out as atsc3_demap_dev with
+ * sigma2 = 1 (w * (min_{y_i=1} - min_{y_i=0})), d2_out [nblk*ncell] f32 =
+ * min_p |z - p|^2 per cell; ws as atsc3_demap_dev with sigma2 NULL.      */
+int32_t atsc3_demap_raw_dev(const float *cells, const float *csi,
+                            int32_t nblk, int32_t ncell, const float *points,
+                            int32_t nbits, const int32_t *lam_of_q,
+                            float *llr_out, float *d2_out, double *ws,
+                            void *stream);
 
 /* ------------------------------------------------------------------------
  * Demap with the time de-interleaver's FEC-block gather fused into the read.

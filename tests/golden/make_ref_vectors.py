@@ -59,6 +59,28 @@ UTC_F64 = [("f64_16k_r6_qpsk", 16200, "6/15", "QPSK", 3, 0.6, 71, 50, 0.75),
            ("f64_64k_r10_nuc", 64800, "10/15", "256QAM", 2, 17.8, 72, 50, 0.85)]
 
 
+def tiny_sets():
+    """Small hand-made graphs whose LLRs make huge messages of opposite sign
+    cancel on one variable, so the association of its sum decides its bit:
+    llr-first ((l + m1) + m2) and zero-first (l + (m1 + m2)) differ.  The
+    second is random with LLR magnitudes 2^-20 .. 2^70."""
+    out = []
+    for prec, e in (("f64", 55), ("f32", 26)):
+        # every check of degree 3: no row padding enters the two-minimum
+        checks = [[0, 1, 4], [0, 2, 5], [1, 2, 3]]
+        big = 2.0 ** e
+        llr = np.array([[-1.0, -big, big, 3.0, 2 * big, 2 * big],
+                        [1.0, big, -big, -3.0, 2 * big, 2 * big]])
+        out.append((f"tiny_cancel_{prec}", checks, 6, llr, 1, 0.75))
+    rng = np.random.default_rng(77)
+    n, nc = 48, 30
+    checks = [sorted(rng.choice(n, rng.integers(3, 7), replace=False).tolist())
+              for _ in range(nc)]
+    llr = np.sign(rng.standard_normal((4, n))) * 2.0 ** rng.uniform(-20, 70, (4, n))
+    out.append(("tiny_wide", checks, n, llr, 12, 0.75))
+    return out
+
+
 def f64_llr(n, rate, mod, nblk, esn0, seed):
     """FP64 max-log LLRs of seeded AWGN cells, codeword order (not rounded
     to float32)."""
@@ -105,8 +127,8 @@ def main(upstream):
     print(f"ref_tables.json: {len(tables)} codes")
 
     # -- update-then-check decoders -----------------------------------------
-    def run(name, n, rate, llr, iters, alpha, meta):
-        checks = checks_of.get((n, rate)) or LD.parity_check(rate, n)
+    def run(name, n, rate, llr, iters, alpha, meta, checks=None):
+        checks = checks or checks_of.get((n, rate)) or LD.parity_check(rate, n)
         g = UG.GpuMinSum(checks, n, iters=iters, alpha=alpha, device="cpu",
                          dtype="float32")
         b32, c32, i32, d32 = g.decode(np.asarray(llr, np.float64))
@@ -140,6 +162,12 @@ def main(upstream):
         run(f"{name}_x{factor:.0e}".replace("+", ""), code.n, code.rate, llr, iters, alpha,
             dict(source=name, scale=factor, n=code.n, rate=code.rate, iters=iters,
                  alpha=alpha))
+    for name, checks, n, llr, iters, alpha in tiny_sets():
+        cn = [len(c) for c in checks]
+        run(name, n, None, llr, iters, alpha,
+            dict(tiny=dict(cptr=np.concatenate([[0], np.cumsum(cn)]).tolist(),
+                           cidx=[v for c in checks for v in c], llr=llr.tolist()),
+                 n=n, rate=None, iters=iters, alpha=alpha), checks=checks)
     for (name, n, rate, mod, nblk, esn0, seed, iters, alpha) in UTC_F64:
         llr = f64_llr(n, rate, mod, nblk, esn0, seed)
         sha = hashlib.sha256(llr.tobytes()).hexdigest()

@@ -67,8 +67,10 @@ def test_empty_batch(device):
     code = get_code("10/15", 64800)
     out = ldpc.decode_dev(torch.empty((0, 64800), device=device), code)
     assert all(x.shape[0] == 0 for x in out)
-    b, c, i, d = ldpc.min_sum_decode_batch(np.zeros((0, 64800), np.float32), code.checks, 64800)
-    assert b.shape == (0, 64800) and c.shape == (0,)
+    for dt in (np.float32, np.float64):
+        b, c, i, d = ldpc.min_sum_decode_batch(np.zeros((0, 64800), dt), code.checks, 64800,
+                                               dtype=dt)
+        assert b.shape == (0, 64800) and c.shape == (0,)
 
 
 def test_determinism_and_large_batch_invariants(device):
@@ -109,8 +111,18 @@ def test_reference_signatures(device):
     assert bits.dtype == np.uint8 and conv.dtype == bool and its.dtype == np.int32
     want = OL.minsum_f32(llr, code, 50, 0.75)
     assert np.array_equal(bits, want[0]) and np.array_equal(its, want[2])
-    g = ldpc.GpuMinSum(checks, 16200, iters=50, alpha=0.75, device=device)
+    # GpuMinSum: the upstream's update-then-check contract (float32 here) ...
+    from oracle.utc_ref import utc_minsum
+    g = ldpc.GpuMinSum(checks, 16200, iters=50, alpha=0.75, device=device, dtype="float32")
     out = g.decode(llr)
+    want_u = utc_minsum(llr, checks, 16200, 50, 0.75, np.float32)
+    for x, y in zip(out, want_u):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    assert out[1].dtype == bool and out[2].dtype == np.int32
+    # ... or, asked for, the compiled kernel's
+    k = ldpc.GpuMinSum(checks, 16200, iters=50, alpha=0.75, device=device, dtype="float32",
+                       contract="kernel")
+    out = k.decode(llr)
     assert np.array_equal(out[0], want[0]) and np.array_equal(out[3], want[3])
 
 
