@@ -16,6 +16,7 @@ HIP graph (``capture``) and replayed per frame.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 
 import numpy as np
@@ -218,7 +219,7 @@ def decode_mixed(llrs, codes, iters=50, alpha=0.75, msg_dtype="float32", slots=N
     return bits, plan.conv, plan.its, plan.bad
 
 
-_SINGLE = {}
+_SINGLE = collections.OrderedDict()
 
 
 def _ladder_arg(ladder):
@@ -238,16 +239,21 @@ def decode_batch_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", msg_s
     ``ladder`` (1..4 alphas) those of ldpc.ladder_dev."""
     dev = llr.device
     nblk, n = llr.shape
-    key = (id(code), nblk, str(dev))
+    # the stream is part of the key: the cached tables were allocated on the
+    # stream that built them, so another stream gets its own copy and no
+    # entry is ever freed while a launch on a different stream reads it
+    key = (id(code), nblk, str(dev), torch.cuda.current_stream(dev).cuda_stream)
     hit = _SINGLE.get(key)
     if hit is None or hit[0] is not code:
         bank = CodeBank([code], dev)
         blk_code = torch.zeros(nblk, dtype=torch.int32, device=dev)
         blk_off = torch.arange(nblk, dtype=torch.int64, device=dev) * n
         hit = (code, bank, blk_code, blk_off)
-        if len(_SINGLE) > 64:
-            _SINGLE.clear()
+        while len(_SINGLE) >= 64:
+            _SINGLE.popitem(last=False)          # evict the oldest entry only
         _SINGLE[key] = hit
+    else:
+        _SINGLE.move_to_end(key)
     _, bank, blk_code, blk_off = hit
     fmt = MSG_FORMATS[msg_dtype]
     bits = torch.empty((nblk, n), dtype=torch.uint8, device=dev)

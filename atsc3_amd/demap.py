@@ -3,9 +3,14 @@
 Mirrors:
   * ``demap_llr(cells, mod, rate, sigma2=None)`` -- lab/m6_bicm.py:213-225:
     one block of cells -> q-ordered LLRs (numpy).
-  * ``demap_batch(cells2d, pts, sigma2=None)`` -- lab/m9_fast.py:756,
-    lab/m44_ldm.py:2454 (demap_batch_qam), lab/m9_gpu.py:301 (GpuBicm.demap):
-    (nblk, ncell) -> (nblk, ncell*nb) q-ordered LLRs, sigma2 per block.
+  * ``demap_batch(cells2d, pts, ones, chunk=8, d2min_out=None)`` --
+    lab/m9_fast.py:756 FrameDecoder.demap_batch: (nblk, ncell) -> (nblk,
+    ncell*nb) q-ordered LLRs, sigma2 estimated per block; with d2min_out the
+    unscaled differences and each cell's nearest-point distance.
+  * ``demap_batch_qam(cells2d, ch, ex=None, threads=4, chunk=4)`` --
+    lab/m44_ldm.py:2454, the same for a chain object's alphabet.
+  * ``demap_q_batch_dev(cells2d, pts, sigma2, csi)`` -- lab/m9_gpu.py:301
+    (GpuBicm.demap): device tensors, sigma2 given or estimated per block.
   * ``Demapper(n, mod, rate).demap_dev(cells, csi, sigma2)`` -- the fused
     form: demap plus the caller's de-interleave ``lam[lam_of_q] = q``
     (lab/m6_bicm.py:331-335), device tensors in and out, codeword order.
@@ -188,21 +193,85 @@ class Demapper:
                                   self.ncell, csi, sigma2, out)
 
 
-def demap_batch(cells2d, pts, sigma2=None, csi=None, device="cuda"):
-    """(nblk, ncell) -> ((nblk, ncell*nb) q-order float32 numpy, sigma2)."""
+def _alphabet_dev(pts, dev):
     pts = np.asarray(pts)
     nb = _check_quadrant_rule(pts)
-    dev = torch.device(device)
     p = torch.from_numpy(np.stack([pts.real, pts.imag], 1).astype(np.float32)).to(dev)
-    z = torch.from_numpy(np.ascontiguousarray(cells2d, np.complex64)).to(dev)
+    return p, nb
+
+
+def demap_q_batch_dev(cells2d, pts, sigma2=None, csi=None, device="cuda"):
+    """(nblk, ncell) cells -> ((nblk, ncell*nb) q-order float32 device tensor,
+    sigma2 (nblk,) float64 device tensor); sigma2 None -> per-block estimate."""
+    dev = torch.device(device)
+    p, nb = _alphabet_dev(pts, dev)
+    if isinstance(cells2d, torch.Tensor):
+        z = cells2d.to(dev, torch.complex64)
+    else:
+        z = torch.from_numpy(np.ascontiguousarray(cells2d, np.complex64)).to(dev)
     z = z.reshape(-1, z.shape[-1])
     ident = _identity_dev(z.shape[1] * nb, dev)
-    llr, s2 = demap_q_dev(z, p, nb, ident, csi, sigma2)
-    return llr.cpu().numpy(), s2.cpu().numpy()
+    return demap_q_dev(z, p, nb, ident, csi, sigma2)
+
+
+def demap_batch(cells2d, pts, ones, chunk=8, d2min_out=None, *, device="cuda"):
+    """lab/m9_fast.py:756 FrameDecoder.demap_batch: (nblk, ncell) cells ->
+    (nblk, ncell*nb) float32 max-log LLRs in q order, sigma^2 the mean of
+    each row's nearest-point distances (floored at 1e-9).
+
+    ones = the label bit masks (bit_masks(nb), y0 the MSB), as the upstream
+    passes them; chunk is the upstream's CPU work split and has no effect
+    here.  d2min_out: an (nblk, ncell) float32 array (or device tensor) that
+    receives each cell's squared distance to the nearest point, and the
+    LLRs come back unscaled (sigma^2 off), for per-cell noise weights
+    (lab/m9_fast.py:767-771).  A device tensor in gives a device tensor out.
+    """
+    pts = np.asarray(pts)
+    ones = np.asarray(ones)
+    nb = _check_quadrant_rule(pts)
+    if ones.shape != (nb, 1 << nb) or not np.array_equal(ones, bicm.bit_masks(nb)):
+        raise ValueError("ones must be the MSB-first label bit masks of the alphabet")
+    on_dev = isinstance(cells2d, torch.Tensor)
+    dev = cells2d.device if on_dev else torch.device(device)
+    if d2min_out is None:
+        llr, _ = demap_q_batch_dev(cells2d, pts, device=dev)
+        return llr if on_dev else llr.cpu().numpy()
+    p, _ = _alphabet_dev(pts, dev)
+    if on_dev:
+        z = cells2d.to(dev, torch.complex64).contiguous()
+    else:
+        z = torch.from_numpy(np.ascontiguousarray(cells2d, np.complex64)).to(dev)
+    B, C = z.shape
+    if tuple(d2min_out.shape) != (B, C):
+        raise ValueError(f"d2min_out must be ({B}, {C})")
+    llr = torch.empty((B, C * nb), dtype=torch.float32, device=dev)
+    d2 = torch.empty((B, C), dtype=torch.float32, device=dev)
+    if B and C:
+        ws = torch.empty(max(1, _lib.lib().atsc3_demap_ws_doubles(B, C)), dtype=torch.float64,
+                         device=dev)
+        _lib.call("atsc3_demap_raw_dev", torch.view_as_real(z).data_ptr(), None, B, C,
+                  p.data_ptr(), nb, _identity_dev(C * nb, dev).data_ptr(), llr.data_ptr(),
+                  d2.data_ptr(), ws.data_ptr(), _stream(dev))
+    if isinstance(d2min_out, torch.Tensor):
+        d2min_out.copy_(d2)
+    else:
+        d2min_out[...] = d2.cpu().numpy()
+    return llr if on_dev else llr.cpu().numpy()
+
+
+def demap_batch_qam(cells2d, ch, ex=None, threads=4, chunk=4, *, device="cuda"):
+    """lab/m44_ldm.py:2454: (nblk, ncell) cells of the chain ``ch`` (any
+    object with .mod, .rate, .mod_bits) -> (nblk, ncell*nb) float32 max-log
+    LLRs in q order, per-row sigma^2 estimate.  ex / threads / chunk are the
+    upstream's CPU thread pool and work split; unused here."""
+    pts = bicm.points(ch.mod, ch.rate)
+    if int(ch.mod_bits) != bicm.MOD_BITS[ch.mod]:
+        raise ValueError("ch.mod_bits does not match ch.mod")
+    return demap_batch(cells2d, pts, bicm.bit_masks(int(ch.mod_bits)), device=device)
 
 
 def demap_llr(cells, mod, rate, sigma2=None, device="cuda"):
     """lab/m6_bicm.py:213: one block -> q-ordered LLRs (numpy float32)."""
-    llr, _ = demap_batch(np.asarray(cells)[None, :], bicm.points(mod, rate), sigma2,
-                         device=device)
-    return llr[0]
+    llr, _ = demap_q_batch_dev(np.asarray(cells)[None, :], bicm.points(mod, rate), sigma2,
+                               device=device)
+    return llr[0].cpu().numpy()

@@ -157,6 +157,13 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     or "auto" (the QC engines when the code has a QC form, else csr: the
     fused engine below PHASE_MIN_BLOCKS blocks, above it the resident
     engine for FP32 and the batched engine for FP16/INT8).
+
+    Failure on the device: the batched engine leaves its = -1 (bad = -1,
+    conv 0) for every codeword it did not finish (a bounded grid-barrier
+    wait ran out); this function does not read the outputs back, so a
+    caller that stays on the device checks ``(its < 0).any()`` itself (the
+    host conversions -- min_sum_decode_batch, GpuMinSum.decode,
+    ladder_decode -- raise on it).
     """
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_dev needs a CUDA (HIP) tensor")

@@ -56,16 +56,40 @@ def test_demap_with_csi_rayleigh(device):
 
 
 def test_demap_reference_signatures(device):
+    """The upstream's demap calls, made exactly as the upstream makes them:
+    lab/m9_fast.py:1117 ``q = self.demap_batch(cells, self.pts0, self.ones0)``,
+    lab/m9_fast.py:1105 with ``d2min_out`` (sigma^2 off, nearest distances
+    out), lab/m44_ldm.py:2359 ``demap_batch_qam(cells, ch, self.ex,
+    self.fec_threads)``, lab/m6_bicm.py:213 ``demap_llr``."""
     from atsc3_amd import demap as D
     n, mod, rate = 16200, "64QAM", "11/15"
-    _, y, n0 = synth.awgn_blocks(rate, n, mod, 2, 16.0, 8)
+    _, y, n0 = synth.awgn_blocks(rate, n, mod, 3, 16.0, 8)
     y = y.astype(np.complex64).astype(np.complex128)
-    q = D.demap_llr(y[0], mod, rate)
-    want, _ = OD.demap_q(y[:1], bicm.points(mod, rate))
-    assert OD.normwise_rel_err(q[None].astype(np.float64), want)[0] <= TOL
-    qb, s2 = D.demap_batch(y, bicm.points(mod, rate), sigma2=n0)
-    want, _ = OD.demap_q(y, bicm.points(mod, rate), sigma2=n0)
-    assert (OD.normwise_rel_err(qb.astype(np.float64), want) <= TOL).all()
+    pts0, ones0 = bicm.points(mod, rate), bicm.bit_masks(6)
+    want, _ = OD.demap_q(y, pts0)
+    q = D.demap_batch(y, pts0, ones0)
+    assert q.shape == (3, n) and q.dtype == np.float32
+    assert (OD.normwise_rel_err(q.astype(np.float64), want) <= TOL).all()
+    d2 = np.empty((3, n // 6), np.float32)
+    raw = D.demap_batch(y, pts0, ones0, d2min_out=d2)
+    want_raw, _ = OD.demap_q(y, pts0, sigma2=np.ones(3))
+    assert (OD.normwise_rel_err(raw.astype(np.float64), want_raw) <= TOL).all()
+    d2w = (np.abs(y[:, :, None] - pts0[None, None, :]) ** 2).min(axis=2)
+    assert np.allclose(d2, d2w, rtol=1e-5, atol=1e-6)
+
+    class Chain:                       # what the upstream's PlpChain exposes
+        def __init__(self, mod, rate):
+            self.mod, self.rate, self.mod_bits = mod, rate, bicm.MOD_BITS[mod]
+    for m2, r2, nn in (("QPSK", "4/15", 64800), ("64QAM", "10/15", 64800)):
+        _, y2, _ = synth.awgn_blocks(r2, nn, m2, 2, 12.0, 9)
+        y2 = y2.astype(np.complex64).astype(np.complex128)
+        q2 = D.demap_batch_qam(y2, Chain(m2, r2), None, 4)
+        w2, _ = OD.demap_q(y2, bicm.points(m2, r2))
+        assert (OD.normwise_rel_err(q2.astype(np.float64), w2) <= TOL).all()
+    ql = D.demap_llr(y[0], mod, rate)
+    assert OD.normwise_rel_err(ql[None].astype(np.float64), want[:1])[0] <= TOL
+    with pytest.raises(ValueError):
+        D.demap_batch(y, pts0, ~ones0)
 
 
 def test_demap_noiseless_decodes(device):
