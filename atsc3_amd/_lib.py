@@ -90,6 +90,10 @@ _SIGS = {
     "atsc3_minsum_resident_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
                                   c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32, c_vp,
                                   ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_resident_noalloc_max_bytes": [],
+    "atsc3_minsum_resident_noalloc_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
+                                          c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp,
+                                          c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
     "atsc3_demap_raw_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                             c_vp],
@@ -103,7 +107,8 @@ _SIGS = {
 
 # functions whose C return type is not int32_t
 _RESTYPES = {"atsc3_minsum_resident_ws_bytes": ctypes.c_int64,
-             "atsc3_minsum_utc_ws_bytes": ctypes.c_int64}
+             "atsc3_minsum_utc_ws_bytes": ctypes.c_int64,
+             "atsc3_minsum_resident_noalloc_max_bytes": ctypes.c_int64}
 
 
 def restype_of(name):
@@ -147,7 +152,7 @@ SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ld
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
                 "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows",
-                "atsc3_minsum_utc_ws_bytes")
+                "atsc3_minsum_utc_ws_bytes", "atsc3_minsum_resident_noalloc_max_bytes")
 
 
 def call(name, *args):

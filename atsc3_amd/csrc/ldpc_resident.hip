@@ -56,6 +56,8 @@
 // mode reproduces the reference kernel's golden outputs bit for bit.
 #include <hip/hip_fp16.h>
 
+#include <mutex>
+
 #include "ldpc_qc_common.h"
 
 #define RES_WG 1024
@@ -740,31 +742,26 @@ extern "C" int32_t atsc3_minsum_resident_grid(void) {
     return cus > 0 ? cus : ATSC3_E_HIP;
 }
 
-extern "C" int32_t atsc3_minsum_resident_dev(
-    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
-    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
-    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
-    const float *msg_scale, int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits,
-    uint8_t *conv, int32_t *its,
-    int32_t *bad, void *stream) {
-    if (nblk < 0 || iters < 0 || iters >= (1 << 24) || !(alpha >= 0.0f)) return ATSC3_E_ARG;
-    if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
-    if (nblk == 0) return ATSC3_OK;
-    if (!llr || !blk_off || !blk_code || !codes || !ws || !bits || !conv || !its || !bad)
-        return ATSC3_E_ARG;
-    if (ncode < 1 || max_checks < 1 || grid < 1 || max_global_rows < 0) return ATSC3_E_ARG;
-    if (max_n < 1 || max_n > RES_NMAX) return ATSC3_E_SIZE;
-    if (max_dc < 1 || max_dc > RES_MAX_DC || max_dv < 1 || max_dv > RES_MAX_DV)
-        return ATSC3_E_DEGREE;
+namespace atsc3 {
+namespace {
+// Validates the arguments and enqueues the queue reset and the decode with
+// the workspace `ws` of `ws_bytes` bytes.
+int32_t resident_launch(const float *llr, int32_t nblk, const int64_t *blk_off,
+                        const int32_t *blk_code, const void *codes, int32_t ncode,
+                        int32_t max_checks, int32_t max_global_rows, int32_t max_n,
+                        int32_t max_dc, int32_t max_dv, int32_t iters, float alpha,
+                        int32_t msg_fmt, const float *msg_scale, int32_t grid, void *ws,
+                        int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
+                        int32_t *bad, hipStream_t st) {
     const bool f32 = msg_fmt == 0;
-    const atsc3::WsLayout L(grid, max_checks, max_global_rows, f32);
+    const WsLayout L(grid, max_checks, max_global_rows, f32);
     if (ws_bytes < L.bytes) return ATSC3_E_SIZE;
     char *base = static_cast<char *>(ws);
-    atsc3::ResArgs a{};
+    ResArgs a{};
     a.llr = llr;
     a.blk_off = (const QC_CONST long long *)blk_off;
     a.blk_code = (const QC_CONST int *)blk_code;
-    a.codes = (const QC_CONST atsc3::ResTab *)codes;
+    a.codes = (const QC_CONST ResTab *)codes;
     a.nblk = nblk;
     a.iters = iters;
     a.alpha = alpha;
@@ -780,15 +777,114 @@ extern "C" int32_t atsc3_minsum_resident_dev(
     a.conv = conv;
     a.its = its;
     a.bad = bad;
-    hipStream_t st = atsc3::as_stream(stream);
-    hipLaunchKernelGGL(atsc3::resident_init, dim3(1), dim3(64), 0, st, a.queue);
+    hipLaunchKernelGGL(resident_init, dim3(1), dim3(64), 0, st, a.queue);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
     a.msg_scale = msg_fmt == 2 ? msg_scale : nullptr;
     if (f32)
-        hipLaunchKernelGGL(atsc3::resident_kernel<0>, dim3(grid), dim3(RES_WG), 0, st, a);
+        hipLaunchKernelGGL(resident_kernel<0>, dim3(grid), dim3(RES_WG), 0, st, a);
     else if (msg_fmt == 1)
-        hipLaunchKernelGGL(atsc3::resident_kernel<1>, dim3(grid), dim3(RES_WG), 0, st, a);
+        hipLaunchKernelGGL(resident_kernel<1>, dim3(grid), dim3(RES_WG), 0, st, a);
     else
-        hipLaunchKernelGGL(atsc3::resident_kernel<2>, dim3(grid), dim3(RES_WG), 0, st, a);
-    return atsc3::launch_status();
+        hipLaunchKernelGGL(resident_kernel<2>, dim3(grid), dim3(RES_WG), 0, st, a);
+    return launch_status();
+}
+
+int32_t resident_check_args(const float *llr, int32_t nblk, const int64_t *blk_off,
+                            const int32_t *blk_code, const void *codes, int32_t ncode,
+                            int32_t max_checks, int32_t max_global_rows, int32_t max_n,
+                            int32_t max_dc, int32_t max_dv, int32_t iters, float alpha,
+                            int32_t msg_fmt, int32_t grid, const uint8_t *bits,
+                            const uint8_t *conv, const int32_t *its, const int32_t *bad) {
+    if (nblk < 0 || iters < 0 || iters >= (1 << 24) || !(alpha >= 0.0f)) return ATSC3_E_ARG;
+    if (msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
+    if (nblk == 0) return ATSC3_OK;
+    if (!llr || !blk_off || !blk_code || !codes || !bits || !conv || !its || !bad)
+        return ATSC3_E_ARG;
+    if (ncode < 1 || max_checks < 1 || grid < 1 || max_global_rows < 0) return ATSC3_E_ARG;
+    if (max_n < 1 || max_n > RES_NMAX) return ATSC3_E_SIZE;
+    if (max_dc < 1 || max_dc > RES_MAX_DC || max_dv < 1 || max_dv > RES_MAX_DV)
+        return ATSC3_E_DEGREE;
+    return ATSC3_OK;
+}
+}  // namespace
+}  // namespace atsc3
+
+extern "C" int32_t atsc3_minsum_resident_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
+    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
+    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits,
+    uint8_t *conv, int32_t *its,
+    int32_t *bad, void *stream) {
+    const int32_t rc = atsc3::resident_check_args(llr, nblk, blk_off, blk_code, codes, ncode,
+                                                  max_checks, max_global_rows, max_n, max_dc,
+                                                  max_dv, iters, alpha, msg_fmt, grid, bits,
+                                                  conv, its, bad);
+    if (rc != ATSC3_OK || nblk == 0) return rc;
+    if (!ws) return ATSC3_E_ARG;
+    return atsc3::resident_launch(llr, nblk, blk_off, blk_code, codes, ncode, max_checks,
+                                  max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
+                                  msg_scale, grid, ws, ws_bytes, bits, conv, its, bad,
+                                  atsc3::as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// The same decode with no allocation at all: the workspace is `scratch`, a
+// file-scope device array of RES_SCRATCH_BYTES that every GPU the library
+// runs on holds until the process ends.  In host code the array's name is
+// not a device address, so its address is taken once per device with
+// hipGetSymbolAddress (with that device current) and only that address is
+// handed to the kernels.  All calls on one GPU share the array: calls that
+// overlap on one GPU race (enqueue them on one stream).  The kernels need
+// no zeroed memory beyond the queue word, which a kernel resets each call.
+// ---------------------------------------------------------------------------
+#define RES_SCRATCH_BYTES (128ll << 20)
+#define RES_SCRATCH_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[RES_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[RES_SCRATCH_DEVICES];
+
+int32_t scratch_of_current_device(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return ATSC3_E_HIP;
+    if (dev < 0 || dev >= RES_SCRATCH_DEVICES) return ATSC3_E_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return ATSC3_E_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ATSC3_OK;
+}
+}  // namespace
+
+extern "C" int64_t atsc3_minsum_resident_noalloc_max_bytes(void) { return RES_SCRATCH_BYTES; }
+
+extern "C" int32_t atsc3_minsum_resident_noalloc_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
+    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
+    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, int32_t grid, uint8_t *bits, uint8_t *conv, int32_t *its,
+    int32_t *bad, void *stream) {
+    const int32_t rc = atsc3::resident_check_args(llr, nblk, blk_off, blk_code, codes, ncode,
+                                                  max_checks, max_global_rows, max_n, max_dc,
+                                                  max_dv, iters, alpha, msg_fmt, grid, bits,
+                                                  conv, its, bad);
+    if (rc != ATSC3_OK || nblk == 0) return rc;
+    // the workspace this launch needs must fit the scratch array
+    if (atsc3::WsLayout(grid, max_checks, max_global_rows, msg_fmt == 0).bytes >
+        RES_SCRATCH_BYTES)
+        return ATSC3_E_SIZE;
+    void *ws = nullptr;
+    const int32_t rs = scratch_of_current_device(&ws);
+    if (rs != ATSC3_OK) return rs;
+    return atsc3::resident_launch(llr, nblk, blk_off, blk_code, codes, ncode, max_checks,
+                                  max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
+                                  msg_scale, grid, ws, RES_SCRATCH_BYTES, bits, conv, its, bad,
+                                  atsc3::as_stream(stream));
 }

@@ -299,6 +299,28 @@ int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
                                   int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Resident min-sum with no allocation.  No counterpart in the upstream: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as atsc3_minsum_resident_dev without ws/ws_bytes:
+ * the workspace is a fixed device array inside the library of
+ * atsc3_minsum_resident_noalloc_max_bytes() bytes (128 MiB) on each GPU the
+ * library runs on, held until the process ends.  Returns ATSC3_E_SIZE
+ * (nothing enqueued) when atsc3_minsum_resident_ws_bytes(grid, max_checks,
+ * max_global_rows, msg_fmt) exceeds it -- lower `grid` to fit.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU race, so
+ * enqueue them on one stream.
+ * ------------------------------------------------------------------------ */
+int64_t atsc3_minsum_resident_noalloc_max_bytes(void);
+int32_t atsc3_minsum_resident_noalloc_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off,
+    const int32_t *blk_code, const void *codes, int32_t ncode,
+    int32_t max_checks, int32_t max_global_rows, int32_t max_n, int32_t max_dc,
+    int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, int32_t grid, uint8_t *bits, uint8_t *conv,
+    int32_t *its, int32_t *bad, void *stream);
+
+/* ------------------------------------------------------------------------
  * Max-log NUC demapper with fused bit de-interleave.
  * Replaces the upstream demap calls lab/m6_bicm.py:213-225 (demap_llr),
  * lab/m9_fast.py:756 (FrameDecoder.demap_batch), lab/m44_ldm.py:2454
