@@ -31,4 +31,14 @@ __device__ inline int wg_sum_i32(int x, int *slots) {
     return t;
 }
 
+// The compiled kernel's FP32 contract (lab/ldpc_kernel.c:82-143) as phase
+// kernels over (codeword, check) and (codeword, variable) threads on CSR
+// tables (ldpc_utc.hip); workspace csr_phase_ws_bytes(nblk, n, nedges)
+// bytes; nblk <= 65535.
+size_t csr_phase_ws_bytes(int nblk, int n, int nedges);
+int32_t csr_phase_kc_f32(const float *llr, int nblk, int n, int nc, int nedges, const int *cptr,
+                         const int *cidx, const int *vptr, const int *vedge, int iters,
+                         float alpha, void *ws, uint8_t *bits, uint8_t *conv, int *its,
+                         int *bad, hipStream_t st);
+
 }  // namespace atsc3

@@ -119,17 +119,145 @@ extern "C" int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int3
 
 // ---------------------------------------------------------------------------
 // Drop-in: the upstream C ABI (lab/ldpc_kernel.c:52,65-71), host pointers.
+//
+// The upstream drives this call from a ThreadPoolExecutor: many concurrent
+// calls of a few blocks each, same tables every time (lab/m3_ldpc.py:485-506,
+// lab/m9_fast.py:893-925).  So a call
+//   * takes an idle context of the current device from a pool -- its own
+//     stream, a grow-only device workspace and grow-only pinned staging --
+//     and gives it back at the end: concurrent callers run on separate
+//     streams, and once the buffers have grown no call allocates or frees;
+//   * finds the code's tables in a per-device cache keyed by a 64-bit hash
+//     of (n, nc, nedges, cptr, cidx, vptr, vedge) and uploads them only the
+//     first time (up to 64 codes per device; beyond that they travel with
+//     the call);
+//   * decodes on the CSR phase kernels (ldpc_utc.hip, the compiled kernel's
+//     contract), which spread the call's checks and variables over the whole
+//     GPU rather than one workgroup per codeword.
+// Outputs are staged and copied to the caller's arrays only after the whole
+// decode succeeded, so on any error nothing is written.
 // ---------------------------------------------------------------------------
 extern "C" int32_t ldpc_kernel_abi(void) { return 1; }
 
+#include <cstring>
+#include <mutex>
+#include <vector>
+
 namespace {
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 1); }
+
+struct DropCtx {
+    int device = -1;
+    bool busy = false;
+    hipStream_t st = nullptr;
+    char *dev = nullptr;
+    size_t dev_cap = 0;
+    char *host = nullptr;
+    size_t host_cap = 0;
 };
+
+struct DropTab {
+    int device, n, nc, ne;
+    uint64_t hash;
+    int32_t *d;   // cptr [nc+1] | cidx [ne] | vptr [n+1] | vedge [ne]
+};
+
+std::mutex g_drop_mu;
+std::vector<DropCtx *> g_drop_ctx;   // never freed: reused for the process lifetime
+std::vector<DropTab> g_drop_tab;
+constexpr int kMaxTablesPerDevice = 64;
+
+uint64_t mix_words(const int32_t *p, size_t n, uint64_t h) {
+    for (size_t i = 0; i < n; ++i) {
+        h ^= (uint32_t)p[i];
+        h *= 0x100000001b3ull;
+        h ^= h >> 29;
+    }
+    return h;
+}
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+DropCtx *acquire_ctx(int device) {
+    std::lock_guard<std::mutex> lock(g_drop_mu);
+    for (DropCtx *c : g_drop_ctx)
+        if (!c->busy && c->device == device) {
+            c->busy = true;
+            return c;
+        }
+    DropCtx *c = new DropCtx;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    c->device = device;
+    c->busy = true;
+    g_drop_ctx.push_back(c);
+    return c;
+}
+
+void release_ctx(DropCtx *c) {
+    std::lock_guard<std::mutex> lock(g_drop_mu);
+    c->busy = false;
+}
+
+bool grow(DropCtx *c, size_t dev_need, size_t host_need) {
+    if (dev_need > c->dev_cap) {
+        if (c->dev) (void)hipFree(c->dev);
+        c->dev = nullptr;
+        c->dev_cap = 0;
+        const size_t cap = dev_need + dev_need / 4;
+        if (hipMalloc(reinterpret_cast<void **>(&c->dev), cap) != hipSuccess) return false;
+        c->dev_cap = cap;
+    }
+    if (host_need > c->host_cap) {
+        if (c->host) (void)hipHostFree(c->host);
+        c->host = nullptr;
+        c->host_cap = 0;
+        const size_t cap = host_need + host_need / 4;
+        if (hipHostMalloc(reinterpret_cast<void **>(&c->host), cap, hipHostMallocDefault) !=
+            hipSuccess)
+            return false;
+        c->host_cap = cap;
+    }
+    return true;
+}
+
+// Device copy of the tables, from the cache or uploaded once; NULL when the
+// cache is full for this device (the caller stages them with the call).
+const int32_t *cached_tables(int device, int n, int nc, int ne, uint64_t hash,
+                             const int32_t *cptr, const int32_t *cidx, const int32_t *vptr,
+                             const int32_t *vedge, bool *failed) {
+    std::lock_guard<std::mutex> lock(g_drop_mu);
+    int count = 0;
+    for (const DropTab &t : g_drop_tab) {
+        if (t.device != device) continue;
+        ++count;
+        if (t.n == n && t.nc == nc && t.ne == ne && t.hash == hash) return t.d;
+    }
+    if (count >= kMaxTablesPerDevice) return nullptr;
+    const size_t words = (size_t)nc + 1 + ne + (size_t)n + 1 + ne;
+    int32_t *d = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&d), words * 4) != hipSuccess) {
+        *failed = true;
+        return nullptr;
+    }
+    size_t o = 0;
+    bool ok = hipMemcpy(d + o, cptr, ((size_t)nc + 1) * 4, hipMemcpyHostToDevice) == hipSuccess;
+    o += (size_t)nc + 1;
+    ok = ok && hipMemcpy(d + o, cidx, (size_t)ne * 4, hipMemcpyHostToDevice) == hipSuccess;
+    o += ne;
+    ok = ok && hipMemcpy(d + o, vptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice) == hipSuccess;
+    o += (size_t)n + 1;
+    ok = ok && hipMemcpy(d + o, vedge, (size_t)ne * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(d);
+        *failed = true;
+        return nullptr;
+    }
+    g_drop_tab.push_back(DropTab{device, n, nc, ne, hash, d});
+    return d;
+}
+
 }  // namespace
 
 extern "C" int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, int32_t nc,
@@ -139,44 +267,99 @@ extern "C" int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, in
                                    uint8_t *bits, uint8_t *conv, int32_t *its,
                                    int32_t *bad, float *work) {
     (void)work;
-    if (nblk < 0 || n <= 0 || nc < 0 || nedges < 0) return ATSC3_E_ARG;
+    if (nblk < 0 || n <= 0 || nc < 0 || nedges < 0 || iters < 0) return ATSC3_E_ARG;
     if (!cptr) return ATSC3_E_ARG;
     if (atsc3::check_degrees(cptr, nc) != ATSC3_OK) return ATSC3_E_DEGREE;
     if (nblk == 0) return ATSC3_OK;
-    const size_t nb = (size_t)nblk, nn = (size_t)n, ne = (size_t)nedges;
-    DevBuf d_llr, d_cptr, d_cidx, d_vptr, d_vedge, d_tot, d_msg, d_bits, d_conv, d_its,
-        d_bad;
-    hipStream_t st = nullptr;
-    if (hipStreamCreate(&st) != hipSuccess) return ATSC3_E_HIP;
+    if (!llr || !cidx || !vptr || !vedge || !bits || !conv || !its || !bad) return ATSC3_E_ARG;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return ATSC3_E_HIP;
+
+    uint64_t h = 0xcbf29ce484222325ull ^ ((uint64_t)n << 40) ^ ((uint64_t)nc << 20) ^ nedges;
+    h = mix_words(cptr, (size_t)nc + 1, h);
+    h = mix_words(cidx, (size_t)nedges, h);
+    h = mix_words(vptr, (size_t)n + 1, h);
+    h = mix_words(vedge, (size_t)nedges, h);
+    bool failed = false;
+    const int32_t *tabs =
+        cached_tables(device, n, nc, nedges, h, cptr, cidx, vptr, vedge, &failed);
+    if (failed) return ATSC3_E_HIP;
+    const size_t tab_words = (size_t)nc + 1 + nedges + (size_t)n + 1 + nedges;
+
+    const size_t nb = (size_t)nblk, nn = (size_t)n;
+    const int chunk = 65535;                      // grid.y limit of the phase kernels
+    const int first_chunk = nblk < chunk ? nblk : chunk;
+    // device: llr | workspace | bits | conv | its | bad (one chunk), [tables]
+    const size_t d_llr = 0;
+    const size_t d_ws = up256(d_llr + (size_t)first_chunk * nn * 4);
+    const size_t d_bits = d_ws + up256(atsc3::csr_phase_ws_bytes(first_chunk, n, nedges));
+    const size_t d_conv = d_bits + up256((size_t)first_chunk * nn);
+    const size_t d_its = d_conv + up256((size_t)first_chunk);
+    const size_t d_bad = d_its + up256((size_t)first_chunk * 4);
+    const size_t d_tab = d_bad + up256((size_t)first_chunk * 4);
+    const size_t d_need = d_tab + (tabs ? 0 : tab_words * 4);
+    // pinned host: llr in (one chunk) | all outputs
+    const size_t h_out = up256((size_t)first_chunk * nn * 4);
+    const size_t h_need = h_out + nb * nn + nb + 8 * nb;
+
+    DropCtx *c = acquire_ctx(device);
+    if (!c) return ATSC3_E_HIP;
     int32_t rc = ATSC3_E_HIP;
     do {
-        if (d_llr.alloc(nb * nn * 4) || d_cptr.alloc(((size_t)nc + 1) * 4) ||
-            d_cidx.alloc(ne * 4) || d_vptr.alloc((nn + 1) * 4) || d_vedge.alloc(ne * 4) ||
-            d_tot.alloc(nb * nn * 4) || d_msg.alloc(nb * ne * 4) || d_bits.alloc(nb * nn) ||
-            d_conv.alloc(nb) || d_its.alloc(nb * 4) || d_bad.alloc(nb * 4))
-            break;
-        if (hipMemcpyAsync(d_llr.p, llr, nb * nn * 4, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d_cptr.p, cptr, ((size_t)nc + 1) * 4, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d_cidx.p, cidx, ne * 4, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d_vptr.p, vptr, (nn + 1) * 4, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d_vedge.p, vedge, ne * 4, hipMemcpyHostToDevice, st))
-            break;
-        rc = atsc3_minsum_csr_f32_dev(
-            (const float *)d_llr.p, nblk, n, nc, nedges, (const int32_t *)d_cptr.p,
-            (const int32_t *)d_cidx.p, (const int32_t *)d_vptr.p, (const int32_t *)d_vedge.p,
-            iters, alpha, (float *)d_tot.p, (float *)d_msg.p, (uint8_t *)d_bits.p,
-            (uint8_t *)d_conv.p, (int32_t *)d_its.p, (int32_t *)d_bad.p, st);
-        if (rc != ATSC3_OK) break;
-        rc = ATSC3_E_HIP;
-        if (hipMemcpyAsync(bits, d_bits.p, nb * nn, hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(conv, d_conv.p, nb, hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(its, d_its.p, nb * 4, hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(bad, d_bad.p, nb * 4, hipMemcpyDeviceToHost, st))
-            break;
-        if (hipStreamSynchronize(st) != hipSuccess) break;
+        if (!grow(c, d_need, h_need)) break;
+        hipStream_t st = c->st;
+        if (!tabs) {
+            int32_t *d = reinterpret_cast<int32_t *>(c->dev + d_tab);
+            size_t o = 0;
+            if (hipMemcpyAsync(d + o, cptr, ((size_t)nc + 1) * 4, hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d + (o += (size_t)nc + 1), cidx, (size_t)nedges * 4,
+                               hipMemcpyHostToDevice, st) ||
+                hipMemcpyAsync(d + (o += nedges), vptr, (nn + 1) * 4, hipMemcpyHostToDevice,
+                               st) ||
+                hipMemcpyAsync(d + (o += nn + 1), vedge, (size_t)nedges * 4,
+                               hipMemcpyHostToDevice, st))
+                break;
+            tabs = d;
+        }
+        const int32_t *t_cptr = tabs, *t_cidx = tabs + nc + 1;
+        const int32_t *t_vptr = t_cidx + nedges, *t_vedge = t_vptr + n + 1;
+        uint8_t *hb = reinterpret_cast<uint8_t *>(c->host + h_out);
+        uint8_t *hc = hb + nb * nn;
+        int32_t *hi = reinterpret_cast<int32_t *>(hc + nb);
+        int32_t *hd = hi + nb;
+        bool ok = true;
+        for (int b0 = 0; b0 < nblk && ok; b0 += chunk) {
+            const int k = nblk - b0 < chunk ? nblk - b0 : chunk;
+            std::memcpy(c->host, llr + (size_t)b0 * nn, (size_t)k * nn * 4);
+            ok = hipMemcpyAsync(c->dev + d_llr, c->host, (size_t)k * nn * 4,
+                                hipMemcpyHostToDevice, st) == hipSuccess;
+            ok = ok && atsc3::csr_phase_kc_f32(
+                           reinterpret_cast<const float *>(c->dev + d_llr), k, n, nc, nedges,
+                           t_cptr, t_cidx, t_vptr, t_vedge, iters, alpha, c->dev + d_ws,
+                           reinterpret_cast<uint8_t *>(c->dev + d_bits),
+                           reinterpret_cast<uint8_t *>(c->dev + d_conv),
+                           reinterpret_cast<int32_t *>(c->dev + d_its),
+                           reinterpret_cast<int32_t *>(c->dev + d_bad), st) == ATSC3_OK;
+            ok = ok &&
+                 hipMemcpyAsync(hb + (size_t)b0 * nn, c->dev + d_bits, (size_t)k * nn,
+                                hipMemcpyDeviceToHost, st) == hipSuccess &&
+                 hipMemcpyAsync(hc + b0, c->dev + d_conv, k, hipMemcpyDeviceToHost, st) ==
+                     hipSuccess &&
+                 hipMemcpyAsync(hi + b0, c->dev + d_its, (size_t)k * 4, hipMemcpyDeviceToHost,
+                                st) == hipSuccess &&
+                 hipMemcpyAsync(hd + b0, c->dev + d_bad, (size_t)k * 4, hipMemcpyDeviceToHost,
+                                st) == hipSuccess;
+            // the staging buffer of the next chunk's LLRs is reused
+            ok = ok && hipStreamSynchronize(st) == hipSuccess;
+        }
+        if (!ok) break;
+        std::memcpy(bits, hb, nb * nn);
+        std::memcpy(conv, hc, nb);
+        std::memcpy(its, hi, nb * 4);
+        std::memcpy(bad, hd, nb * 4);
         rc = ATSC3_OK;
     } while (0);
-    (void)hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
+    (void)hipStreamSynchronize(c->st);
+    release_ctx(c);
     return rc;
 }

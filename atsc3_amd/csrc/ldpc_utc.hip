@@ -68,8 +68,11 @@ __device__ __forceinline__ void two_min(T a, T &lo1, T &lo2) {
     }
 }
 
-// Check launch of iteration `it`: syndrome (it >= 1) and update (it < iters).
-template <typename T>
+// Check launch of iteration `it`: syndrome (it >= 1; KC: it >= 0) and
+// update (it < iters).  KC = the compiled kernel's contract
+// (lab/ldpc_kernel.c:82-143): no row padding, and the edge that first
+// attains min1 in the strict-< scan takes min2 (a NaN magnitude never wins).
+template <typename T, bool KC>
 __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
     const int blk = blockIdx.y;
     if (a.done_it[blk] < it) return;
@@ -84,28 +87,42 @@ __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
         if (it < a.iters) {
             unsigned par = 0u;
             T lo1 = (T)INFINITY, lo2 = (T)INFINITY;
+            int first = e0;
             for (int e = e0; e < e1; ++e) {
                 const T tv = src[a.cidx[e]];
                 const T x = tv - (it == 0 ? zero : m[e]);
                 syn ^= (unsigned)(tv < zero);
                 par ^= (unsigned)(x < zero);
-                two_min(x < zero ? -x : x, lo1, lo2);
+                const T mag = x < zero ? -x : x;
+                if constexpr (KC) {
+                    if (mag < lo1) {
+                        lo2 = lo1;
+                        lo1 = mag;
+                        first = e;
+                    } else if (mag < lo2) {
+                        lo2 = mag;
+                    }
+                } else {
+                    two_min(mag, lo1, lo2);
+                }
             }
-            const int npad = a.dmax - (e1 - e0);
-            if (npad >= 1) two_min(big, lo1, lo2);
-            if (npad >= 2) two_min(big, lo1, lo2);
+            if constexpr (!KC) {
+                const int npad = a.dmax - (e1 - e0);
+                if (npad >= 1) two_min(big, lo1, lo2);
+                if (npad >= 2) two_min(big, lo1, lo2);
+            }
             const T s1 = a.alpha * lo1, s2 = a.alpha * lo2;
             for (int e = e0; e < e1; ++e) {
                 const T x = src[a.cidx[e]] - (it == 0 ? zero : m[e]);
                 const T mag = x < zero ? -x : x;
-                const T v = mag == lo1 ? s2 : s1;
+                const T v = (KC ? e == first : mag == lo1) ? s2 : s1;
                 m[e] = (par ^ (unsigned)(x < zero)) ? -v : v;
             }
         } else {
             for (int e = e0; e < e1; ++e) syn ^= (unsigned)(src[a.cidx[e]] < zero);
         }
     }
-    if (it >= 1) {
+    if (KC || it >= 1) {
         // one atomic per wave
         const unsigned long long b = __ballot(syn != 0u);
         if ((threadIdx.x & 63) == 0 && b)
@@ -114,18 +131,20 @@ __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
 }
 
 // Variable launch of iteration `it`: retire (syndrome read zero, or it ==
-// iters) or update the posteriors.
-template <typename T>
+// iters) or update the posteriors.  KC: a codeword may retire at it = 0
+// (channel decisions already satisfy H) and the sum is lam + (0 + m_1 + ...).
+template <typename T, bool KC>
 __global__ __launch_bounds__(UTC_WG) void utc_var(UtcArgs<T> a, int it) {
     const int blk = blockIdx.y;
     const int v = blockIdx.x * UTC_WG + threadIdx.x;
     if (v == 0) a.cnt[((it + 1) & 1) * a.nblk + blk] = 0;   // next count
     if (a.done_it[blk] < it) return;
     const int count = a.cnt[(it & 1) * a.nblk + blk];
-    const bool retire = (it >= 1 && count == 0) || it == a.iters;
+    const bool retire = ((KC || it >= 1) && count == 0) || it == a.iters;
     T *tot = a.tot + (size_t)blk * a.n;
     if (retire) {
-        if (v < a.n) a.bits[(size_t)blk * a.n + v] = (uint8_t)(tot[v] < (T)0);
+        const T *src = it == 0 ? a.llr + (size_t)blk * a.n : tot;
+        if (v < a.n) a.bits[(size_t)blk * a.n + v] = (uint8_t)(src[v] < (T)0);
         if (v == 0) {
             a.conv[blk] = (uint8_t)(count == 0);
             a.its[blk] = count == 0 ? it : a.iters;
@@ -136,9 +155,10 @@ __global__ __launch_bounds__(UTC_WG) void utc_var(UtcArgs<T> a, int it) {
     }
     if (v < a.n) {
         const T *m = a.msg + (size_t)blk * a.nedges;
-        T acc = a.llr[(size_t)blk * a.n + v];
+        const T lam = a.llr[(size_t)blk * a.n + v];
+        T acc = KC ? (T)0 : lam;
         for (int k = a.vptr[v]; k < a.vptr[v + 1]; ++k) acc = acc + m[a.vedge[k]];
-        tot[v] = acc;
+        tot[v] = KC ? lam + acc : acc;
     }
 }
 
@@ -164,7 +184,7 @@ static size_t utc_ws_bytes(int nblk, int n, int nedges, size_t tsize) {
            align256((size_t)nblk * 3 * sizeof(int));
 }
 
-template <typename T>
+template <typename T, bool KC>
 static int32_t utc_launch(const T *llr, int nblk, int n, int nc, int nedges, const int *cptr,
                           const int *cidx, const int *vptr, const int *vedge, int dmax,
                           int iters, double alpha, void *ws, uint8_t *bits, uint8_t *conv,
@@ -194,7 +214,7 @@ static int32_t utc_launch(const T *llr, int nblk, int n, int nc, int nedges, con
     a.its = its;
     a.bad = bad;
     const dim3 vg((n + UTC_WG - 1) / UTC_WG, nblk);
-    if (iters == 0) {
+    if (!KC && iters == 0) {
         hipLaunchKernelGGL(utc_zero<T>, vg, dim3(UTC_WG), 0, st, a);
         return launch_status();
     }
@@ -202,10 +222,20 @@ static int32_t utc_launch(const T *llr, int nblk, int n, int nc, int nedges, con
     hipLaunchKernelGGL(utc_init, dim3((nblk + 255) / 256), dim3(256), 0, st, a.done_it, a.cnt,
                        nblk);
     for (int it = 0; it <= iters; ++it) {
-        hipLaunchKernelGGL(utc_check<T>, cg, dim3(UTC_WG), 0, st, a, it);
-        hipLaunchKernelGGL(utc_var<T>, vg, dim3(UTC_WG), 0, st, a, it);
+        hipLaunchKernelGGL((utc_check<T, KC>), cg, dim3(UTC_WG), 0, st, a, it);
+        hipLaunchKernelGGL((utc_var<T, KC>), vg, dim3(UTC_WG), 0, st, a, it);
     }
     return launch_status();
+}
+
+size_t csr_phase_ws_bytes(int nblk, int n, int nedges) { return utc_ws_bytes(nblk, n, nedges, 4); }
+
+int32_t csr_phase_kc_f32(const float *llr, int nblk, int n, int nc, int nedges, const int *cptr,
+                         const int *cidx, const int *vptr, const int *vedge, int iters,
+                         float alpha, void *ws, uint8_t *bits, uint8_t *conv, int *its,
+                         int *bad, hipStream_t st) {
+    return utc_launch<float, true>(llr, nblk, n, nc, nedges, cptr, cidx, vptr, vedge, 0, iters,
+                                   (double)alpha, ws, bits, conv, its, bad, st);
 }
 
 }  // namespace atsc3
@@ -233,10 +263,10 @@ extern "C" int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t 
         return ATSC3_E_SIZE;
     hipStream_t st = atsc3::as_stream(stream);
     if (dtype == 1)
-        return atsc3::utc_launch<double>(static_cast<const double *>(llr), nblk, n, nc, nedges,
+        return atsc3::utc_launch<double, false>(static_cast<const double *>(llr), nblk, n, nc, nedges,
                                          cptr, cidx, vptr, vedge, dmax, iters, alpha, ws, bits,
                                          conv, its, bad, st);
-    return atsc3::utc_launch<float>(static_cast<const float *>(llr), nblk, n, nc, nedges, cptr,
+    return atsc3::utc_launch<float, false>(static_cast<const float *>(llr), nblk, n, nc, nedges, cptr,
                                     cidx, vptr, vedge, dmax, iters, alpha, ws, bits, conv, its,
                                     bad, st);
 }

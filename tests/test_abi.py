@@ -137,3 +137,33 @@ def test_resident_entry_rejects_bad_arguments():
     assert call(max_dc=90) == E_DEGREE
     assert call(max_dv=33) == E_DEGREE
     assert call(ws_bytes=ws_need - 1) == E_SIZE
+
+
+def test_dropin_degree_limit_writes_nothing():
+    """ldpc_minsum_f32 returns -1 when a check degree exceeds 128, as
+    lab/ldpc_kernel.c:73-80 does, before any device work, and leaves every
+    output untouched (lab/m3_ldpc.py:498-500 then falls back)."""
+    import numpy as np
+    from atsc3_amd.ldpc import _GenericCode
+    L = _lib().lib()
+    n = 200
+    checks = [list(range(129)), [0, 150, 199], [5, 6]]
+    code = _GenericCode(checks, n)
+    cptr, cidx, vptr, vedge = code.csr()
+    llr = np.ones((2, n), np.float32)
+    bits = np.full((2, n), 7, np.uint8)
+    conv = np.full(2, 7, np.uint8)
+    its = np.full(2, -9, np.int32)
+    bad = np.full(2, -9, np.int32)
+    work = np.empty(n + len(cidx), np.float32)
+    p = lambda a: a.ctypes.data
+    rc = L.ldpc_minsum_f32(p(llr), 2, n, len(checks), len(cidx), p(cptr), p(cidx), p(vptr),
+                           p(vedge), 50, 0.75, p(bits), p(conv), p(its), p(bad), p(work))
+    assert rc == -1
+    assert (bits == 7).all() and (conv == 7).all() and (its == -9).all() and (bad == -9).all()
+    # degree exactly 128 passes the check (the empty batch needs no device)
+    checks[0] = list(range(128))
+    code = _GenericCode(checks, n)
+    cptr, cidx, vptr, vedge = code.csr()
+    assert L.ldpc_minsum_f32(p(llr), 0, n, len(checks), len(cidx), p(cptr), p(cidx), p(vptr),
+                             p(vedge), 50, 0.75, p(bits), p(conv), p(its), p(bad), p(work)) == 0
