@@ -90,6 +90,9 @@ _SIGS = {
     "atsc3_minsum_resident_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
                                   c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32, c_vp,
                                   ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_resident_llr_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
+                                      c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32,
+                                      c_vp, ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_minsum_resident_noalloc_max_bytes": [],
     "atsc3_minsum_resident_noalloc_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                           c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp,

@@ -122,6 +122,7 @@ struct ResArgs {
     int *queue;
     uint8_t *bits, *conv;
     int *its, *bad;
+    float *post;                  // NULL, or the final posterior of every bit
 };
 
 // FP16 storage of a posterior or a message magnitude: saturated to the
@@ -631,9 +632,11 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                     const int t = task / 6;
                     const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
                     const unsigned vw = (unsigned)ct->grp_v[t];
-                    if (p < (unsigned)ATSC3_Z)
-                        bout[(unsigned)t * ATSC3_Z + p] =
-                            (uint8_t)(tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p) < 0.0f);
+                    if (p < (unsigned)ATSC3_Z) {
+                        const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
+                        bout[(unsigned)t * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
+                        if (a.post) a.post[off + (unsigned)t * ATSC3_Z + p] = v;
+                    }
                 }
                 if (tid == 0) {
                     a.conv[blk] = (uint8_t)(total == 0);
@@ -752,7 +755,7 @@ int32_t resident_launch(const float *llr, int32_t nblk, const int64_t *blk_off,
                         int32_t max_dc, int32_t max_dv, int32_t iters, float alpha,
                         int32_t msg_fmt, const float *msg_scale, int32_t grid, void *ws,
                         int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
-                        int32_t *bad, hipStream_t st) {
+                        int32_t *bad, float *post, hipStream_t st) {
     const bool f32 = msg_fmt == 0;
     const WsLayout L(grid, max_checks, max_global_rows, f32);
     if (ws_bytes < L.bytes) return ATSC3_E_SIZE;
@@ -777,6 +780,7 @@ int32_t resident_launch(const float *llr, int32_t nblk, const int64_t *blk_off,
     a.conv = conv;
     a.its = its;
     a.bad = bad;
+    a.post = post;
     hipLaunchKernelGGL(resident_init, dim3(1), dim3(64), 0, st, a.queue);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
     a.msg_scale = msg_fmt == 2 ? msg_scale : nullptr;
@@ -825,6 +829,27 @@ extern "C" int32_t atsc3_minsum_resident_dev(
     return atsc3::resident_launch(llr, nblk, blk_off, blk_code, codes, ncode, max_checks,
                                   max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
                                   msg_scale, grid, ws, ws_bytes, bits, conv, its, bad,
+                                  nullptr, atsc3::as_stream(stream));
+}
+
+// The same decode also writing each codeword's final posterior LLRs
+// (llr_out, laid out like bits): FP32 mode the FP32 posteriors, FP16 / INT8
+// modes the FP16 posteriors widened to FP32.
+extern "C" int32_t atsc3_minsum_resident_llr_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
+    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
+    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, int32_t grid, void *ws, int64_t ws_bytes, uint8_t *bits,
+    uint8_t *conv, int32_t *its, int32_t *bad, float *llr_out, void *stream) {
+    const int32_t rc = atsc3::resident_check_args(llr, nblk, blk_off, blk_code, codes, ncode,
+                                                  max_checks, max_global_rows, max_n, max_dc,
+                                                  max_dv, iters, alpha, msg_fmt, grid, bits,
+                                                  conv, its, bad);
+    if (rc != ATSC3_OK || nblk == 0) return rc;
+    if (!ws || !llr_out) return ATSC3_E_ARG;
+    return atsc3::resident_launch(llr, nblk, blk_off, blk_code, codes, ncode, max_checks,
+                                  max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
+                                  msg_scale, grid, ws, ws_bytes, bits, conv, its, bad, llr_out,
                                   atsc3::as_stream(stream));
 }
 
@@ -886,5 +911,5 @@ extern "C" int32_t atsc3_minsum_resident_noalloc_dev(
     return atsc3::resident_launch(llr, nblk, blk_off, blk_code, codes, ncode, max_checks,
                                   max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
                                   msg_scale, grid, ws, RES_SCRATCH_BYTES, bits, conv, its, bad,
-                                  atsc3::as_stream(stream));
+                                  nullptr, atsc3::as_stream(stream));
 }

@@ -87,13 +87,17 @@ class Trainer:
 
     @torch.no_grad()
     def evaluate(self, ebn0_db, nblk, chunk=2048, fallback_iters=50, seed_offset=10 ** 6):
-        """FER at one Eb/N0: learned table, plain table (every factor alpha0),
-        the plain bit-exact min-sum with early stop (iters = T), and the learned
-        decoder with the plain 50-iteration min-sum as fallback."""
+        """FER at one Eb/N0 on the same fresh blocks: learned table, plain
+        table (every factor alpha0), the plain bit-exact min-sum with early
+        stop at iters = T and at the upstream's 50 iterations, the upstream's
+        alpha ladder (1.0, 0.85, 0.75 at 50 iterations, lab/m9_fast.py:
+        843-891), and the learned decoder with the plain 50-iteration
+        min-sum as fallback."""
         es = esn0_of_ebn0(ebn0_db, self.rate, self.mod)
         plain = torch.full_like(self.alphas, self.alpha0)
         learned = self.alphas.detach()
-        err = dict(learned=0, plain_unrolled=0, plain_minsum=0, learned_fallback=0)
+        err = dict(learned=0, plain_unrolled=0, plain_minsum=0, plain_minsum_50=0,
+                   plain_ladder_50=0, learned_fallback=0)
         done = 0
         while done < nblk:
             b = min(chunk, nblk - done)
@@ -116,6 +120,10 @@ class Trainer:
                     err["learned_fallback"] += int(bad_fb.sum().item())
             o = ldpc.decode_dev(llr, self.code, self.iters, self.alpha0)
             err["plain_minsum"] += int((o[0] != ref).any(1).sum().item())
+            o = ldpc.decode_dev(llr, self.code, fallback_iters, self.alpha0)
+            err["plain_minsum_50"] += int((o[0] != ref).any(1).sum().item())
+            o = ldpc.ladder_dev(llr, self.code, (1.0, 0.85, 0.75), fallback_iters)
+            err["plain_ladder_50"] += int((o[0] != ref).any(1).sum().item())
             done += b
         return {k: v / nblk for k, v in err.items()}, err
 

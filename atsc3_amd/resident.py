@@ -311,10 +311,12 @@ class ResidentPlan:
             if self.msg_scale.numel() != self.nblk:
                 raise ValueError("msg_scale needs one value per codeword")
 
-    def run(self, llr=None, out=None):
+    def run(self, llr=None, out=None, post=None):
         """Enqueue the decode of ``llr`` (flat, or None: self.llr as filled)
         into ``out`` = (bits, conv, its, bad) device tensors of the plan's
-        sizes (None: the plan's own buffers, overwritten by the next run)."""
+        sizes (None: the plan's own buffers, overwritten by the next run).
+        ``post``: a float32 device tensor of the plan's length that receives
+        every bit's final posterior LLR (the soft output)."""
         src = self.llr if llr is None else llr
         if src.numel() != self.total or src.dtype != torch.float32 or not src.is_contiguous():
             raise ValueError("llr must be a contiguous float32 tensor of the plan's length")
@@ -323,14 +325,22 @@ class ResidentPlan:
                 or bad.numel() != self.nblk):
             raise ValueError("output tensors do not match the plan")
         b = self.bank
-        _lib.call("atsc3_minsum_resident_dev", src.data_ptr(), self.nblk,
-                  self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
-                  len(b), b.max_checks, b.max_global_rows, b.max_n, b.max_dc, b.max_dv,
-                  self.iters, self.alpha, self.fmt,
-                  None if self.msg_scale is None else self.msg_scale.data_ptr(),
-                  self.grid, self.ws.data_ptr(),
-                  self.ws_bytes, bits.data_ptr(),
-                  conv.data_ptr(), its.data_ptr(), bad.data_ptr(), _stream_ptr(self.device))
+        args = (src.data_ptr(), self.nblk,
+                self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
+                len(b), b.max_checks, b.max_global_rows, b.max_n, b.max_dc, b.max_dv,
+                self.iters, self.alpha, self.fmt,
+                None if self.msg_scale is None else self.msg_scale.data_ptr(),
+                self.grid, self.ws.data_ptr(),
+                self.ws_bytes, bits.data_ptr(),
+                conv.data_ptr(), its.data_ptr(), bad.data_ptr())
+        if post is None:
+            _lib.call("atsc3_minsum_resident_dev", *args, _stream_ptr(self.device))
+        else:
+            if (post.numel() != self.total or post.dtype != torch.float32
+                    or not post.is_contiguous()):
+                raise ValueError("post must be a contiguous float32 tensor of the plan's length")
+            _lib.call("atsc3_minsum_resident_llr_dev", *args, post.data_ptr(),
+                      _stream_ptr(self.device))
         return bits, conv, its, bad
 
     def check(self):

@@ -298,6 +298,25 @@ int32_t atsc3_minsum_resident_dev(const float *llr, int32_t nblk,
                                   uint8_t *bits, uint8_t *conv, int32_t *its,
                                   int32_t *bad, void *stream);
 
+/* The same decode also writing the final posterior LLR of every codeword
+ * bit to llr_out (laid out like bits; FP32 mode the FP32 posteriors, FP16 /
+ * INT8 modes the stored FP16 posteriors widened to FP32): the soft output
+ * the LLR-error measurement of the message forms reads.  No upstream
+ * counterpart (the upstream decoders return hard bits only). */
+int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
+                                      const int64_t *blk_off,
+                                      const int32_t *blk_code,
+                                      const void *codes, int32_t ncode,
+                                      int32_t max_checks,
+                                      int32_t max_global_rows, int32_t max_n,
+                                      int32_t max_dc, int32_t max_dv,
+                                      int32_t iters, float alpha,
+                                      int32_t msg_fmt, const float *msg_scale,
+                                      int32_t grid, void *ws, int64_t ws_bytes,
+                                      uint8_t *bits, uint8_t *conv,
+                                      int32_t *its, int32_t *bad,
+                                      float *llr_out, void *stream);
+
 /* ------------------------------------------------------------------------
  * Resident min-sum with no allocation.  No counterpart in the upstream: it
  * is for callers that cannot allocate device memory (code under HIP graph

@@ -68,13 +68,15 @@ def int8_step(llr_block):
 
 
 def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant=None,
-                    steps=None):
+                    steps=None, return_post=False):
     """(nblk, n) LLRs -> (bits u8 (nblk, n), conv bool, its i32, bad i32).
 
     storage "int8": posteriors as for "float16"; each message magnitude is an
     INT8 level of the codeword's step, min(rint(a / step), 127) * step (the
     INT8 message format's codec), with ``steps`` one per codeword or None
-    (``int8_step`` of each codeword's LLRs).
+    (``int8_step`` of each codeword's LLRs).  return_post: also return each
+    codeword's posteriors when it finished (nblk, n) float32 (the stored
+    values: FP16-rounded in the FP16 / INT8 modes).
 
     ``_plant="list_order"`` is a negative control for the tests: with FP32
     storage each variable's messages are summed in table order (the FP16 /
@@ -122,6 +124,7 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
     conv = np.zeros(B, bool)
     its = np.zeros(B, np.int32)
     bad = np.zeros(B, np.int32)
+    post = np.zeros((B, nvg * Z), f32)
     done = np.zeros(B, bool)
     j = np.arange(Z)
     with np.errstate(invalid="ignore", over="ignore"):
@@ -155,6 +158,7 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
             fin = ~done & ((unsat == 0) | (it == iters))
             if fin.any():
                 bits[fin] = (tot[fin] < 0).reshape(int(fin.sum()), -1)
+                post[fin] = tot[fin].reshape(int(fin.sum()), -1)
                 conv[fin] = unsat[fin] == 0
                 its[fin] = np.where(unsat[fin] == 0, it, iters)
                 bad[fin] = unsat[fin]
@@ -179,4 +183,6 @@ def resident_minsum(llr2d, code, iters=50, alpha=0.75, storage="float16", _plant
                             m = np.where(sg[:, g, k, jj], -m, m)
                             s = np.where(use[None, :], (s + m).astype(f32), s)
                 tot[:, tg] = rnd(llr[:, tg] + s)
+    if return_post:
+        return bits, conv, its, bad, post
     return bits, conv, its, bad

@@ -530,3 +530,34 @@ def test_two_stream_pipeline_equals_one_stream(device):
     torch.cuda.synchronize()
     for k, out in enumerate(outs):
         _check([o.cpu().numpy() for o in out], want, f"window {k}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float16", "float32"])
+def test_soft_output_equals_oracle(device, msg):
+    """atsc3_minsum_resident_llr_dev: the same hard results as the plain
+    entry, and every bit's final posterior equal to the oracle's (FP16 mode:
+    the stored FP16 value; FP32 mode: the FP32 posterior of the reference's
+    arithmetic), bit for bit; bits are the posteriors' signs."""
+    import torch
+    from atsc3_amd import synth
+    from atsc3_amd.codes import get_code
+    from atsc3_amd.resident import ResidentPlan
+    code = get_code("6/15", 16200)
+    cw = synth.codewords("6/15", 16200, 6, 811)
+    llr = np.concatenate([synth.llr_bpsk_like(cw[:3], 1.5, 812),
+                          synth.llr_bpsk_like(cw[3:], 1.0, 813)]).astype(np.float32)
+    print(f"[gpu] soft output {msg}", flush=True)
+    plan = ResidentPlan([code] * 6, device, 30, 0.75, 4, msg)
+    t = torch.from_numpy(llr.ravel()).to(device)
+    post = torch.empty(plan.total, dtype=torch.float32, device=device)
+    outs = [x.clone() for x in plan.run(t, post=post)]
+    ref = [x.clone() for x in plan.run(t)]
+    torch.cuda.synchronize()
+    for a, b in zip(outs, ref):
+        assert torch.equal(a, b)
+    want = resident_minsum(llr, code, 30, 0.75, msg, return_post=True)
+    p = post.cpu().numpy().reshape(6, -1)
+    assert np.array_equal(p.view(np.uint32), want[4].view(np.uint32))
+    assert np.array_equal(outs[0].cpu().numpy().reshape(6, -1), (p < 0).astype(np.uint8))
+    assert np.array_equal(outs[2].cpu().numpy(), want[2])
