@@ -22,6 +22,7 @@ _SIGS = {
     "ldpc_kernel_abi": [],
     "ldpc_minsum_f32": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                         c_i32, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_dropin_engine": [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
     "atsc3_minsum_csr_f32_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
                                  c_vp, c_i32, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp],
@@ -155,7 +156,8 @@ SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ld
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
                 "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows",
-                "atsc3_minsum_utc_ws_bytes", "atsc3_minsum_resident_noalloc_max_bytes")
+                "atsc3_minsum_utc_ws_bytes", "atsc3_minsum_resident_noalloc_max_bytes",
+                "atsc3_dropin_engine")
 
 
 def call(name, *args):

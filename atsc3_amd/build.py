@@ -62,6 +62,10 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed ({r.returncode}):\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # the resident engine's tables of every A/322 code, for the host-pointer
+    # drop-in ldpc_minsum_f32 (read beside the library at its first call)
+    from .resident import DROPIN_TABLES, write_dropin_tables
+    write_dropin_tables(os.path.join(HERE, DROPIN_TABLES))
     return LIB
 
 

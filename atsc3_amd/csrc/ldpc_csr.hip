@@ -131,16 +131,24 @@ extern "C" int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int3
 //     of (n, nc, nedges, cptr, cidx, vptr, vedge) and uploads them only the
 //     first time (up to 64 codes per device; beyond that they travel with
 //     the call);
-//   * decodes on the CSR phase kernels (ldpc_utc.hip, the compiled kernel's
-//     contract), which spread the call's checks and variables over the whole
-//     GPU rather than one workgroup per codeword.
+//   * decodes an A/322 code -- recognised by the same hash among the tables
+//     the build writes beside this library (libatsc3fec_tables.bin) -- on
+//     the resident engine's FP32 mode (bit-exact with the reference kernel),
+//     and any other graph on the CSR phase kernels (ldpc_utc.hip, the
+//     compiled kernel's contract), which spread the call's checks and
+//     variables over the whole GPU.
 // Outputs are staged and copied to the caller's arrays only after the whole
 // decode succeeded, so on any error nothing is written.
 // ---------------------------------------------------------------------------
 extern "C" int32_t ldpc_kernel_abi(void) { return 1; }
 
+#include <dlfcn.h>
+
+#include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace {
@@ -166,14 +174,22 @@ std::vector<DropCtx *> g_drop_ctx;   // never freed: reused for the process life
 std::vector<DropTab> g_drop_tab;
 constexpr int kMaxTablesPerDevice = 64;
 
-uint64_t mix_words(const int32_t *p, size_t n, uint64_t h) {
-    for (size_t i = 0; i < n; ++i) {
-        h ^= (uint32_t)p[i];
-        h *= 0x100000001b3ull;
-        h ^= h >> 29;
+// Order-sensitive hash of the CSR tables (same value as csr_hash in
+// atsc3_amd/resident.py): the sum over the concatenated words w_i of a
+// 64-bit finaliser of (w_i, i), plus the sizes.
+struct CsrHash {
+    uint64_t h = 0, i = 0;
+    void add(const int32_t *p, size_t n) {
+        for (size_t k = 0; k < n; ++k, ++i) {
+            uint64_t x = (uint64_t)(uint32_t)p[k] * 0x9E3779B97F4A7C15ull +
+                         i * 0xC2B2AE3D27D4EB4Full;
+            x ^= x >> 31;
+            x *= 0xBF58476D1CE4E5B9ull;
+            x ^= x >> 29;
+            h += x;
+        }
     }
-    return h;
-}
+};
 
 size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -258,7 +274,207 @@ const int32_t *cached_tables(int device, int n, int nc, int ne, uint64_t hash,
     return d;
 }
 
+// ---- the resident route -------------------------------------------------
+// Entries of the table file written by the build (libatsc3fec_tables.bin
+// beside this library); per device, each code's tables and its
+// atsc3_res_code descriptor are uploaded at the first call that needs them.
+struct ResEntry {
+    uint64_t hash;
+    int32_t n, nc, ne, ncg, nvg, nbe, max_dc, max_dv, nsc, nsv, grows;
+    std::vector<std::vector<int32_t>> tab;   // 13 tables, atsc3_res_code order
+};
+struct ResCode {
+    int device;
+    const ResEntry *e;
+    void *desc;                              // device atsc3_res_code
+};
+std::vector<ResEntry> g_res_file;
+bool g_res_loaded = false;
+std::deque<ResCode> g_res_dev;   // a deque: entries handed out stay valid as it grows
+
+std::string table_file_path() {
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void *>(&ldpc_kernel_abi), &info) || !info.dli_fname)
+        return std::string();
+    std::string p(info.dli_fname);
+    const size_t slash = p.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) +
+           "/libatsc3fec_tables.bin";
+}
+
+// Parses the file once; a missing or malformed file leaves the list empty
+// (every code then runs on the CSR phase kernels).
+void load_table_file() {
+    if (g_res_loaded) return;
+    g_res_loaded = true;
+    FILE *f = std::fopen(table_file_path().c_str(), "rb");
+    if (!f) return;
+    std::vector<ResEntry> out;
+    char magic[4];
+    uint32_t ver = 0, count = 0;
+    bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "A3RT", 4) == 0 &&
+              std::fread(&ver, 4, 1, f) == 1 && ver == 1 && std::fread(&count, 4, 1, f) == 1 &&
+              count < 1024;
+    for (uint32_t c = 0; ok && c < count; ++c) {
+        ResEntry e;
+        int32_t hdr[11];
+        ok = std::fread(&e.hash, 8, 1, f) == 1 && std::fread(hdr, 4, 11, f) == 11;
+        if (!ok) break;
+        e.n = hdr[0]; e.nc = hdr[1]; e.ne = hdr[2]; e.ncg = hdr[3]; e.nvg = hdr[4];
+        e.nbe = hdr[5]; e.max_dc = hdr[6]; e.max_dv = hdr[7]; e.nsc = hdr[8]; e.nsv = hdr[9];
+        e.grows = hdr[10];
+        for (int t = 0; ok && t < 13; ++t) {
+            int32_t len = 0;
+            ok = std::fread(&len, 4, 1, f) == 1 && len >= 0 && len < (1 << 26);
+            if (!ok) break;
+            std::vector<int32_t> v((size_t)len);
+            ok = len == 0 || std::fread(v.data(), 4, (size_t)len, f) == (size_t)len;
+            e.tab.push_back(std::move(v));
+        }
+        if (ok) out.push_back(std::move(e));
+    }
+    std::fclose(f);
+    if (ok) g_res_file = std::move(out);
+}
+
+// The device copy of code `hash` (uploaded on first use), or NULL when the
+// file has no such code.
+const ResCode *resident_code(int device, uint64_t hash, int n, int nc, int ne, bool *failed) {
+    std::lock_guard<std::mutex> lock(g_drop_mu);
+    load_table_file();
+    for (const ResCode &r : g_res_dev)
+        if (r.device == device && r.e->hash == hash) return &r;
+    const ResEntry *e = nullptr;
+    for (const ResEntry &x : g_res_file)
+        if (x.hash == hash && x.n == n && x.nc == nc && x.ne == ne) e = &x;
+    if (!e || e->n > 64800 || e->max_dc > 89 || e->max_dv > 32) return nullptr;
+    size_t words = 0;
+    for (const auto &v : e->tab) words += v.size() + 64;   // 256-byte aligned tables
+    char *buf = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&buf), words * 4 + 256) != hipSuccess) {
+        *failed = true;
+        return nullptr;
+    }
+    struct {
+        int32_t n, ncg, nvg, nbe, max_dc, max_dv, nsc, nsv;
+        const int32_t *t[13];
+    } d;
+    static_assert(sizeof(d) == sizeof(atsc3_res_code), "descriptor layout");
+    d.n = e->n; d.ncg = e->ncg; d.nvg = e->nvg; d.nbe = e->nbe; d.max_dc = e->max_dc;
+    d.max_dv = e->max_dv; d.nsc = e->nsc; d.nsv = e->nsv;
+    size_t off = 256;                                  // the descriptor first
+    for (int t = 0; t < 13; ++t) {
+        const auto &v = e->tab[t];
+        d.t[t] = reinterpret_cast<const int32_t *>(buf + off);
+        if (!v.empty() &&
+            hipMemcpy(buf + off, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            *failed = true;
+            return nullptr;
+        }
+        off += up256(v.size() * 4);
+    }
+    if (hipMemcpy(buf, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess) {
+        *failed = true;
+        return nullptr;
+    }
+    g_res_dev.push_back(ResCode{device, e, buf});
+    return &g_res_dev.back();
+}
+
+int32_t resident_dropin(const ResCode *rc, int device, const float *llr, int32_t nblk, int32_t n,
+                        int32_t iters, float alpha, uint8_t *bits, uint8_t *conv, int32_t *its,
+                        int32_t *bad) {
+    const ResEntry &e = *rc->e;
+    const int cus = atsc3_minsum_resident_grid();
+    if (cus <= 0) return ATSC3_E_HIP;
+    const int grid = nblk < cus ? nblk : cus;
+    const int max_checks = e.ncg * ATSC3_Z;
+    const int64_t wsb = atsc3_minsum_resident_ws_bytes(grid, max_checks, e.grows, 0);
+    if (wsb <= 0) return ATSC3_E_HIP;
+    const size_t nb = (size_t)nblk, nn = (size_t)n;
+    // device and pinned host share the first three offsets: llr | blk_off | blk_code
+    const size_t o_off = up256(nb * nn * 4);
+    const size_t o_code = o_off + up256(nb * 8);
+    const size_t o_in_end = o_code + up256(nb * 4);
+    const size_t o_ws = o_in_end;
+    const size_t o_bits = o_ws + up256((size_t)wsb);
+    const size_t o_conv = o_bits + up256(nb * nn);
+    const size_t o_its = o_conv + up256(nb);
+    const size_t o_bad = o_its + up256(nb * 4);
+    const size_t d_need = o_bad + up256(nb * 4);
+    const size_t h_out = o_in_end;
+    const size_t h_need = h_out + nb * nn + nb + 8 * nb;
+    DropCtx *c = acquire_ctx(device);
+    if (!c) return ATSC3_E_HIP;
+    int32_t rc_out = ATSC3_E_HIP;
+    do {
+        if (!grow(c, d_need, h_need)) break;
+        hipStream_t st = c->st;
+        std::memcpy(c->host, llr, nb * nn * 4);
+        int64_t *hoff = reinterpret_cast<int64_t *>(c->host + o_off);
+        int32_t *hcode = reinterpret_cast<int32_t *>(c->host + o_code);
+        for (size_t b = 0; b < nb; ++b) {
+            hoff[b] = (int64_t)(b * nn);
+            hcode[b] = 0;
+        }
+        if (hipMemcpyAsync(c->dev, c->host, o_in_end, hipMemcpyHostToDevice, st) != hipSuccess)
+            break;
+        uint8_t *dbits = reinterpret_cast<uint8_t *>(c->dev + o_bits);
+        uint8_t *dconv = reinterpret_cast<uint8_t *>(c->dev + o_conv);
+        int32_t *dits = reinterpret_cast<int32_t *>(c->dev + o_its);
+        int32_t *dbad = reinterpret_cast<int32_t *>(c->dev + o_bad);
+        if (atsc3_minsum_resident_dev(reinterpret_cast<const float *>(c->dev), nblk,
+                                      reinterpret_cast<const int64_t *>(c->dev + o_off),
+                                      reinterpret_cast<const int32_t *>(c->dev + o_code),
+                                      rc->desc, 1, max_checks, e.grows, e.n, e.max_dc, e.max_dv,
+                                      iters, alpha, 0, nullptr, grid, c->dev + o_ws, wsb, dbits,
+                                      dconv, dits, dbad, st) != ATSC3_OK)
+            break;
+        uint8_t *hb = reinterpret_cast<uint8_t *>(c->host + h_out);
+        uint8_t *hc = hb + nb * nn;
+        int32_t *hi = reinterpret_cast<int32_t *>(hc + nb);
+        int32_t *hd = hi + nb;
+        if (hipMemcpyAsync(hb, dbits, nb * nn, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hc, dconv, nb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hi, dits, nb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hd, dbad, nb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            break;
+        std::memcpy(bits, hb, nb * nn);
+        std::memcpy(conv, hc, nb);
+        std::memcpy(its, hi, nb * 4);
+        std::memcpy(bad, hd, nb * 4);
+        rc_out = ATSC3_OK;
+    } while (0);
+    (void)hipStreamSynchronize(c->st);
+    release_ctx(c);
+    return rc_out;
+}
+
+uint64_t csr_hash(int32_t n, int32_t nc, int32_t nedges, const int32_t *cptr,
+                  const int32_t *cidx, const int32_t *vptr, const int32_t *vedge) {
+    CsrHash ch;
+    ch.add(cptr, (size_t)nc + 1);
+    ch.add(cidx, (size_t)nedges);
+    ch.add(vptr, (size_t)n + 1);
+    ch.add(vedge, (size_t)nedges);
+    return ch.h + (uint64_t)n * 0x100000001B3ull + (uint64_t)nc * 0x1000193ull +
+           (uint64_t)nedges;
+}
+
 }  // namespace
+
+extern "C" int32_t atsc3_dropin_engine(int32_t n, int32_t nc, int32_t nedges,
+                                       const int32_t *cptr, const int32_t *cidx,
+                                       const int32_t *vptr, const int32_t *vedge) {
+    if (n <= 0 || nc < 0 || nedges < 0 || !cptr || !cidx || !vptr || !vedge) return ATSC3_E_ARG;
+    const uint64_t h = csr_hash(n, nc, nedges, cptr, cidx, vptr, vedge);
+    std::lock_guard<std::mutex> lock(g_drop_mu);
+    load_table_file();
+    for (const ResEntry &x : g_res_file)
+        if (x.hash == h && x.n == n && x.nc == nc && x.ne == nedges) return 1;
+    return 0;
+}
 
 extern "C" int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, int32_t nc,
                                    int32_t nedges, const int32_t *cptr,
@@ -275,11 +491,16 @@ extern "C" int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, in
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return ATSC3_E_HIP;
 
-    uint64_t h = 0xcbf29ce484222325ull ^ ((uint64_t)n << 40) ^ ((uint64_t)nc << 20) ^ nedges;
-    h = mix_words(cptr, (size_t)nc + 1, h);
-    h = mix_words(cidx, (size_t)nedges, h);
-    h = mix_words(vptr, (size_t)n + 1, h);
-    h = mix_words(vedge, (size_t)nedges, h);
+    const uint64_t h = csr_hash(n, nc, nedges, cptr, cidx, vptr, vedge);
+    // an A/322 code whose resident tables the build wrote: the resident
+    // engine's FP32 mode (bit-exact with the reference kernel)
+    if (alpha >= 0.0f && iters < (1 << 24)) {
+        bool rfailed = false;
+        const ResCode *rc = resident_code(device, h, n, nc, nedges, &rfailed);
+        if (rfailed) return ATSC3_E_HIP;
+        if (rc)
+            return resident_dropin(rc, device, llr, nblk, n, iters, alpha, bits, conv, its, bad);
+    }
     bool failed = false;
     const int32_t *tabs =
         cached_tables(device, n, nc, nedges, h, cptr, cidx, vptr, vedge, &failed);

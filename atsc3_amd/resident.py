@@ -427,3 +427,60 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))
     return plan.run(llr.reshape(-1), (bits, conv, its, bad))
+
+
+# ---------------------------------------------------------------------------
+# Tables for the host-pointer drop-in (ldpc_minsum_f32): the C side cannot
+# build the resident engine's tables, so the build writes them, for every
+# A/322 code, to a file beside the library; the drop-in recognises a code by
+# csr_hash of the CSR arrays the caller passes.
+# ---------------------------------------------------------------------------
+DROPIN_TABLES = "libatsc3fec_tables.bin"
+_M64 = (1 << 64) - 1
+
+
+def csr_hash(n, cptr, cidx, vptr, vedge):
+    """Order-sensitive 64-bit hash of (n, nc, nedges, cptr, cidx, vptr,
+    vedge): the sum over the concatenated words w_i of mix(w_i, i), mix a
+    64-bit finaliser (xor-shift-multiply).  The drop-in computes the same
+    value (csrc/ldpc_csr.hip)."""
+    w = np.concatenate([np.asarray(a, np.int64).astype(np.uint32).astype(np.uint64)
+                        for a in (cptr, cidx, vptr, vedge)])
+    i = np.arange(len(w), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = w * np.uint64(0x9E3779B97F4A7C15) + i * np.uint64(0xC2B2AE3D27D4EB4F)
+        x ^= x >> np.uint64(31)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(29)
+        h = int(x.sum(dtype=np.uint64))
+    nc, ne = len(cptr) - 1, len(cidx)
+    return (h + n * 0x100000001B3 + nc * 0x1000193 + ne) & _M64
+
+
+def write_dropin_tables(path, codes=None):
+    """Serialise the FP32-mode resident tables of ``codes`` (default: all 24
+    A/322 codes), little-endian: "A3RT", version 1, count; per code the
+    csr_hash (u64), n, nc, nedges, ncg, nvg, nbe, max_dc, max_dv, nsc, nsv,
+    global_rows (i32), then each table of _TABLES as (length, words)."""
+    import struct
+    from .codes import RATES, get_code
+    if codes is None:
+        codes = [get_code(r, n) for n in (16200, 64800) for r in RATES]
+    out = [b"A3RT", struct.pack("<II", 1, len(codes))]
+    for code in codes:
+        cptr, cidx, vptr, vedge = code.csr()
+        h = _host_tables(code, True)
+        out.append(struct.pack("<Q", csr_hash(code.n, cptr, cidx, vptr, vedge)))
+        out.append(struct.pack("<11i", code.n, len(cptr) - 1, len(cidx), h["ncg"], h["nvg"],
+                               h["nbe"], h["max_dc"], h["max_dv"], h["nsc"], h["nsv"],
+                               h["global_rows"]))
+        for k in _TABLES:
+            a = np.ascontiguousarray(h[k], "<i4")
+            out.append(struct.pack("<i", len(a)))
+            out.append(a.tobytes())
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as f:
+        f.write(b"".join(out))
+    import os
+    os.replace(tmp, path)
+    return path

@@ -10,9 +10,9 @@
  *   - Every entry point returns 0 on success or a negative ATSC3_E* code; on
  *     error nothing is written to the outputs.
  *   - LLR sign convention: llr > 0 means bit 0 (lab/m6_bicm.py:213).
- *   - Thread safety: no global mutable state; concurrent calls on different
- *     streams with disjoint buffers are safe (the host-pointer drop-in
- *     allocates its own device buffers and stream per call).
+ *   - Thread safety: concurrent calls on different streams with disjoint
+ *     buffers are safe; the host-pointer drop-in keeps a mutex-guarded pool
+ *     of per-call contexts and a per-device table cache.
  */
 #ifndef ATSC3_FEC_H
 #define ATSC3_FEC_H
@@ -38,13 +38,25 @@ int32_t ldpc_kernel_abi(void);
 /* Replaces lab/ldpc_kernel.c:65-71 ldpc_minsum_f32 with identical arguments
  * and results (bits, conv, its, bad bit-exact).  Host pointers; the decode
  * runs on the current HIP device (H2D, kernel, D2H) and `work` is unused.
- * Returns -1 when a check degree exceeds 128, like the upstream kernel.   */
+ * Returns -1 when a check degree exceeds 128, like the upstream kernel,
+ * with nothing written.  Thread-safe: concurrent calls take separate
+ * pooled contexts (stream, grow-only device and pinned buffers), so after
+ * the first calls none allocates.                                          */
 int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, int32_t nc,
                         int32_t nedges, const int32_t *cptr,
                         const int32_t *cidx, const int32_t *vptr,
                         const int32_t *vedge, int32_t iters, float alpha,
                         uint8_t *bits, uint8_t *conv, int32_t *its,
                         int32_t *bad, float *work);
+
+/* Which engine ldpc_minsum_f32 uses for these tables: 1 = the resident
+ * engine's FP32 mode (an A/322 code found, by a hash of the four arrays,
+ * among the tables the build writes beside the library,
+ * libatsc3fec_tables.bin), 0 = the CSR phase kernels (any other graph).
+ * No device call; no upstream counterpart (a diagnostic of the drop-in). */
+int32_t atsc3_dropin_engine(int32_t n, int32_t nc, int32_t nedges,
+                            const int32_t *cptr, const int32_t *cidx,
+                            const int32_t *vptr, const int32_t *vedge);
 
 /* ------------------------------------------------------------------------
  * Device-pointer min-sum over the Tanner graph in CSR form.

@@ -167,3 +167,32 @@ def test_dropin_degree_limit_writes_nothing():
     cptr, cidx, vptr, vedge = code.csr()
     assert L.ldpc_minsum_f32(p(llr), 0, n, len(checks), len(cidx), p(cptr), p(cidx), p(vptr),
                              p(vedge), 50, 0.75, p(bits), p(conv), p(its), p(bad), p(work)) == 0
+
+
+def test_dropin_recognises_every_a322_code():
+    """The drop-in finds the resident engine's tables of all 24 A/322 codes
+    by the hash of the CSR arrays a caller passes (the C hash equals
+    atsc3_amd.resident.csr_hash), and routes any other graph to the CSR
+    kernels.  No device call."""
+    import numpy as np
+    from atsc3_amd.codes import RATES, get_code
+    from atsc3_amd.ldpc import _GenericCode
+    from atsc3_amd.resident import DROPIN_TABLES
+    L = _lib().lib()
+    if not os.path.exists(os.path.join(ROOT, "atsc3_amd", DROPIN_TABLES)):
+        pytest.skip("drop-in tables not built")
+    p = lambda a: a.ctypes.data
+    for n in (16200, 64800):
+        for rate in RATES:
+            cptr, cidx, vptr, vedge = get_code(rate, n).csr()
+            assert L.atsc3_dropin_engine(n, len(cptr) - 1, len(cidx), p(cptr), p(cidx), p(vptr),
+                                         p(vedge)) == 1, (n, rate)
+    # one edge moved: not an A/322 code any more
+    cptr, cidx, vptr, vedge = get_code("10/15", 64800).csr()
+    cidx2 = cidx.copy()
+    cidx2[[5, 6]] = cidx2[[6, 5]]
+    assert L.atsc3_dropin_engine(64800, len(cptr) - 1, len(cidx), p(cptr), p(cidx2), p(vptr),
+                                 p(vedge)) == 0
+    g = _GenericCode([[0, 1, 2], [2, 3], [1, 3, 4]], 5)
+    a, b, c, d = g.csr()
+    assert L.atsc3_dropin_engine(5, 3, len(b), p(a), p(b), p(c), p(d)) == 0

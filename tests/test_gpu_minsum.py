@@ -44,6 +44,32 @@ def test_host_dropin_matches_fixture(device):
                           name)
 
 
+def test_host_dropin_generic_graphs(device):
+    """Graphs that are not A/322 codes take the drop-in's CSR phase kernels:
+    equal to the C oracle on random graphs (degrees 2..12, and one check of
+    degree 128, the upstream limit), iters 0, 1 and 40."""
+    from atsc3_amd import _lib, ldpc
+    from atsc3_amd.ldpc import _GenericCode
+    rng = np.random.default_rng(55)
+    for n, nc, lo, hi in ((600, 300, 2, 12), (4000, 1800, 3, 7)):
+        checks = [sorted(rng.choice(n, rng.integers(lo, hi + 1), replace=False).tolist())
+                  for _ in range(nc)]
+        if n == 4000:
+            checks[7] = sorted(rng.choice(n, 128, replace=False).tolist())
+        code = _GenericCode(checks, n)
+        cptr, cidx, vptr, vedge = code.csr()
+        p = lambda a: a.ctypes.data
+        assert _lib.lib().atsc3_dropin_engine(n, nc, len(cidx), p(cptr), p(cidx), p(vptr),
+                                              p(vedge)) == 0
+        llr = (rng.standard_normal((5, n)) * 2 + 1.5).astype(np.float32)
+        for iters in (0, 1, 40):
+            print(f"[gpu] drop-in generic graph n={n} iters={iters}", flush=True)
+            got = ldpc.minsum_f32_host(llr, checks, n, iters, 0.75)
+            want = OL.minsum_f32(llr, code, iters, 0.75)
+            for x, y in zip(got, want):
+                assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
 def test_host_dropin_threaded(device):
     """The upstream's calling pattern (lab/m3_ldpc.py:485-506,
     lab/m9_fast.py:893-925): a thread pool of 8 workers, each calling
