@@ -290,6 +290,30 @@ def dropin_bench(llr_dev, code, nblk=512, threads=8, per_call=8, reps=3):
     return out
 
 
+def utc_bench(llr_dev, code, nblk=512, reps=3):
+    """The upstream's other decode contract (GpuMinSum.decode,
+    lab/m9_gpu.py:138-211; min_sum_decode, lab/m3_ldpc.py:245-286) on the
+    same blocks, device-resident, in FP32 and FP64: blocks/s and mean
+    iterations (its >= 1 under this contract)."""
+    import torch
+    from atsc3_amd import ldpc
+    out = {"blocks": nblk}
+    for name, dt in (("float32", torch.float32), ("float64", torch.float64)):
+        t = llr_dev[:nblk].to(dt).contiguous()
+        r = ldpc.decode_utc_dev(t, code, 50, 0.75)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = ldpc.decode_utc_dev(t, code, 50, 0.75)
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / reps
+        out[name] = {"blocks_per_s": round(nblk / secs, 1), "ms": round(secs * 1e3, 3),
+                     "mean_its": round(float(r[2].float().mean().item()), 3)}
+    log(f"[bench] update-then-check (GpuMinSum contract) {nblk} blocks: FP32 "
+        f"{out['float32']['blocks_per_s']:.0f} blocks/s, FP64 {out['float64']['blocks_per_s']:.0f}")
+    return out
+
+
 def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     """Uneven frames: 74 blocks per frame mixing 64800-bit rates 6/8/10/12/15
     and 16200-bit rates 4/9/15 on QPSK..256-NUC, per-block AWGN Es/N0 drawn
@@ -974,6 +998,7 @@ def main():
         f"(one stream {ms32s:.3f} ms/step)")
     llr_w, _ = dm.demap_dev(z, csi=w, sigma2=n0)
     plain["dropin"] = dropin_bench(llr_w, code)
+    plain["update_then_check"] = utc_bench(llr_w, code)
     del llr_w
 
     # -- message-storage forms at the same operating point ----------------------
