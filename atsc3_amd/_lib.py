@@ -102,9 +102,9 @@ _SIGS = {
     "atsc3_demap_raw_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                             c_vp],
     "atsc3_minsum_utc_ws_bytes": [c_i32, c_i32, c_i32, c_i32],
-    "atsc3_minsum_utc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
-                             c_i32, c_i32, c_f64, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp, c_vp,
-                             c_vp],
+    "atsc3_minsum_utc_dev": [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+                             c_vp, c_i32, c_i32, c_f64, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp,
+                             c_vp, c_vp],
     "atsc3_demap_dev": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp,
                         c_vp, c_vp],
 }

@@ -22,6 +22,18 @@
 // of T (no contraction), so FP32 reproduces the torch decoder's roundings
 // and FP64 numpy's.
 //
+// A check whose magnitudes include NaN (NaN LLRs, inf - inf) follows one of
+// two rules, where the upstream's functions differ (NaN-free checks are the
+// same under both):
+//   * UTC_NAN_TORCH (GpuMinSum, lab/m9_gpu.py:160-166): min1 = amin, which
+//     propagates NaN, so every message is NaN except edge 0's, which takes
+//     min2 = amin of the other entries: the smallest of them when edge 0
+//     holds the only NaN, else NaN;
+//   * UTC_NAN_NUMPY (min_sum_decode, lab/m3_ldpc.py:264-270): min1 <= min2
+//     are the two smallest non-NaN entries (np.partition orders NaN last;
+//     NaN when fewer exist), and np.argmin's choice -- the first NaN edge --
+//     takes min2, every other edge min1.
+//
 // Schedule: one launch pair per iteration, over (codeword, check) and
 // (codeword, variable) threads.  The check launch of iteration it gathers
 // tot once for both the syndrome of the posteriors after `it` updates (for
@@ -31,6 +43,8 @@
 #include "fec_common.h"
 
 #define UTC_WG 256
+#define UTC_NAN_TORCH 0
+#define UTC_NAN_NUMPY 1
 
 namespace atsc3 {
 
@@ -72,7 +86,8 @@ __device__ __forceinline__ void two_min(T a, T &lo1, T &lo2) {
 // update (it < iters).  KC = the compiled kernel's contract
 // (lab/ldpc_kernel.c:82-143): no row padding, and the edge that first
 // attains min1 in the strict-< scan takes min2 (a NaN magnitude never wins).
-template <typename T, bool KC>
+// Otherwise the rows are padded and NR (UTC_NAN_*) rules NaN.
+template <typename T, bool KC, int NR>
 __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
     const int blk = blockIdx.y;
     if (a.done_it[blk] < it) return;
@@ -88,6 +103,7 @@ __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
             unsigned par = 0u;
             T lo1 = (T)INFINITY, lo2 = (T)INFINITY;
             int first = e0;
+            int nanpos = -1, nnan = 0;   // first NaN-magnitude edge, NaN count
             for (int e = e0; e < e1; ++e) {
                 const T tv = src[a.cidx[e]];
                 const T x = tv - (it == 0 ? zero : m[e]);
@@ -103,19 +119,46 @@ __global__ __launch_bounds__(UTC_WG) void utc_check(UtcArgs<T> a, int it) {
                         lo2 = mag;
                     }
                 } else {
-                    two_min(mag, lo1, lo2);
+                    two_min(mag, lo1, lo2);   // a NaN compares below nothing
+                    if (mag != mag) {
+                        if (nanpos < 0) nanpos = e;
+                        ++nnan;
+                    }
                 }
             }
+            const int npad = KC ? 0 : a.dmax - (e1 - e0);
             if constexpr (!KC) {
-                const int npad = a.dmax - (e1 - e0);
                 if (npad >= 1) two_min(big, lo1, lo2);
                 if (npad >= 2) two_min(big, lo1, lo2);
             }
             const T s1 = a.alpha * lo1, s2 = a.alpha * lo2;
+            const T qnan = (T)__builtin_nan("");
+            // NaN rows: the values of the non-minimal and the min2 edge
+            T t1 = s1, t2 = s2;
+            int take2 = nanpos;
+            if (!KC && nanpos >= 0) {
+                if constexpr (NR == UTC_NAN_TORCH) {
+                    // edge 0 takes amin of the others (their smallest when it
+                    // holds the only NaN, so s1), every other edge NaN
+                    t1 = qnan;
+                    t2 = (nanpos == e0 && nnan == 1) ? s1 : qnan;
+                    take2 = e0;
+                } else {
+                    const int nok = (e1 - e0) - nnan + npad;
+                    t1 = nok >= 1 ? s1 : qnan;
+                    t2 = nok >= 2 ? s2 : qnan;
+                }
+            }
             for (int e = e0; e < e1; ++e) {
                 const T x = src[a.cidx[e]] - (it == 0 ? zero : m[e]);
                 const T mag = x < zero ? -x : x;
-                const T v = (KC ? e == first : mag == lo1) ? s2 : s1;
+                T v;
+                if (KC)
+                    v = e == first ? s2 : s1;
+                else if (nanpos < 0)
+                    v = mag == lo1 ? s2 : s1;
+                else
+                    v = e == take2 ? t2 : t1;
                 m[e] = (par ^ (unsigned)(x < zero)) ? -v : v;
             }
         } else {
@@ -184,7 +227,7 @@ static size_t utc_ws_bytes(int nblk, int n, int nedges, size_t tsize) {
            align256((size_t)nblk * 3 * sizeof(int));
 }
 
-template <typename T, bool KC>
+template <typename T, bool KC, int NR>
 static int32_t utc_launch(const T *llr, int nblk, int n, int nc, int nedges, const int *cptr,
                           const int *cidx, const int *vptr, const int *vedge, int dmax,
                           int iters, double alpha, void *ws, uint8_t *bits, uint8_t *conv,
@@ -222,7 +265,7 @@ static int32_t utc_launch(const T *llr, int nblk, int n, int nc, int nedges, con
     hipLaunchKernelGGL(utc_init, dim3((nblk + 255) / 256), dim3(256), 0, st, a.done_it, a.cnt,
                        nblk);
     for (int it = 0; it <= iters; ++it) {
-        hipLaunchKernelGGL((utc_check<T, KC>), cg, dim3(UTC_WG), 0, st, a, it);
+        hipLaunchKernelGGL((utc_check<T, KC, NR>), cg, dim3(UTC_WG), 0, st, a, it);
         hipLaunchKernelGGL((utc_var<T, KC>), vg, dim3(UTC_WG), 0, st, a, it);
     }
     return launch_status();
@@ -234,8 +277,9 @@ int32_t csr_phase_kc_f32(const float *llr, int nblk, int n, int nc, int nedges, 
                          const int *cidx, const int *vptr, const int *vedge, int iters,
                          float alpha, void *ws, uint8_t *bits, uint8_t *conv, int *its,
                          int *bad, hipStream_t st) {
-    return utc_launch<float, true>(llr, nblk, n, nc, nedges, cptr, cidx, vptr, vedge, 0, iters,
-                                   (double)alpha, ws, bits, conv, its, bad, st);
+    return utc_launch<float, true, UTC_NAN_TORCH>(llr, nblk, n, nc, nedges, cptr, cidx, vptr,
+                                                  vedge, 0, iters, (double)alpha, ws, bits,
+                                                  conv, its, bad, st);
 }
 
 }  // namespace atsc3
@@ -246,15 +290,16 @@ extern "C" int64_t atsc3_minsum_utc_ws_bytes(int32_t nblk, int32_t n, int32_t ne
     return (int64_t)atsc3::utc_ws_bytes(nblk, n, nedges, dtype == 1 ? 8 : 4);
 }
 
-extern "C" int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t nblk,
-                                        int32_t n, int32_t nc, int32_t nedges,
+extern "C" int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t nan_rule,
+                                        int32_t nblk, int32_t n, int32_t nc, int32_t nedges,
                                         const int32_t *cptr, const int32_t *cidx,
                                         const int32_t *vptr, const int32_t *vedge,
                                         int32_t dmax, int32_t iters, double alpha, void *ws,
                                         int64_t ws_bytes, uint8_t *bits, uint8_t *conv,
                                         int32_t *its, int32_t *bad, void *stream) {
     if (nblk < 0 || n <= 0 || nc < 0 || nedges < 0 || iters < 0 || dmax < 0 ||
-        (dtype != 0 && dtype != 1) || nblk > 65535)
+        (dtype != 0 && dtype != 1) || (nan_rule != UTC_NAN_TORCH && nan_rule != UTC_NAN_NUMPY) ||
+        nblk > 65535)
         return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!llr || !cptr || !cidx || !vptr || !vedge || !ws || !bits || !conv || !its || !bad)
@@ -262,11 +307,13 @@ extern "C" int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t 
     if (ws_bytes < (int64_t)atsc3::utc_ws_bytes(nblk, n, nedges, dtype == 1 ? 8 : 4))
         return ATSC3_E_SIZE;
     hipStream_t st = atsc3::as_stream(stream);
+#define UTC_GO(T, NR)                                                                          \
+    atsc3::utc_launch<T, false, NR>(static_cast<const T *>(llr), nblk, n, nc, nedges, cptr,     \
+                                    cidx, vptr, vedge, dmax, iters, alpha, ws, bits, conv, its, \
+                                    bad, st)
     if (dtype == 1)
-        return atsc3::utc_launch<double, false>(static_cast<const double *>(llr), nblk, n, nc, nedges,
-                                         cptr, cidx, vptr, vedge, dmax, iters, alpha, ws, bits,
-                                         conv, its, bad, st);
-    return atsc3::utc_launch<float, false>(static_cast<const float *>(llr), nblk, n, nc, nedges, cptr,
-                                    cidx, vptr, vedge, dmax, iters, alpha, ws, bits, conv, its,
-                                    bad, st);
+        return nan_rule == UTC_NAN_NUMPY ? UTC_GO(double, UTC_NAN_NUMPY)
+                                         : UTC_GO(double, UTC_NAN_TORCH);
+    return nan_rule == UTC_NAN_NUMPY ? UTC_GO(float, UTC_NAN_NUMPY) : UTC_GO(float, UTC_NAN_TORCH);
+#undef UTC_GO
 }

@@ -15,7 +15,8 @@ Two decode contracts, as the upstream has them:
   padded with 1e9 entries.  Served by ``decode_utc_dev`` in FP32 or FP64,
   bit-exact with those functions (``GpuMinSum(dtype="float32")``,
   ``GpuMinSum(dtype="float64")`` -- the upstream default -- and the float64
-  functions).
+  functions), also on NaN / inf LLRs, where GpuMinSum and the numpy
+  functions follow different rules and each mirror follows its own.
 
 Mirrors (same arguments, same return tuples):
   * ``min_sum_decode(llr, checks, n, iters=50, alpha=0.75)``
@@ -257,18 +258,28 @@ def _to_numpy(bits, conv, its, bad):
 
 
 UTC_DTYPES = {"float32": 0, "float64": 1}
+# how a check holding NaN magnitudes updates: GpuMinSum's torch.amin rule or
+# min_sum_decode's np.partition / np.argmin rule (identical without NaN)
+NAN_RULES = {"torch": 0, "numpy": 1}
 
 
-def decode_utc_dev(llr, code, iters=50, alpha=0.75):
+def decode_utc_dev(llr, code, iters=50, alpha=0.75, nan_rule="torch"):
     """Update-then-check decode of device LLRs (nblk, n), float32 or float64
     (the arithmetic follows the tensor's dtype); no host sync.
 
-    Contract of lab/m9_gpu.py:138-211 (float32) and lab/m3_ldpc.py:245-286
-    (float64): its >= 1 for iters >= 1, tot = ((llr + m_1) + m_2) + ... in
-    ascending check order, rows padded to the largest check degree with
-    1e9 entries; iters == 0 gives bits = llr < 0, conv 0, its 0, bad -1.
+    Contract of lab/m9_gpu.py:138-211 (GpuMinSum, float32 / float64) and
+    lab/m3_ldpc.py:245-286 (min_sum_decode, float64): its >= 1 for
+    iters >= 1, tot = ((llr + m_1) + m_2) + ... in ascending check order,
+    rows padded to the largest check degree with 1e9 entries; iters == 0
+    gives bits = llr < 0, conv 0, its 0, bad -1.  ``nan_rule`` "torch"
+    follows GpuMinSum where a check's magnitudes include NaN (torch.amin
+    propagates it, lab/m9_gpu.py:160-166), "numpy" follows min_sum_decode
+    (np.partition sorts NaN last and np.argmin picks the first NaN,
+    lab/m3_ldpc.py:264-270).
     Returns device tensors (bits u8 (nblk, n), conv u8, its i32, bad i32).
     """
+    if nan_rule not in NAN_RULES:
+        raise ValueError(f"nan_rule must be one of {tuple(NAN_RULES)}")
     if not (isinstance(llr, torch.Tensor) and llr.is_cuda):
         raise TypeError("decode_utc_dev needs a CUDA (HIP) tensor")
     if llr.dtype not in (torch.float32, torch.float64):
@@ -291,7 +302,8 @@ def decode_utc_dev(llr, code, iters=50, alpha=0.75):
         nb = hi - lo
         wsb = int(_lib.lib().atsc3_minsum_utc_ws_bytes(nb, n, tabs.nedges, dt))
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
-        _lib.call("atsc3_minsum_utc_dev", llr[lo:hi].data_ptr(), dt, nb, n, tabs.nc,
+        _lib.call("atsc3_minsum_utc_dev", llr[lo:hi].data_ptr(), dt, NAN_RULES[nan_rule], nb,
+                  n, tabs.nc,
                   tabs.nedges, tabs.cptr.data_ptr(), tabs.cidx.data_ptr(),
                   tabs.vptr.data_ptr(), tabs.vedge.data_ptr(), tabs.dmax, int(iters),
                   float(alpha), ws.data_ptr(), wsb, bits[lo:hi].data_ptr(),
@@ -308,7 +320,7 @@ def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float6
     the upstream's float32 route), bit-exact, on the fast engines (msg_dtype
     selects the FP16 / INT8 message forms).  dtype float64 (the upstream's
     default): the update-then-check float64 decoder, row-exact with
-    lab/m3_ldpc.py:245-286, run in FP64 on the GPU.
+    lab/m3_ldpc.py:245-286 (NaN LLRs included), run in FP64 on the GPU.
     """
     dt = np.dtype(dtype)
     if dt not in (np.float32, np.float64):
@@ -325,7 +337,7 @@ def min_sum_decode_batch(lam2d, checks, n, iters=50, alpha=0.75, dtype=np.float6
     else:
         t = torch.from_numpy(np.ascontiguousarray(lam, dt)).to(device)
     if dt == np.float64:
-        return _to_numpy(*decode_utc_dev(t, code, iters, alpha))
+        return _to_numpy(*decode_utc_dev(t, code, iters, alpha, "numpy"))
     return _to_numpy(*decode_dev(t, code, iters, alpha, msg_dtype, engine))
 
 
@@ -342,7 +354,7 @@ class GpuMinSum:
 
     dtype "float32" / "float64" (the upstream default): the upstream's
     update-then-check contract in that precision, bit-exact with
-    GpuMinSum.decode (lab/m9_gpu.py:138-211).  contract="kernel" instead
+    GpuMinSum.decode (lab/m9_gpu.py:138-211), its NaN propagation included.  contract="kernel" instead
     serves the compiled kernel's contract (lab/ldpc_kernel.c:82-143) on the
     fast engines, where msg_dtype selects the FP16 / INT8 message forms.
     """
@@ -383,7 +395,7 @@ class GpuMinSum:
         if t.dim() != 2 or t.shape[1] != self.n:
             raise ValueError(f"LLRs must be (nblk, {self.n})")
         if self.contract == "upstream":
-            out = _to_numpy(*decode_utc_dev(t, self.code, iters, alpha))
+            out = _to_numpy(*decode_utc_dev(t, self.code, iters, alpha, "torch"))
         else:
             out = _to_numpy(*decode_dev(t, self.code, iters, alpha, self.msg_dtype,
                                         self.engine))

@@ -82,8 +82,14 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
  * check's two-minimum also sees (dmax - degree) pad entries of magnitude
  * 1e9 (the upstream's rows padded to the largest degree dmax); alpha is
  * rounded to T.  iters == 0: bits = llr < 0, conv 0, its 0, bad -1.
- * Finite LLRs whose sums stay finite (the upstream's own implementations
- * treat NaN differently from one another).
+ *   nan_rule how a check whose magnitudes include NaN (NaN LLRs, inf - inf)
+ *            updates, where the upstream's functions differ:
+ *            0 = GpuMinSum's (lab/m9_gpu.py:160-166: amin propagates NaN, so
+ *            every message is NaN but edge 0's, which takes the smallest
+ *            other entry when it holds the only NaN);
+ *            1 = min_sum_decode's (lab/m3_ldpc.py:264-270: the two smallest
+ *            non-NaN entries, the first NaN edge taking the second).
+ *            Checks without NaN are the same under both.
  *   llr      [nblk][n] float (dtype 0) or double (dtype 1), device
  *   cptr..vedge  CSR tables as ldpc_minsum_f32 (any degrees)
  *   dmax     the largest check degree of the code (the row padding width)
@@ -92,8 +98,8 @@ int32_t atsc3_minsum_csr_f32_dev(const float *llr, int32_t nblk, int32_t n,
  * ------------------------------------------------------------------------ */
 int64_t atsc3_minsum_utc_ws_bytes(int32_t nblk, int32_t n, int32_t nedges,
                                   int32_t dtype);
-int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t nblk,
-                             int32_t n, int32_t nc, int32_t nedges,
+int32_t atsc3_minsum_utc_dev(const void *llr, int32_t dtype, int32_t nan_rule,
+                             int32_t nblk, int32_t n, int32_t nc, int32_t nedges,
                              const int32_t *cptr, const int32_t *cidx,
                              const int32_t *vptr, const int32_t *vedge,
                              int32_t dmax, int32_t iters, double alpha,

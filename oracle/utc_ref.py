@@ -19,16 +19,27 @@ from the compiled kernel's contract (lab/ldpc_kernel.c:82-143) in three ways:
   lab/m3_ldpc.py:259-260); they change a message only when every real
   |x| of the check is >= 1e9.
 
-Contract covered: finite LLRs whose sums stay finite.  (For NaN inputs the
-upstream's own implementations disagree with each other: numpy's partition /
-argmin and torch's amin treat NaN differently.)
+Two rules for a check row holding NaN magnitudes (NaN LLRs, or inf - inf
+sums), where the upstream's own implementations differ; on NaN-free rows
+they agree:
+
+* rule "torch" (GpuMinSum at float32 and float64, lab/m9_gpu.py:160-166):
+  min1 = amin, which propagates NaN; the edge that takes min2 is the first
+  with |x| == min1 (edge 0 when min1 is NaN, since no entry compares equal);
+  min2 = amin over the other entries, NaN-propagating;
+* rule "numpy" (min_sum_decode and min_sum_decode_batch(float64),
+  lab/m3_ldpc.py:264-270, 565-569): min1, min2 = the two smallest entries
+  in np.partition's order, which puts NaN last; the edge that takes min2 is
+  np.argmin's, which is the first NaN edge when there is one.
 
 iters == 0 follows min_sum_decode: bits = llr < 0, conv 0, its 0, bad -1.
 
 Pinned by tests/golden/utc_*.npz: outputs of the upstream's own
-``min_sum_decode`` (float64) and ``GpuMinSum`` (float32, run on torch's CPU
-device, whose elementwise IEEE operations and exact min/argmax match its
-device path) on the same LLRs, written by tests/golden/make_ref_vectors.py.
+``min_sum_decode`` (float64, rule "numpy") and ``GpuMinSum`` (float32 and
+float64, rule "torch", run on torch's CPU device, whose elementwise IEEE
+operations and exact min/argmax match its device path) on the same LLRs,
+written by tests/golden/make_ref_vectors.py; the *_nonfinite and infnan sets
+hold NaN and +-inf LLRs.
 """
 from __future__ import annotations
 
@@ -65,13 +76,20 @@ def var_columns(idx, mask, n):
     return vcol
 
 
-def utc_minsum(llr, checks, n, iters=50, alpha=0.75, dtype=np.float32, _plant=None):
-    """(nblk, n) -> (bits u8, conv bool, its i32, bad i32).
+RULES = ("torch", "numpy")
+
+
+def utc_minsum(llr, checks, n, iters=50, alpha=0.75, dtype=np.float32, rule="torch",
+               _plant=None):
+    """(nblk, n) -> (bits u8, conv bool, its i32, bad i32); ``rule`` picks
+    the NaN handling of the check update ("torch" or "numpy", above).
 
     _plant (negative controls for the tests only): "zero_first" sums
     (0 + m_1 + ...) then adds llr (the compiled kernel's association);
     "no_pad" drops the 1e9 row padding; "check_first" checks the channel
     values before the first update."""
+    if rule not in RULES:
+        raise ValueError(f"rule must be one of {RULES}")
     dt = np.dtype(dtype)
     lam = np.ascontiguousarray(llr, dt).reshape(-1, n)
     nb = lam.shape[0]
@@ -102,10 +120,22 @@ def utc_minsum(llr, checks, n, iters=50, alpha=0.75, dtype=np.float32, _plant=No
         sg = v < 0
         parity = (np.sum(sg & mask, axis=2) & 1).astype(bool)
         mag = np.abs(v)
-        m1 = mag.min(axis=2, keepdims=True)
-        first = np.argmax(mag == m1, axis=2)
-        onehot = ar[None, None, :] == first[:, :, None]
-        m2 = np.where(onehot, dt.type(np.inf), mag).min(axis=2, keepdims=True)
+        if rule == "torch":
+            # np.min propagates NaN as torch.amin does; argmax of an all-False
+            # row is 0
+            m1 = mag.min(axis=2, keepdims=True)
+            first = np.argmax(mag == m1, axis=2)
+            onehot = ar[None, None, :] == first[:, :, None]
+            m2 = np.where(onehot, dt.type(np.inf), mag).min(axis=2, keepdims=True)
+        else:
+            # ascending with NaN last; the first NaN edge, else the first
+            # minimal one, takes min2
+            srt = np.sort(mag, axis=2)
+            m1, m2 = srt[:, :, 0:1], srt[:, :, 1:2]
+            isnan = np.isnan(mag)
+            first = np.where(isnan.any(axis=2), np.argmax(isnan, axis=2),
+                             np.argmax(mag == m1, axis=2))
+            onehot = ar[None, None, :] == first[:, :, None]
         out = np.where(onehot, m2, m1)
         sgn = np.where(parity[:, :, None] ^ sg, dt.type(-1.0), dt.type(1.0))
         mm = np.where(mask, (a * sgn) * out, dt.type(0.0))

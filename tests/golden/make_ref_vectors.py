@@ -15,9 +15,10 @@ DATA fixtures for functions nothing else pins byte for byte:
                   lab/m3_ldpc.py:166-212 parity_check) for every A/322 code.
   utc_<name>.npz  outputs (bits packed, conv, its, bad) of the update-then-
                   check decoders on the LLRs of golden fixture <name>:
-                  GpuMinSum.decode at float32 (lab/m9_gpu.py:84-211, run on
-                  torch's CPU device) and min_sum_decode at float64
-                  (lab/m3_ldpc.py:245-286, one call per block).  Sets marked
+                  GpuMinSum.decode at float32 and at float64 (lab/m9_gpu.py:
+                  84-211, run on torch's CPU device) and min_sum_decode at
+                  float64 (lab/m3_ldpc.py:245-286, one call per block); the
+                  two float64 functions differ only where LLRs are NaN.  Sets marked
                   "f64" regenerate float64 LLRs (FP64 demap of seeded AWGN
                   cells) that are not float32 values, checked by sha256.
 
@@ -50,7 +51,11 @@ FULL_MAPS = [(64800, "256QAM", "10/15"), (64800, "256QAM", "6/15"),
 
 # golden min-sum fixtures whose LLRs feed the update-then-check vectors
 UTC_FROM = ["s16k_r3_smoke", "s16k_r3_zeros", "g64k_r10_nuc20", "s16k_r6_ties",
-            "s16k_r6_ties_a85", "g16k_r4", "g64k_r4", "s16k_r9_near", "g16k_r11"]
+            "s16k_r6_ties_a85", "g16k_r4", "g64k_r4", "s16k_r9_near", "g16k_r11",
+            "s16k_r3_infnan"]
+# the LLRs of a fixture with NaN and +-inf planted at seeded positions:
+# (fixture, count of each, seed)
+UTC_NONFINITE = [("g16k_r4", 40, 5), ("s16k_r6_ties", 25, 6)]
 # the LLRs of a fixture scaled so that most |x| exceed the 1e9 row padding
 # (the padding then decides messages): (fixture, factor)
 UTC_SCALED = [("g16k_r4", 4e9), ("s16k_r6_ties", 2e9)]
@@ -78,6 +83,19 @@ def tiny_sets():
               for _ in range(nc)]
     llr = np.sign(rng.standard_normal((4, n))) * 2.0 ** rng.uniform(-20, 70, (4, n))
     out.append(("tiny_wide", checks, n, llr, 12, 0.75))
+    return out
+
+
+def nonfinite(llr, count, seed):
+    """``count`` NaN, +inf and -inf each at seeded distinct positions of
+    every block (float32)."""
+    out = np.array(llr, np.float32)
+    rng = np.random.default_rng(seed)
+    for row in out:
+        pos = rng.choice(row.size, 3 * count, replace=False)
+        row[pos[:count]] = np.nan
+        row[pos[count:2 * count]] = np.inf
+        row[pos[2 * count:]] = -np.inf
     return out
 
 
@@ -132,6 +150,9 @@ def main(upstream):
         g = UG.GpuMinSum(checks, n, iters=iters, alpha=alpha, device="cpu",
                          dtype="float32")
         b32, c32, i32, d32 = g.decode(np.asarray(llr, np.float64))
+        g64 = UG.GpuMinSum(checks, n, iters=iters, alpha=alpha, device="cpu",
+                           dtype="float64")
+        t64 = g64.decode(np.asarray(llr, np.float64))
         rows = [LD.min_sum_decode(np.asarray(r, np.float64), checks, n, iters, alpha)
                 for r in llr]
         b64 = np.stack([r[0] for r in rows]).astype(np.uint8)
@@ -148,9 +169,13 @@ def main(upstream):
             f32_bits=np.packbits(np.asarray(b32, np.uint8), axis=1),
             f32_conv=np.asarray(c32, bool), f32_its=np.asarray(i32, np.int32),
             f32_bad=np.asarray(d32, np.int32),
-            f64_bits=np.packbits(b64, axis=1), f64_conv=c64, f64_its=i64, f64_bad=d64)
+            f64_bits=np.packbits(b64, axis=1), f64_conv=c64, f64_its=i64, f64_bad=d64,
+            t64_bits=np.packbits(np.asarray(t64[0], np.uint8), axis=1),
+            t64_conv=np.asarray(t64[1], bool), t64_its=np.asarray(t64[2], np.int32),
+            t64_bad=np.asarray(t64[3], np.int32))
         print(f"utc_{name}: f32 its {np.asarray(i32).tolist()} conv "
-              f"{np.asarray(c32).astype(int).tolist()}; f64 its {i64.tolist()}")
+              f"{np.asarray(c32).astype(int).tolist()}; f64 its {i64.tolist()} conv "
+              f"{c64.astype(int).tolist()}; torch f64 its {np.asarray(t64[2]).tolist()}")
 
     for name in UTC_FROM:
         code, llr, iters, alpha, _ = F.load(name)
@@ -162,6 +187,12 @@ def main(upstream):
         run(f"{name}_x{factor:.0e}".replace("+", ""), code.n, code.rate, llr, iters, alpha,
             dict(source=name, scale=factor, n=code.n, rate=code.rate, iters=iters,
                  alpha=alpha))
+    for name, cnt, seed in UTC_NONFINITE:
+        code, llr, iters, alpha, _ = F.load(name)
+        llr = nonfinite(llr, cnt, seed)
+        run(f"{name}_nonfinite", code.n, code.rate, llr, iters, alpha,
+            dict(source=name, nonfinite=dict(count=cnt, seed=seed), n=code.n, rate=code.rate,
+                 iters=iters, alpha=alpha))
     for name, checks, n, llr, iters, alpha in tiny_sets():
         cn = [len(c) for c in checks]
         run(name, n, None, llr, iters, alpha,

@@ -1,12 +1,14 @@
 """GPU update-then-check decoder (csrc/ldpc_utc.hip) against the upstream's
-own decoders: GpuMinSum.decode at float32 (lab/m9_gpu.py:84-211) and
-min_sum_decode / min_sum_decode_batch at float64 (lab/m3_ldpc.py:245-286,
+own decoders: GpuMinSum.decode at float32 and float64 (lab/m9_gpu.py:84-211)
+and min_sum_decode / min_sum_decode_batch at float64 (lab/m3_ldpc.py:245-286,
 509-630), bit-exact on bits, conv, its and bad.
 
 Fixtures: tests/golden/utc_*.npz (the upstream's outputs on golden LLR sets,
-scaled sets past the 1e9 row padding, float64-native sets and hand-made
-graphs where the association of the variable sum decides a bit); live cases
-against the oracle that those fixtures pin (oracle/utc_ref.py).
+scaled sets past the 1e9 row padding, float64-native sets, hand-made graphs
+where the association of the variable sum decides a bit, and sets holding
+NaN and +-inf LLRs, on which GpuMinSum and min_sum_decode differ); live cases
+against the oracle that those fixtures pin (oracle/utc_ref.py), under both
+NaN rules.
 """
 import numpy as np
 import pytest
@@ -35,7 +37,7 @@ def test_utc_fixtures(device, name):
     _eq(g32.decode(llr), want["f32"], f"{name} GpuMinSum float32")
     g64 = ldpc.GpuMinSum(checks, code.n, iters=iters, alpha=alpha, device=device)
     assert g64.dtype_name == "float64"                 # the upstream default
-    _eq(g64.decode(llr), want["f64"], f"{name} GpuMinSum float64")
+    _eq(g64.decode(llr), want["t64"], f"{name} GpuMinSum float64")
     _eq(ldpc.min_sum_decode_batch(llr, checks, code.n, iters, alpha), want["f64"],
         f"{name} min_sum_decode_batch float64")
     b, c, i, d = ldpc.min_sum_decode(llr[0], checks, code.n, iters, alpha)
@@ -63,10 +65,40 @@ def test_utc_live_vs_oracle(device, rate, n, snrs, iters, alpha):
     for dt, tdt in ((np.float32, torch.float32), (np.float64, torch.float64)):
         want = utc_minsum(llr, code.checks, n, iters, alpha, dt)
         t = torch.from_numpy(llr.astype(dt)).to(device)
-        got = ldpc.decode_utc_dev(t, code, iters, alpha)
+        for rule in ("torch", "numpy"):     # identical on finite LLRs
+            got = ldpc.decode_utc_dev(t, code, iters, alpha, rule)
+            torch.cuda.synchronize()
+            _eq(_host(got), want, f"{rate} {np.dtype(dt).name} {rule}")
+
+
+def _host(got):
+    return ([x.cpu().numpy() for x in got[:1]] + [got[1].cpu().numpy().astype(bool)]
+            + [x.cpu().numpy() for x in got[2:]])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_utc_nonfinite_live_vs_oracle(device, dtype):
+    """NaN and +-inf LLRs at seeded positions, scattered over several blocks
+    in different numbers (a check then holds 0, 1 or several NaN entries,
+    or inf - inf sums), both NaN rules, against the pinned oracle."""
+    import torch
+    from atsc3_amd import ldpc
+    code = get_code("8/15", 16200)
+    cw = synth.codewords("8/15", 16200, 6, 41)
+    llr = synth.llr_bpsk_like(cw, 2.2, 42).astype(np.float64)
+    rng = np.random.default_rng(43)
+    for b, cnt in enumerate((0, 1, 3, 12, 60, 400)):
+        pos = rng.choice(16200, 3 * cnt, replace=False)
+        llr[b, pos[:cnt]] = np.nan
+        llr[b, pos[cnt:2 * cnt]] = np.inf
+        llr[b, pos[2 * cnt:]] = -np.inf
+    t = torch.from_numpy(llr.astype(dtype)).to(device)
+    for rule in ("torch", "numpy"):
+        print(f"[gpu] utc nonfinite {np.dtype(dtype).name} {rule}", flush=True)
+        want = utc_minsum(llr, code.checks, 16200, 20, 0.75, dtype, rule)
+        got = ldpc.decode_utc_dev(t, code, 20, 0.75, rule)
         torch.cuda.synchronize()
-        _eq([x.cpu().numpy() for x in got[:1]] + [got[1].cpu().numpy().astype(bool)]
-            + [x.cpu().numpy() for x in got[2:]], want, f"{rate} {np.dtype(dt).name}")
+        _eq(_host(got), want, f"nonfinite {np.dtype(dtype).name} {rule}")
 
 
 def test_utc_iters_zero_and_properties(device):
