@@ -41,6 +41,12 @@ _SIGS = {
     "atsc3_minsum_batch_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
                                c_i32, c_f32, c_vp, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp,
                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_batch_xcd_ctl_words": [c_i32, c_i32],
+    "atsc3_minsum_batch_xcd_max_grid": [c_i32],
+    "atsc3_minsum_batch_xcd_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_i32,
+                                   c_i32, c_f32, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32,
+                                   c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                   c_vp],
     "atsc3_unrolled_check_fwd_f32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                                      c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
     "atsc3_unrolled_check_fwd_f64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
@@ -153,6 +159,7 @@ def lib():
 
 SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ldm_ws_doubles",
                 "atsc3_minsum_batch_ctl_words", "atsc3_minsum_batch_max_grid",
+                "atsc3_minsum_batch_xcd_ctl_words", "atsc3_minsum_batch_xcd_max_grid",
                 "atsc3_minsum_batch_status_word", "atsc3_soft_demap_bwd_grid",
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
                 "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows",

@@ -73,12 +73,34 @@ class CodeBank:
         return len(self.codes)
 
 
-def max_grid(msg_dtype="float32"):
-    """Workgroups of the persistent kernel that fit on the current device."""
-    g = _lib.lib().atsc3_minsum_batch_max_grid(MSG_FORMATS[msg_dtype])
+def max_grid(msg_dtype="float32", xcd=False):
+    """Workgroups of the persistent kernel (grid-wide or XCD-partitioned
+    mode) that fit on the current device."""
+    fn = "atsc3_minsum_batch_xcd_max_grid" if xcd else "atsc3_minsum_batch_max_grid"
+    g = getattr(_lib.lib(), fn)(MSG_FORMATS[msg_dtype])
     if g <= 0:
-        raise _lib.FecError("atsc3_minsum_batch_max_grid", g)
+        raise _lib.FecError(fn, g)
     return g
+
+
+# parts of the queue in the XCD-partitioned mode: one per XCD of an MI355X
+XCD_PARTS = 8
+
+
+def xcd_parts(n_of, nparts=XCD_PARTS):
+    """(order, part_off): codewords dealt to ``nparts`` parts longest first,
+    each to the part with the fewest code bits so far (ties: lowest part), a
+    part keeping the input order of its codewords."""
+    n_of = np.asarray(n_of, np.int64)
+    load = np.zeros(nparts, np.int64)
+    part = np.empty(len(n_of), np.int64)
+    for b in sorted(range(len(n_of)), key=lambda b: (-n_of[b], b)):
+        p = int(np.argmin(load))
+        part[b] = p
+        load[p] += n_of[b]
+    order = np.concatenate([np.flatnonzero(part == p) for p in range(nparts)]).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(np.bincount(part, minlength=nparts))]).astype(np.int32)
+    return order, off
 
 
 class BatchPlan:
@@ -90,7 +112,12 @@ class BatchPlan:
     """
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75,
-                 msg_dtype="float32", slots=None, grid=None, msg_scale=None, ladder=None):
+                 msg_dtype="float32", slots=None, grid=None, msg_scale=None, ladder=None,
+                 xcd=False):
+        """xcd: the XCD-partitioned mode (atsc3_minsum_batch_xcd_dev): each XCD
+        decodes its own part of the queue with barriers among its own
+        workgroups; ``slots`` then counts all XCDs' slots (default: every
+        codeword of the largest part on each XCD)."""
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -114,12 +141,21 @@ class BatchPlan:
         self.off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
         self.total = int(self.off[-1])
         self.code_idx = np.array(idx, np.int32)
-        cap = max_grid(msg_dtype)
+        self.xcd = bool(xcd)
+        cap = max_grid(msg_dtype, self.xcd)
         self.grid = cap if grid is None else int(grid)
         if not 1 <= self.grid <= cap:
             raise ValueError(f"grid {self.grid} outside 1..{cap} (co-resident capacity)")
-        self.slots = max(1, min(self.nblk, 1024 if slots is None else int(slots)))
         dev = self.device
+        if self.xcd:
+            order, part_off = xcd_parts(ns)
+            widest = int(np.diff(part_off).max(initial=1))
+            want = XCD_PARTS * widest if slots is None else int(slots)
+            self.slots = max(16, min(want, 2048))
+            self.order = torch.from_numpy(order).to(dev)
+            self.part_off = torch.from_numpy(part_off).to(dev)
+        else:
+            self.slots = max(1, min(self.nblk, 1024 if slots is None else int(slots)))
         self.blk_code = torch.from_numpy(self.code_idx).to(dev)
         self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
         self.llr = torch.zeros(self.total, dtype=torch.float32, device=dev)
@@ -131,7 +167,9 @@ class BatchPlan:
                                device=dev)
         self.msg = torch.empty(self.slots * self.bank.max_nbe * ROW * MSG_BYTES[self.fmt],
                                dtype=torch.uint8, device=dev)
-        nctl = _lib.lib().atsc3_minsum_batch_ctl_words(max(self.nblk, 0), self.slots)
+        words = (_lib.lib().atsc3_minsum_batch_xcd_ctl_words if self.xcd
+                 else _lib.lib().atsc3_minsum_batch_ctl_words)
+        nctl = words(max(self.nblk, 0), self.slots)
         self.ctl = torch.zeros(nctl, dtype=torch.int32, device=dev)
         self.status_word = _lib.lib().atsc3_minsum_batch_status_word()
         self.msg_scale = None
@@ -156,22 +194,28 @@ class BatchPlan:
         if self.nblk == 0:
             return
         st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
-        _lib.call("atsc3_minsum_batch_dev", self.llr.data_ptr(), self.nblk,
-                  self.blk_off.data_ptr(), self.blk_code.data_ptr(), self.bank.desc.data_ptr(),
-                  len(self.bank), self.bank.max_nvg, self.bank.max_nbe, self.bank.max_vdeg,
-                  self.iters, self.alpha, *self.ladder,
-                  self.fmt, None if self.msg_scale is None else self.msg_scale.data_ptr(),
-                  self.slots, self.grid, self.tot.data_ptr(), self.msg.data_ptr(),
-                  self.ctl.data_ptr(), self.bits.data_ptr(), self.conv.data_ptr(),
-                  self.its.data_ptr(), self.bad.data_ptr(), st)
+        head = (self.llr.data_ptr(), self.nblk, self.blk_off.data_ptr(),
+                self.blk_code.data_ptr(), self.bank.desc.data_ptr(), len(self.bank),
+                self.bank.max_nvg, self.bank.max_nbe, self.bank.max_vdeg, self.iters,
+                self.alpha, *self.ladder, self.fmt,
+                None if self.msg_scale is None else self.msg_scale.data_ptr())
+        tail = (self.slots, self.grid, self.tot.data_ptr(), self.msg.data_ptr(),
+                self.ctl.data_ptr(), self.bits.data_ptr(), self.conv.data_ptr(),
+                self.its.data_ptr(), self.bad.data_ptr(), st)
+        if self.xcd:
+            _lib.call("atsc3_minsum_batch_xcd_dev", *head, self.order.data_ptr(),
+                      self.part_off.data_ptr(), int(self.part_off.numel()) - 1, *tail)
+        else:
+            _lib.call("atsc3_minsum_batch_dev", *head, *tail)
 
     def check(self):
-        """Synchronize and raise if the kernel aborted (a grid-barrier wait
-        timed out, e.g. when the workgroups could not all be resident)."""
+        """Synchronize and raise if the kernel aborted (a barrier wait timed
+        out, e.g. when the workgroups could not all be resident)."""
         if self.nblk and int(self.ctl[self.status_word].item()) != 0:
             raise RuntimeError("batched min-sum aborted: grid barrier wait timed out")
 
     def rounds(self):
+        """Rounds of the last launch (XCD-partitioned: of the busiest XCD)."""
         return int(self.ctl[9].item())
 
     def capture(self, pre=None):

@@ -71,6 +71,50 @@ __device__ inline void wg_scan2(int x0, int x1, int &e0, int &e1, int &t0, int &
     __syncthreads();
 }
 
+// The workgroups that share one round's lists: the whole grid (grid-wide
+// mode) or the workgroups of one XCD (XCD-partitioned mode).  h: the group's
+// control region; rank / size: this workgroup's place in the group; k / ng:
+// the group's index and the number of groups (its queue: parts k, k + ng, ...).
+struct Grp {
+    int *h;
+    int rank, size, k, ng;
+};
+
+// Unsatisfied-check counts are summed with agent-scope atomics; every other
+// access to them is agent-scope too, so all of them see one order per word
+// whichever L2 the workgroup's XCD has.
+__device__ __forceinline__ int unsat_load(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void unsat_clear(int *p) {
+    __hip_atomic_store(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void set_abort(const BatchArgs &b) {
+    __hip_atomic_store(b.ctl + H_ABORT, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The group's queue: its length, and the codeword at a position of it (-1
+// when the host's order or part table points outside the batch).
+__device__ int queue_len(const BatchArgs &b, const Grp &G) {
+    if (!b.order) return b.nblk;
+    int len = 0;
+    for (int c = G.k; c < b.nparts; c += G.ng) len += b.part_off[c + 1] - b.part_off[c];
+    return len;
+}
+__device__ int queue_blk(const BatchArgs &b, const Grp &G, int pos) {
+    if (!b.order) return pos;
+    for (int c = G.k; c < b.nparts; c += G.ng) {
+        const int lo = b.part_off[c], len = b.part_off[c + 1] - lo;
+        if (pos < len) {
+            if (lo + pos < 0 || lo + pos >= b.nblk) return -1;
+            const int blk = b.order[lo + pos];
+            return blk >= 0 && blk < b.nblk ? blk : -1;
+        }
+        pos -= len;
+    }
+    return -1;
+}
+
 __device__ __forceinline__ void put4(int *p, int x, int y, int z, int w) {
     p[0] = x;
     p[1] = y;
@@ -78,14 +122,14 @@ __device__ __forceinline__ void put4(int *p, int x, int y, int z, int w) {
     p[3] = w;
 }
 
-// Next round's lists (run by one whole workgroup).  With n_chk = 0 it only
-// admits codewords (the initial fill).
-__device__ void compact(const BatchArgs &b) {
+// Next round's lists of group G (run by one whole workgroup).  With n_chk =
+// 0 it only admits codewords (the initial fill).
+__device__ void compact(const BatchArgs &b, const Grp &G) {
     __shared__ int lds[4 * (ATSC3_WG / 64)];
-    int *h = b.ctl;
-    int *var_list = b.ctl + b.var_list, *var_pre = b.ctl + b.var_pre;
-    int *chk_list = b.ctl + b.chk_list, *chk_pre = b.ctl + b.chk_pre;
-    int *unsat = b.ctl + b.unsat, *fl = b.ctl + b.free_list, *pl = b.ctl + b.pend_list;
+    int *h = G.h;
+    int *var_list = h + b.var_list, *var_pre = h + b.var_pre;
+    int *chk_list = h + b.chk_list, *chk_pre = h + b.chk_pre;
+    int *unsat = b.ctl + b.unsat, *fl = h + b.free_list, *pl = h + b.pend_list;
     const int tid = threadIdx.x;
     const int n = ldw(h + H_NCHK), n_pend = ldw(h + H_NPEND), qn = ldw(h + H_QNEXT);
     int n_free = ldw(h + H_NFREE);
@@ -107,7 +151,7 @@ __device__ void compact(const BatchArgs &b) {
             code = cz & 255;
             rung = (cz >> 10) & 3;
             bit = cz >> 12;
-            u = ldw(unsat + slot);
+            u = unsat_load(unsat + slot);
             ret = u == 0 || bit >= b.iters;
             if (ret) {
                 // a rung's result is taken when it converged or leaves fewer
@@ -134,7 +178,7 @@ __device__ void compact(const BatchArgs &b) {
                 const int nz = ret ? code | ((rung + 1) << 10) : code | (rung << 10) | ((bit + 1) << 12);
                 put4(chk_list + 4 * q, slot, blk, nz, 0);
                 chk_pre[q] = bc + ec;
-                unsat[slot] = 0;
+                unsat_clear(unsat + slot);
             } else {
                 pl[n_ret + er] = slot;
             }
@@ -146,15 +190,21 @@ __device__ void compact(const BatchArgs &b) {
         __syncthreads();
     }
     // admit queued codewords into free slots (most recently freed first)
-    const int rem = b.nblk - qn;
+    const int qlen = queue_len(b, G);
+    const int rem = qlen - qn;
     const int adm = n_free < rem ? n_free : rem;
     for (int i0 = 0; i0 < adm; i0 += ATSC3_WG) {
         const int i = i0 + tid;
-        const bool valid = i < adm;
+        bool valid = i < adm;
         int slot = 0, blk = 0, code = 0;
         if (valid) {
             slot = ldw(fl + n_free - 1 - i);
-            blk = qn + i;
+            blk = queue_blk(b, G, qn + i);
+            if (blk < 0) {
+                set_abort(b);
+                valid = false;
+                blk = 0;
+            }
             code = b.blk_code[blk];
         }
         int ev, ec, tv, tc;
@@ -165,7 +215,7 @@ __device__ void compact(const BatchArgs &b) {
             var_pre[n + i] = bv + ev;
             put4(chk_list + 4 * (n_keep + i), slot, blk, code, 0);
             chk_pre[n_keep + i] = bc + ec;
-            unsat[slot] = 0;
+            unsat_clear(unsat + slot);
         }
         bv += tv;
         bc += tc;
@@ -179,13 +229,17 @@ __device__ void compact(const BatchArgs &b) {
         h[H_NFREE] = n_free - adm;
         h[H_NPEND] = n_ret;
         h[H_QNEXT] = qn + adm;
-        h[H_DONE] = (n_keep + adm == 0 && qn + adm == b.nblk) ? 1 : 0;
+        h[H_DONE] = (n_keep + adm == 0 && qn + adm == qlen) ? 1 : 0;
         const int rounds = ldw(h + H_ROUNDS) + 1;
         h[H_ROUNDS] = rounds;
+        // the most rounds of any group, where the host reads them
+        if (h != b.ctl)
+            __hip_atomic_fetch_max(b.ctl + H_ROUNDS, rounds, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         // every codeword leaves its slot within iters + 3 rounds of admission,
         // so this bound is never reached by a correct schedule: it only
         // guarantees that the kernel ends
-        if (rounds > b.max_rounds) h[H_ABORT] = 1;
+        if (rounds > b.max_rounds) set_abort(b);
     }
 }
 
@@ -205,17 +259,19 @@ __device__ __forceinline__ unsigned xcc_id() {
 // at launch (every workgroup counts itself on its XCD, then a counted
 // one-time barrier).  False when the census wait timed out.
 struct BarGroup {
-    unsigned g, gsize, ng;
+    unsigned g, gsize, ng, rank, k;   // rank: order of arrival in the XCD's count;
+                                      // k: the XCD's index among the non-empty ones
 };
 
 __device__ bool bar_census(const BatchArgs &b, BarGroup &bg, int *s_flag) {
     unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
     unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
     __shared__ unsigned s_cnt[BATCH_BAR_GROUPS];
+    __shared__ unsigned s_rank;
     const unsigned g = xcc_id();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar + BAR_CENSUS + g, 1u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        s_rank = __hip_atomic_fetch_add(bar + BAR_CENSUS + g, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned old = __hip_atomic_fetch_add(bar + BAR_CENSUS_ARRIVE, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -244,11 +300,16 @@ __device__ bool bar_census(const BatchArgs &b, BarGroup &bg, int *s_flag) {
         s_flag[1] = __hip_atomic_load(h + H_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    unsigned ng = 0;
-    for (int q = 0; q < BATCH_BAR_GROUPS; ++q) ng += s_cnt[q] ? 1u : 0u;
+    unsigned ng = 0, k = 0;
+    for (int q = 0; q < BATCH_BAR_GROUPS; ++q) {
+        ng += s_cnt[q] ? 1u : 0u;
+        k += s_cnt[q] && (unsigned)q < g ? 1u : 0u;
+    }
     bg.g = (unsigned)uni((int)g);
     bg.gsize = (unsigned)uni((int)s_cnt[g]);
     bg.ng = (unsigned)uni((int)ng);
+    bg.rank = (unsigned)uni((int)s_rank);
+    bg.k = (unsigned)uni((int)k);
     const bool ok = s_flag[1] == 0;
     __syncthreads();
     return ok;
@@ -270,8 +331,8 @@ __device__ bool bar_census(const BatchArgs &b, BarGroup &bg, int *s_flag) {
 // write-back per workgroup made a one-codeword round 20% slower.  Counts on
 // separate 128-byte lines (a flat counter serialised 512 same-line atomics);
 // waiters poll the generation word on a line of its own.
-__device__ bool grid_barrier(const BatchArgs &b, const BarGroup &bg, unsigned epoch,
-                             bool do_compact, int *s_flag) {
+__device__ bool grid_barrier(const BatchArgs &b, const Grp &G, const BarGroup &bg,
+                             unsigned epoch, bool do_compact, int *s_flag) {
     unsigned *h = reinterpret_cast<unsigned *>(b.ctl);
     unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
     wait_vmem();
@@ -292,7 +353,7 @@ __device__ bool grid_barrier(const BatchArgs &b, const BarGroup &bg, unsigned ep
     }
     __syncthreads();
     if (s_flag[0]) {
-        if (do_compact) compact(b);
+        if (do_compact) compact(b, G);
         wait_vmem();
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -317,12 +378,62 @@ __device__ bool grid_barrier(const BatchArgs &b, const BarGroup &bg, unsigned ep
     return s_flag[1] == 0;
 }
 
+// Barrier number `epoch` of one XCD group (XCD-partitioned mode): the
+// group's workgroups share one L2, so what they publish to each other needs
+// no L2 write-back.  Every wave waits for its stores to complete in that L2,
+// one wave per workgroup arrives in the group's count; the last to arrive
+// drops its CU's L1 (agent-scope acquire), runs the compaction when asked,
+// waits for its stores and bumps the group's generation; every other
+// workgroup polls the generation and then drops its L1.  A bounded wait, as
+// in the grid barrier.  Returns false when the abort word is set.
+__device__ bool xcd_barrier(const BatchArgs &b, const Grp &G, const BarGroup &bg,
+                            unsigned epoch, bool do_compact, int *s_flag) {
+    unsigned *ab = reinterpret_cast<unsigned *>(b.ctl + H_ABORT);
+    unsigned *bar = reinterpret_cast<unsigned *>(b.ctl + b.bar);
+    unsigned *cnt = bar + BAR_GROUP + 32 * bg.g, *gen = bar + BAR_XGEN + 32 * bg.g;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    wait_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        const int last = old + 1u == epoch * bg.gsize;
+        if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_flag[0] = last;
+    }
+    __syncthreads();
+    if (s_flag[0]) {
+        if (do_compact) compact(b, G);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        wait_vmem();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(gen, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+            if (__hip_atomic_load(ab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (++spins >= BATCH_SPIN_LIMIT) {
+                __hip_atomic_store(ab, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (threadIdx.x == 0)
+        s_flag[1] = __hip_atomic_load(ab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return s_flag[1] == 0;
+}
+
 // This workgroup's contiguous item range of a list; k0 = the entry holding
 // item i0 (found by the whole workgroup).  Returns false when empty.
-__device__ bool my_range(const int *pre, int n, int &i0, int &i1, int &k0, int *s_k) {
+__device__ bool my_range(const Grp &G, const int *pre, int n, int &i0, int &i1, int &k0,
+                         int *s_k) {
     const int total = uni(ldw(pre + n));
-    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    i0 = (int)blockIdx.x * chunk;
+    const int chunk = (total + G.size - 1) / G.size;
+    i0 = G.rank * chunk;
     i1 = i0 + chunk < total ? i0 + chunk : total;
     if (i0 >= i1) return false;
     if (threadIdx.x == 0) s_k[0] = 0;
@@ -343,12 +454,12 @@ __device__ __forceinline__ void flush_unsat_slot(int *unsat, int slot, unsigned 
 // check_phase and var_phase are inlined into each kernel variant (a shared
 // out-of-line copy would be a real call and multiply the register use).
 template <int FMT, int CMAX>
-__device__ __forceinline__ void check_phase(const BatchArgs &b, int *s_k) {
+__device__ __forceinline__ void check_phase(const BatchArgs &b, const Grp &G, int *s_k) {
     using T = typename Msg<FMT>::T;
-    const int n = uni(ldw(b.ctl + H_NCHK));
-    const int *list = b.ctl + b.chk_list, *pre = b.ctl + b.chk_pre;
+    const int n = uni(ldw(G.h + H_NCHK));
+    const int *list = G.h + b.chk_list, *pre = G.h + b.chk_pre;
     int i0, i1, k;
-    if (!my_range(pre, n, i0, i1, k, s_k)) return;
+    if (!my_range(G, pre, n, i0, i1, k, s_k)) return;
     const unsigned j = threadIdx.x;
     for (int i = i0; i < i1; ++k) {
         const int slot = uni(ldw(list + 4 * k)), blk = uni(ldw(list + 4 * k + 1));
@@ -437,12 +548,12 @@ __device__ __forceinline__ void var_step(const QcArgs &a, const float *lam, floa
 }
 
 template <int FMT, int VMAX>
-__device__ __forceinline__ void var_phase(const BatchArgs &b, int *s_k) {
+__device__ __forceinline__ void var_phase(const BatchArgs &b, const Grp &G, int *s_k) {
     using T = typename Msg<FMT>::T;
-    const int n = uni(ldw(b.ctl + H_NVAR));
-    const int *list = b.ctl + b.var_list, *pre = b.ctl + b.var_pre;
+    const int n = uni(ldw(G.h + H_NVAR));
+    const int *list = G.h + b.var_list, *pre = G.h + b.var_pre;
     int i0, i1, k;
-    if (!my_range(pre, n, i0, i1, k, s_k)) return;
+    if (!my_range(G, pre, n, i0, i1, k, s_k)) return;
     const unsigned p = threadIdx.x;
     for (int i = i0; i < i1; ++k) {
         const int slot = uni(ldw(list + 4 * k)), blk = uni(ldw(list + 4 * k + 1));
@@ -514,22 +625,67 @@ void batch_kernel(BatchArgs b) {
     __shared__ int s_flag[2];
     BarGroup bg;
     if (!bar_census(b, bg, s_flag)) return;
+    const Grp G{b.ctl, (int)blockIdx.x, (int)gridDim.x, 0, 1};
     unsigned epoch = 0;
     for (;;) {
-        var_phase<FMT, VMAX>(b, s_k);
-        if (!grid_barrier(b, bg, ++epoch, false, s_flag)) return;
-        if (uni(ldw(b.ctl + H_DONE))) return;
-        check_phase<FMT, 16>(b, s_k);
-        if (!grid_barrier(b, bg, ++epoch, true, s_flag)) return;
+        var_phase<FMT, VMAX>(b, G, s_k);
+        if (!grid_barrier(b, G, bg, ++epoch, false, s_flag)) return;
+        if (uni(ldw(G.h + H_DONE))) return;
+        check_phase<FMT, 16>(b, G, s_k);
+        if (!grid_barrier(b, G, bg, ++epoch, true, s_flag)) return;
     }
 }
 
-// Control words reset, every slot free, first admissions.
+// XCD-partitioned mode: the workgroups of each XCD decode their own queue
+// (parts k, k + ng, ... of the host's order) in their own slots
+// (slots / ng of them) with their own lists and barriers; no wait ever
+// spans two XCDs after the launch census.  The group's first workgroup
+// fills its region (every slot free, first admissions) before the group's
+// first barrier.
+template <int FMT, int VMAX>
+__global__ __launch_bounds__(ATSC3_WG) __attribute__((amdgpu_waves_per_eu(4)))
+void batch_kernel_xcd(BatchArgs b) {
+    __shared__ int s_k[1];
+    __shared__ int s_flag[2];
+    BarGroup bg;
+    if (!bar_census(b, bg, s_flag)) return;
+    const Grp G{b.ctl + b.region + (int)bg.k * b.region_words, (int)bg.rank, (int)bg.gsize,
+                (int)bg.k, (int)bg.ng};
+    if (G.rank == 0) {
+        const int nslot = b.slots / G.ng, slot0 = G.k * nslot;
+        for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) G.h[k] = 0;
+        for (int k = threadIdx.x; k < nslot; k += ATSC3_WG)
+            G.h[b.free_list + k] = slot0 + nslot - 1 - k;
+        if (threadIdx.x == 0) {
+            G.h[H_NFREE] = nslot;
+            // a group with codewords but no slot could never finish
+            if (nslot == 0 && queue_len(b, G) > 0) set_abort(b);
+        }
+        __syncthreads();
+        compact(b, G);
+    }
+    unsigned epoch = 0;
+    if (!xcd_barrier(b, G, bg, ++epoch, false, s_flag)) return;
+    for (;;) {
+        var_phase<FMT, VMAX>(b, G, s_k);
+        if (!xcd_barrier(b, G, bg, ++epoch, false, s_flag)) return;
+        if (uni(ldw(G.h + H_DONE))) return;
+        check_phase<FMT, 16>(b, G, s_k);
+        if (!xcd_barrier(b, G, bg, ++epoch, true, s_flag)) return;
+    }
+}
+
+// Control words reset, every slot free, first admissions (grid-wide mode);
+// the XCD-partitioned mode resets the shared words only (its groups fill
+// their regions themselves).
 __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
     int *h = b.ctl;
+    const bool xcd = b.order != nullptr;
     for (int k = threadIdx.x; k < H_WORDS; k += ATSC3_WG) h[k] = 0;
     for (int k = threadIdx.x; k < BAR_WORDS; k += ATSC3_WG) b.ctl[b.bar + k] = 0;
-    for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG) b.ctl[b.free_list + k] = b.slots - 1 - k;
+    if (!xcd)
+        for (int k = threadIdx.x; k < b.slots; k += ATSC3_WG)
+            b.ctl[b.free_list + k] = b.slots - 1 - k;
     for (int k = threadIdx.x; k < b.nblk; k += ATSC3_WG) {
         b.ctl[b.best + k] = 0x7fffffff;
         // "not decoded" until the codeword's result is written: a launch
@@ -539,9 +695,10 @@ __global__ __launch_bounds__(ATSC3_WG) void batch_init(BatchArgs b) {
         b.its[k] = -1;
         b.bad[k] = -1;
     }
+    if (xcd) return;
     if (threadIdx.x == 0) h[H_NFREE] = b.slots;
     __syncthreads();
-    compact(b);
+    compact(b, Grp{b.ctl, 0, 1, 0, 1});
 }
 
 // INT8 message step per codeword (given, or 8 * mean|llr| / 127).
@@ -560,7 +717,7 @@ __global__ __launch_bounds__(ATSC3_WG) void batch_steps(BatchArgs b) {
 // only floor(waves_per_simd / 2) workgroups per CU are certain to be resident
 // together (with 3 waves per SIMD the query reports 2 but 1 is guaranteed;
 // a grid that is not co-resident makes every barrier wait time out).
-template <int FMT, int VMAX>
+template <int FMT, int VMAX, bool XCD>
 static int capacity_of() {
     static int cap[64];
     int dev = 0;
@@ -568,16 +725,16 @@ static int capacity_of() {
     int &c = cap[dev];
     if (c == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<FMT, VMAX>,
-                                                         ATSC3_WG, 0) != hipSuccess)
+        const void *fn = XCD ? reinterpret_cast<const void *>(batch_kernel_xcd<FMT, VMAX>)
+                             : reinterpret_cast<const void *>(batch_kernel<FMT, VMAX>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ATSC3_WG, 0) !=
+            hipSuccess)
             return 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
             hipSuccess)
             return 0;
         hipFuncAttributes fa;
-        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(batch_kernel<FMT, VMAX>)) !=
-            hipSuccess)
-            return 0;
+        if (hipFuncGetAttributes(&fa, fn) != hipSuccess) return 0;
         const int vgprs = fa.numRegs > 0 ? ((fa.numRegs + 7) / 8) * 8 : 512;
         int waves_per_simd = 512 / vgprs;
         if (waves_per_simd > 8) waves_per_simd = 8;
@@ -591,26 +748,34 @@ static int capacity_of() {
     return c;
 }
 
-template <int FMT, int VMAX>
+template <int FMT, int VMAX, bool XCD>
 static int32_t run_batch(const BatchArgs &b, int grid, hipStream_t st) {
     if (FMT == 2)
         hipLaunchKernelGGL(batch_steps, dim3(b.nblk), dim3(ATSC3_WG), 0, st, b);
     hipLaunchKernelGGL(batch_init, dim3(1), dim3(ATSC3_WG), 0, st, b);
-    hipLaunchKernelGGL((batch_kernel<FMT, VMAX>), dim3(grid), dim3(ATSC3_WG), 0, st, b);
+    if (XCD)
+        hipLaunchKernelGGL((batch_kernel_xcd<FMT, VMAX>), dim3(grid), dim3(ATSC3_WG), 0, st, b);
+    else
+        hipLaunchKernelGGL((batch_kernel<FMT, VMAX>), dim3(grid), dim3(ATSC3_WG), 0, st, b);
     return launch_status();
 }
 
 
 }  // namespace
 
-int32_t BATCH_LAUNCH(const BatchArgs &b, int grid, hipStream_t st, bool v16) {
-    return v16 ? run_batch<BATCH_FMT, 16>(b, grid, st) : run_batch<BATCH_FMT, 32>(b, grid, st);
+int32_t BATCH_LAUNCH(const BatchArgs &b, int grid, hipStream_t st, bool v16, bool xcd) {
+    if (xcd)
+        return v16 ? run_batch<BATCH_FMT, 16, true>(b, grid, st)
+                   : run_batch<BATCH_FMT, 32, true>(b, grid, st);
+    return v16 ? run_batch<BATCH_FMT, 16, false>(b, grid, st)
+               : run_batch<BATCH_FMT, 32, false>(b, grid, st);
 }
 
 // The smaller of the two register variants' capacities, so a grid valid for
 // a format is valid whatever the codes' degrees.
-int BATCH_CAPACITY() {
-    const int a = capacity_of<BATCH_FMT, 16>(), c = capacity_of<BATCH_FMT, 32>();
+int BATCH_CAPACITY(bool xcd) {
+    const int a = xcd ? capacity_of<BATCH_FMT, 16, true>() : capacity_of<BATCH_FMT, 16, false>();
+    const int c = xcd ? capacity_of<BATCH_FMT, 32, true>() : capacity_of<BATCH_FMT, 32, false>();
     return a < c ? a : c;
 }
 

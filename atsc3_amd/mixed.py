@@ -68,6 +68,7 @@ class MixedPipeline:
     def __init__(self, specs, frames=1, device="cuda", iters=50, alpha=0.75,
                  msg_dtype="float32", slots=None, grid=None, engine="batch"):
         """engine "batch": the persistent batched engine (slots, grid);
+        "xcd": the same engine partitioned per XCD (slots over all XCDs);
         "resident": one workgroup per codeword (grid = workgroups, slots
         unused)."""
         self.device = torch.device(device)
@@ -85,8 +86,11 @@ class MixedPipeline:
         if engine == "resident":
             from .resident import ResidentPlan
             self.plan = ResidentPlan(codes, self.device, iters, alpha, grid, msg_dtype)
+        elif engine in ("batch", "xcd"):
+            self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid,
+                                  xcd=engine == "xcd")
         else:
-            self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid)
+            raise ValueError(f"engine {engine!r}: 'batch', 'xcd' or 'resident'")
         self.engine = engine
         self.demappers = {}
         self.cells, self.sigma2 = [], []

@@ -224,6 +224,40 @@ int32_t atsc3_minsum_batch_dev(const float *llr, int32_t nblk,
                                int32_t *its, int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
+ * XCD-partitioned batched min-sum: the decode of atsc3_minsum_batch_dev
+ * (same arguments, results bit-identical) for frames whose slowest codeword
+ * sets their time.  The workgroups of each XCD decode their own share of the
+ * queue in their own slots with barriers among themselves only (no L2
+ * write-back, no wait for the other XCDs), so a 50-iteration codeword no
+ * longer holds every workgroup of the chip to its rounds.
+ *   order     [nblk] int32 device: a permutation of 0 .. nblk-1
+ *   part_off  [nparts + 1] int32 device, 0 = part_off[0] <= ... <=
+ *             part_off[nparts] = nblk: part c is order[part_off[c] ..
+ *             part_off[c + 1]); the k-th of the ng XCDs holding workgroups
+ *             decodes parts k, k + ng, ... in that order (balance the parts:
+ *             a launch takes as long as its slowest XCD)
+ *   slots     >= 16; each XCD gets slots / ng of them
+ *   grid      1 .. atsc3_minsum_batch_xcd_max_grid(msg_fmt)
+ *   ctl_ws    atsc3_minsum_batch_xcd_ctl_words(nblk, slots) int32 (status
+ *             word as atsc3_minsum_batch_dev)
+ * ------------------------------------------------------------------------ */
+int32_t atsc3_minsum_batch_xcd_ctl_words(int32_t nblk, int32_t slots);
+int32_t atsc3_minsum_batch_xcd_max_grid(int32_t msg_fmt);
+int32_t atsc3_minsum_batch_xcd_dev(const float *llr, int32_t nblk,
+                                   const int64_t *blk_off, const int32_t *blk_code,
+                                   const void *codes, int32_t ncode,
+                                   int32_t max_nvg, int32_t max_nbe,
+                                   int32_t max_vdeg, int32_t iters,
+                                   float alpha, const float *ladder,
+                                   int32_t nrungs, int32_t msg_fmt,
+                                   const float *msg_scale, const int32_t *order,
+                                   const int32_t *part_off, int32_t nparts,
+                                   int32_t slots, int32_t grid, float *tot_ws,
+                                   void *msg_ws, int32_t *ctl_ws, uint8_t *bits,
+                                   uint8_t *conv, int32_t *its, int32_t *bad,
+                                   void *stream);
+
+/* ------------------------------------------------------------------------
  * Resident min-sum: the same decode calls as atsc3_minsum_batch_dev replaces
  * (lab/m9_fast.py:843-925, lab/m44_ldm.py:2497-2576 over
  * lab/ldpc_kernel.c:65-146), one workgroup per codeword holding the

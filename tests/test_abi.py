@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -105,6 +106,50 @@ def test_size_queries():
     assert L.atsc3_minsum_batch_ctl_words(-1, 10) < 0
     assert L.atsc3_soft_demap_bwd_grid(3, 256) == 3
     assert L.atsc3_minsum_batch_status_word() >= 0
+    # the XCD-partitioned mode keeps one list region per possible XCD
+    assert L.atsc3_minsum_batch_xcd_ctl_words(100, 16) > 16 * 10 * 16
+    assert L.atsc3_minsum_batch_xcd_ctl_words(100, 0) < 0
+
+
+def test_batch_xcd_entry_rejects_bad_arguments():
+    """atsc3_minsum_batch_xcd_dev checks its partition arguments before any
+    device call: no order or part table, no part, fewer than 16 slots."""
+    L = _lib().lib()
+    E_ARG, OK = -2, 0
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+
+    def call(nblk=4, order=p, part_off=p, nparts=8, slots=16):
+        return L.atsc3_minsum_batch_xcd_dev(p, nblk, p, p, p, 1, 180, 500, 13, 50, 0.75, None, 0,
+                                            0, None, order, part_off, nparts, slots, 64, p, p,
+                                            p, p, p, p, p, None)
+    assert call(nblk=-1) == E_ARG
+    assert call(nblk=0, order=None) == OK
+    assert call(order=None) == E_ARG
+    assert call(part_off=None) == E_ARG
+    assert call(nparts=0) == E_ARG
+    assert call(slots=15) == E_ARG
+
+
+def test_xcd_parts_balance_code_bits():
+    """The host's split of a queue into per-XCD parts: a permutation, parts
+    in input order, longest codewords dealt first to the lightest part."""
+    from atsc3_amd.batch import xcd_parts
+    rng = np.random.default_rng(5)
+    for nblk in (0, 1, 5, 8, 74, 300):
+        n_of = rng.choice([16200, 64800], nblk)
+        order, off = xcd_parts(n_of)
+        assert off[0] == 0 and off[-1] == nblk and len(off) == 9 and (np.diff(off) >= 0).all()
+        assert sorted(order.tolist()) == list(range(nblk))
+        load = [int(n_of[order[a:b]].sum()) for a, b in zip(off[:-1], off[1:])]
+        for a, b in zip(off[:-1], off[1:]):
+            assert (np.diff(order[a:b]) > 0).all()
+        if nblk:
+            # greedy longest-first: no part exceeds the lightest by more than
+            # one longest codeword
+            assert max(load) - min(load) <= int(n_of.max())
+    order, off = xcd_parts([64800] * 3 + [16200] * 4)
+    assert np.diff(off).tolist() == [1, 1, 1, 1, 1, 1, 1, 0]
 
 
 def test_resident_entry_rejects_bad_arguments():

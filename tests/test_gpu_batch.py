@@ -37,10 +37,10 @@ def _mixed_llrs(nblk, seed, spread=0.6):
 
 
 def _batch(codes, llrs, device, iters, alpha, msg="float32", slots=None, grid=None,
-           ladder=None):
+           ladder=None, xcd=False):
     import torch
     from atsc3_amd.batch import BatchPlan
-    plan = BatchPlan(codes, device, iters, alpha, msg, slots, grid, ladder=ladder)
+    plan = BatchPlan(codes, device, iters, alpha, msg, slots, grid, ladder=ladder, xcd=xcd)
     for b, l in enumerate(llrs):
         plan.llr_view(b).copy_(torch.from_numpy(np.ascontiguousarray(l, np.float32)))
     plan.run()
