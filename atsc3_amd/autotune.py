@@ -16,7 +16,10 @@ file keyed by device name and workload signature (``ATSC3_TUNE_CACHE``, else
 engine (one workgroup per codeword, one per compute unit) is a candidate
 too, built as slots = "resident": it wins when a launch carries many
 codewords, the batched engine when a few slow codewords would leave most
-compute units idle.
+compute units idle.  With ``xcd=True`` the batched engine's XCD-partitioned
+mode is a candidate too (slots = "xcd": every codeword of the largest part
+on each XCD): it wins when the rounds of a few slow codewords dominate, since
+its barriers span one XCD and need no L2 write-back.
 """
 from __future__ import annotations
 
@@ -55,11 +58,15 @@ def _store(d):
         pass
 
 
-def candidates(nblk, msg_dtype):
+def candidates(nblk, msg_dtype, xcd=False):
     cap = max_grid(msg_dtype)
     grids = sorted({g for g in (cap, cap // 2, cap // 4, 128) if 1 <= g <= cap}, reverse=True)
     slots = sorted({max(1, s) for s in (nblk, (nblk + 1) // 2, (nblk + 3) // 4)}, reverse=True)
-    return [(s, g) for s in slots for g in grids]
+    out = [(s, g) for s in slots for g in grids]
+    if xcd:
+        capx = max_grid(msg_dtype, xcd=True)
+        out += [("xcd", g) for g in sorted({capx, capx // 2} - {0}, reverse=True)]
+    return out
 
 
 def time_graph(g, reps):
@@ -76,20 +83,20 @@ def time_graph(g, reps):
 
 
 def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True, log=None,
-         resident=False):
+         resident=False, xcd=False):
     """build(slots, grid) -> object with .capture() and .plan; prepare(obj)
     loads the tuning input.  Returns (slots, grid, table of ms per candidate);
     slots == "resident" (grid None: one workgroup per compute unit) when the
-    resident engine won."""
+    resident engine won, slots == "xcd" when the XCD-partitioned mode did."""
     dev = torch.cuda.get_device_name()
-    key = f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}"
+    key = f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}{'x' if xcd else ''}"
     cache = _load() if use_cache else {}
     if key in cache:
         c = cache[key]
         return c["slots"], c["grid"], c.get("table", {})
     table = {}
     best = None
-    cands = candidates(nblk, msg_dtype) + ([("resident", None)] if resident else [])
+    cands = candidates(nblk, msg_dtype, xcd) + ([("resident", None)] if resident else [])
     for slots, grid in cands:
         obj = build(slots, grid)
         if prepare is not None:

@@ -335,8 +335,8 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
 
     def build(fmt):
         return lambda slots, grid: MixedPipeline(
-            specs, F, dev, 50, 0.75, fmt, None if slots == "resident" else slots, grid,
-            engine="resident" if slots == "resident" else "batch")
+            specs, F, dev, 50, 0.75, fmt, None if slots in ("resident", "xcd") else slots, grid,
+            engine=slots if slots in ("resident", "xcd") else "batch")
 
     def prepare(p):
         src = MixedSource(p, 8, args.seed + 5)
@@ -346,7 +346,8 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     ref_bits = None
     for fmt in formats:
         slots, grid, table = autotune.tune(f"mixed{len(specs)}x{F}", nblk, fmt, build(fmt),
-                                           prepare, use_cache=False, log=log, resident=True)
+                                           prepare, use_cache=False, log=log, resident=True,
+                                           xcd=True)
         pipe = build(fmt)(slots, grid)
         src = MixedSource(pipe, 8, args.seed + 5)
         errs, blocks, its, disagree = 0, 0, [], 0

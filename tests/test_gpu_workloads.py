@@ -1,9 +1,10 @@
 """The bench's workloads run as tests (smaller where noted), so what the
 benchmark reports is also checked on every GPU run:
 
-* the autotuner on an uneven 74-block frame (every candidate captured and
+* the autotuner on an uneven 74-block frame (every candidate, the
+  XCD-partitioned mode and the resident engine included, captured and
   timed; the choice it returns decodes the frame exactly as the default
-  configuration and as the plain decoder per code);
+  configuration and as the XCD-partitioned mode);
 * an Es/N0 sweep of the live-frame workload (reduced: 256 blocks per point)
   through demap + resident decode, FP32 and FP16: FER falls with Es/N0, the
   operating point's blocks all decode, and FP16 stays within FP32's errors;
@@ -26,32 +27,34 @@ def test_autotune_on_uneven_frame(device, tmp_path, monkeypatch):
 
     def build(slots, grid):
         return MixedPipeline(specs, 1, device, 50, 0.75, "float32",
-                             None if slots == "resident" else slots, grid,
-                             engine="resident" if slots == "resident" else "batch")
+                             None if slots in ("resident", "xcd") else slots, grid,
+                             engine=slots if slots in ("resident", "xcd") else "batch")
 
     def prepare(p):
         src = MixedSource(p, 4, 31)
         cells, s2, _ = src.frame(block_esn0(p.specs, 33))
         p.load(cells, s2)
     slots, grid, table = autotune.tune("test74", 74, "float32", build, prepare, reps=2,
-                                       use_cache=True, resident=True)
-    cands = autotune.candidates(74, "float32")
+                                       use_cache=True, resident=True, xcd=True)
+    cands = autotune.candidates(74, "float32", xcd=True)
+    assert sum(c[0] == "xcd" for c in cands) == 2
     assert len(table) == len(cands) + 1 and all(v > 0 for v in table.values())
     best = min(table, key=table.get)
     assert best == ("resident" if slots == "resident" else f"{slots}x{grid}")
     # cached: the second call returns the same choice without timing
-    assert autotune.tune("test74", 74, "float32", build, prepare, resident=True)[:2] == \
-        (slots, grid)
+    assert autotune.tune("test74", 74, "float32", build, prepare, resident=True,
+                         xcd=True)[:2] == (slots, grid)
     outs = []
-    for sl, gr in ((slots, grid), (None, None)):
+    for sl, gr in ((slots, grid), (None, None), ("xcd", None)):
         p = build(sl if sl is not None else 74, gr)
         prepare(p)
         p.run()
         p.plan.check()
         torch.cuda.synchronize()
         outs.append([x.clone() for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad)])
-    for a, b in zip(*outs):
-        assert torch.equal(a, b)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.equal(a, b)
 
 
 def test_live_frame_sweep_reduced(device):
