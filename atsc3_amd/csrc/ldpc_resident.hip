@@ -21,9 +21,11 @@
 //
 // Check records: min-sum messages of a check take only two magnitudes, so
 // each check keeps one record: a1 = alpha * min1 and a2 = alpha * min2 (FP16:
-// one word a1 | a2 << 16; FP32: two words), word 1 = the signs of edges 0..24
-// and the index of the edge that takes a2 (bits 25..31), and a 64-bit word
-// with the signs of edges 25..88, touched only by checks of degree > 25.
+// word 0 = a1 | a2 << 16; FP32: word 0 = a1 and a separate word a2), word 1 =
+// the signs of edges 0..23 and the index of the edge that takes a2 (bits
+// 24..31, one byte), and a 64-bit word with the signs of edges 24..88,
+// touched only by checks of degree > 24.  Words 0 and 1 sit side by side (one
+// 8-byte access per record).
 // Checks of degree <= 16 without missing entries (check_multi) build their
 // sign word by shifting in one bit per edge, so edge k's sign sits at bit
 // D - 1 - k; the tables give the variable side each entry's bit.
@@ -31,7 +33,10 @@
 // or 12 bytes for rate 10/15, against 2 or 4 bytes per EDGE for stored
 // messages); the check phase rewrites them in place (only the owning lane
 // reads a record back there) and the variable phase gathers them, 64
-// consecutive checks of one group per wave load.
+// consecutive checks of one group per wave load.  The FP16 variable phase
+// rebuilds each message as an FP16 value (the stored half with the edge's
+// sign bit rotated in) and adds it with one mixed-precision fma: the FP32
+// sum of the FP16 value, rounded once, as a conversion and an add give.
 //
 // Per iteration (plain semantics, lab/ldpc_kernel.c:82-143):
 //   CHECK  lane j of a wave takes check (g, j): reads its old record and the
@@ -65,7 +70,7 @@
 #define RES_NMAX 64800
 #define RES_MAX_DC 89
 #define RES_MAX_DV 32
-#define RES_SIGN_LO 25                  // sign bits held in record word 1
+#define RES_SIGN_LO 24                  // sign bits held in record word 1
 #define RES_F32_LDS_ROWS 113            // FP32 posterior rows that fit in LDS
 // FP16 posterior rows in LDS are padded: positions 0..359, then 0..63 again,
 // so the 64 positions a wave reads for one edge are contiguous (no mod 360).
@@ -116,7 +121,8 @@ struct ResArgs {
     const float *msg_scale;       // INT8 mode: step per codeword, or NULL (median rule)
     long long stride;             // records per workgroup slot
     long long tstride;            // global posterior floats per workgroup slot
-    unsigned *p0, *p0b, *p1;      // p0b: FP32 mode's a2 word
+    uint2 *p01;                   // record words 0 and 1
+    unsigned *p0b;                // FP32 mode's a2 word
     unsigned long long *p2;
     float *totg;
     int *queue;
@@ -127,9 +133,15 @@ struct ResArgs {
 
 // FP16 storage of a posterior or a message magnitude: saturated to the
 // largest finite half (+-65504), NaN kept (as the FP32 path keeps it).
+// Rounded first: a value that rounds past 65504 becomes +-inf, which steps
+// back one code to +-65504 -- the same half as clamping to +-65504 first.
+// v is first made an FP32 value in a register: the compiler would otherwise
+// fuse the product or sum that forms it with the conversion into one
+// mixed-precision op (one rounding to FP16 instead of FP32 then FP16).
 __device__ __forceinline__ unsigned short h_bits_sat(float v) {
-    v = v > 65504.0f ? 65504.0f : (v < -65504.0f ? -65504.0f : v);
-    return __half_as_ushort(__float2half_rn(v));
+    asm volatile("" : "+v"(v));
+    const unsigned short h = __half_as_ushort(__float2half_rn(v));
+    return (h & 0x7fffu) == 0x7c00u ? (unsigned short)(h - 1u) : h;
 }
 __device__ __forceinline__ float h_val(unsigned short h) {
     return __half2float(__ushort_as_half(h));
@@ -170,7 +182,8 @@ __device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, un
 
 // Record planes of this workgroup.
 struct Rec {
-    unsigned *p0, *p0b, *p1;
+    uint2 *p01;
+    unsigned *p0b;
     unsigned long long *p2;
     float step, inv;              // INT8 mode: the codeword's step and 1 / step
 };
@@ -194,17 +207,20 @@ __device__ __forceinline__ float rec_a2(unsigned w0, unsigned w0b, const Rec &R)
 __device__ __forceinline__ unsigned i8_level(float a, float inv) {
     return (unsigned)fminf(rintf(a * inv), 127.0f);
 }
-// Magnitude words of (a1, a2) = alpha * (m1, m2).
+// Record r: magnitudes (a1, a2) = alpha * (m1, m2) and word 1.
 template <int M>
-__device__ __forceinline__ void rec_put_mag(const Rec &R, unsigned r, float a1, float a2) {
+__device__ __forceinline__ void rec_put(const Rec &R, unsigned r, float a1, float a2,
+                                        unsigned w1) {
+    unsigned w0;
     if constexpr (M == 0) {
-        at(R.p0, r) = __float_as_uint(a1);
+        w0 = __float_as_uint(a1);
         at(R.p0b, r) = __float_as_uint(a2);
     } else if constexpr (M == 1) {
-        at(R.p0, r) = (unsigned)h_bits_sat(a1) | ((unsigned)h_bits_sat(a2) << 16);
+        w0 = (unsigned)h_bits_sat(a1) | ((unsigned)h_bits_sat(a2) << 16);
     } else {
-        at(R.p0, r) = i8_level(a1, R.inv) | (i8_level(a2, R.inv) << 8);
+        w0 = i8_level(a1, R.inv) | (i8_level(a2, R.inv) << 8);
     }
+    at(R.p01, r) = make_uint2(w0, w1);
 }
 
 // neg * 2 + this lane's `bit`, as one v_addc_co_u32 whose carry-in is the
@@ -263,9 +279,10 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST int *sd, const To
         w0b[gi] = 0u;
         w1[gi] = 0u;
         if (!first) {
-            w0[gi] = at(R.p0, rix[gi]);
+            const uint2 w = at(R.p01, rix[gi]);
+            w0[gi] = w.x;
+            w1[gi] = w.y;
             if (M == 0) w0b[gi] = at(R.p0b, rix[gi]);
-            w1[gi] = at(R.p1, rix[gi]);
         }
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -294,7 +311,10 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST int *sd, const To
         // converted once per check: keeps the compiler from converting the
         // selected FP16/INT8 magnitude again for every edge (no instruction)
         asm volatile("" : "+v"(a1o), "+v"(a2o));
-        const unsigned idxo = w1[gi] >> RES_SIGN_LO;
+        // shifted out once per check: compared against inline constants
+        // (a byte-select compare per edge would need each constant in a VGPR)
+        unsigned idxo = w1[gi] >> RES_SIGN_LO;
+        asm volatile("" : "+v"(idxo));
         bool syn = false, par = false;
         unsigned idx = 0u, neg = 0u;
         float m1 = INFINITY, m2 = INFINITY;
@@ -327,15 +347,19 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST int *sd, const To
         }
         if (update && valid[gi]) {
             const unsigned sg = par ? ~neg : neg;
-            rec_put_mag<M>(R, rix[gi], m1 * alpha, m2 * alpha);
-            at(R.p1, rix[gi]) = (sg & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
+            // the edge index as it is: folding the shift into the selects
+            // would turn their inline constants into literals
+            unsigned ix = idx;
+            asm volatile("" : "+v"(ix));
+            rec_put<M>(R, rix[gi], m1 * alpha, m2 * alpha,
+                       (sg & ((1u << RES_SIGN_LO) - 1u)) | (ix << RES_SIGN_LO));
         }
         syn_total += (valid[gi] && syn) ? 1u : 0u;
     }
     return syn_total;
 }
 
-// Any check degree up to RES_MAX_DC (records of degree > 25 use the third
+// Any check degree up to RES_MAX_DC (records of degree > 24 use the third
 // word): the same steps as check_multi with a runtime edge loop.
 template <int M>
 __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const Rec &R, int g,
@@ -346,9 +370,10 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
     unsigned w0 = 0u, w0b = 0u, w1 = 0u;
     unsigned long long w2 = 0ull;
     if (!first) {
-        w0 = at(R.p0, rix);
+        const uint2 w = at(R.p01, rix);
+        w0 = w.x;
+        w1 = w.y;
         if (M == 0) w0b = at(R.p0b, rix);
-        w1 = at(R.p1, rix);
         if (dc > RES_SIGN_LO) w2 = at(R.p2, rix);
     }
     const float a1o = rec_a1<M>(w0, w0b, R);
@@ -383,15 +408,15 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
     }
     if (update && valid) {
         const unsigned sg_lo = par ? ~neg_lo : neg_lo;
-        rec_put_mag<M>(R, rix, m1 * alpha, m2 * alpha);
-        at(R.p1, rix) = (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO);
+        rec_put<M>(R, rix, m1 * alpha, m2 * alpha,
+                   (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
         if (dc > RES_SIGN_LO) at(R.p2, rix) = par ? ~neg_hi : neg_hi;
     }
     return valid ? syn : 0u;
 }
 
 // G variable tasks of degree D <= 16 whose entries are all full circulants
-// and all name check edges below 25 (record word 1 holds their signs): every
+// and all name check edges below 24 (record word 1 holds their signs): every
 // record and LLR load of the G tasks first, then the sums: FP16 / INT8 in
 // table order; FP32 in ascending check index.  Within a region the entries are sorted by u, so the ones whose
 // check position wrapped past 359 (p + u >= 360) are a suffix of the
@@ -404,7 +429,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
 template <int M, int D, int G, bool TA>
 __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
                                           const float *lam, const Rec &R, unsigned lane) {
-    unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G], pp4[G];
+    unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G], pp8[G];
     bool wr[G][D], glob[G];
     float lv[G];
     bool valid[G];
@@ -417,7 +442,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
         valid[gi] = p < (unsigned)ATSC3_Z;
         const unsigned pp = valid[gi] ? p : 0u;
         dst[gi] = pp;
-        pp4[gi] = 4u * pp;
+        pp8[gi] = 8u * pp;
         n0[gi] = TA ? (hw >> 11) & 63 : 0;
         row[gi] = (hw >> 17) & 255u;
         glob[gi] = (hw >> 25) & 1u;
@@ -429,23 +454,47 @@ __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
             if constexpr (M == 0) {
                 wr[gi][k] = pp + u >= (unsigned)ATSC3_Z;
                 const unsigned r = g * ATSC3_Z + pp + u - (wr[gi][k] ? (unsigned)ATSC3_Z : 0u);
-                w0[gi][k] = at(R.p0, r);
+                const uint2 w = at(R.p01, r);
+                w0[gi][k] = w.x;
+                w1[gi][k] = w.y;
                 w0b[gi][k] = at(R.p0b, r);
-                w1[gi][k] = at(R.p1, r);
             } else {
-                // byte offset within group g's records, wrapped (min(q, q - 1440)),
+                // byte offset within group g's records, wrapped (min(q, q - 2880)),
                 // on a wave-uniform base: add, subtract, min per entry
-                const unsigned q = pp4[gi] + 4u * u;
-                const unsigned ob = min(q, q - 4u * ATSC3_Z);
-                w0[gi][k] = at(R.p0 + g * ATSC3_Z, 0u, ob);
-                w1[gi][k] = at(R.p1 + g * ATSC3_Z, 0u, ob);
+                const unsigned q = pp8[gi] + 8u * u;
+                const unsigned ob = min(q, q - 8u * ATSC3_Z);
+                const uint2 w = at(R.p01 + g * ATSC3_Z, 0u, ob);
+                w0[gi][k] = w.x;
+                w1[gi][k] = w.y;
             }
         }
     }
     __builtin_amdgcn_sched_barrier(0);
+    // FP16: the 1.0 of the mixed-precision fma, hidden from constant folding
+    float one = 1.0f;
+    asm volatile("" : "+s"(one));
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
         float m[D];
+        float s = 0.0f;
+        if constexpr (M == 1) {
+            // message = the selected FP16 magnitude with the edge's sign
+            // (record bit sp) rotated into bit 15, added as s = fma(h, 1, s)
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const unsigned ew = (unsigned)sd[gi * (D + 1) + 1 + k];
+                const unsigned kk = (ew >> 17) & 127u, sp = ew >> 24;
+                const bool a2 = kk == (w1[gi][k] >> RES_SIGN_LO);
+                const unsigned hm = a2 ? (w0[gi][k] >> 16) : w0[gi][k];
+                const unsigned sg = __builtin_amdgcn_alignbit(w1[gi][k], w1[gi][k], (sp + 17u) & 31u);
+                // magnitudes are non-negative (bit 15 clear), so the sign is
+                // OR-ed in, as the FP32 forms do (bits 16..31 are not read)
+                const unsigned h = hm | (sg & 0x8000u);
+                s = __builtin_fmaf(h_val((unsigned short)h), one, s);
+            }
+            if (valid[gi]) tot_st<M>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const unsigned ew = (unsigned)sd[gi * (D + 1) + 1 + k];
@@ -454,14 +503,11 @@ __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
             float v;
             if constexpr (M == 0) {
                 v = __uint_as_float(a2 ? w0b[gi][k] : w0[gi][k]);
-            } else if constexpr (M == 1) {
-                v = h_val((unsigned short)(a2 ? (w0[gi][k] >> 16) : w0[gi][k]));
             } else {
                 v = (float)((a2 ? (w0[gi][k] >> 8) : w0[gi][k]) & 0xffu) * R.step;
             }
             m[k] = __uint_as_float(__float_as_uint(v) | ((w1[gi][k] << (31u - sp)) & 0x80000000u));
         }
-        float s = 0.0f;
         if constexpr (M != 0 || (!TA && D == 2)) {
 #pragma unroll
             for (int k = 0; k < D; ++k) s += m[k];
@@ -495,8 +541,8 @@ __device__ float var_term(const QC_CONST ResTab *ct, const Rec &R, int e, unsign
     const unsigned u = ew & 511u, g = (ew >> 9) & 255u, kk = (ew >> 17) & 127u, sp = ew >> 24;
     const unsigned rix = g * ATSC3_Z + wrap360(pp + u);
     const unsigned long long w2 = sp >= RES_SIGN_LO ? at(R.p2, rix) : 0ull;
-    const float mv = rec_msg<M>(at(R.p0, rix), M == 0 ? at(R.p0b, rix) : 0u, at(R.p1, rix),
-                                w2, kk, sp, R);
+    const uint2 w = at(R.p01, rix);
+    const float mv = rec_msg<M>(w.x, M == 0 ? at(R.p0b, rix) : 0u, w.y, w2, kk, sp, R);
     // a missing circulant entry adds +0.0: the sum (started at +0.0) is unchanged
     return vm != pp + 1u ? mv : 0.0f;
 }
@@ -542,9 +588,8 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
     const unsigned tid = threadIdx.x, lane = tid & 63u;
     const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
     Rec R;
-    R.p0 = a.p0 + (size_t)blockIdx.x * a.stride;
+    R.p01 = a.p01 + (size_t)blockIdx.x * a.stride;
     R.p0b = M == 0 ? a.p0b + (size_t)blockIdx.x * a.stride : nullptr;
-    R.p1 = a.p1 + (size_t)blockIdx.x * a.stride;
     R.p2 = a.p2 + (size_t)blockIdx.x * a.stride;
     Tot T;
     T.lds = lds;
@@ -708,13 +753,12 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
 }
 
 struct WsLayout {
-    int64_t stride, tstride, p0, p0b, p1, p2, totg, bytes;
+    int64_t stride, tstride, p01, p0b, p2, totg, bytes;
     WsLayout(int64_t grid, int64_t max_checks, int64_t max_grows, bool f32) {
         stride = (max_checks + 63) & ~(int64_t)63;
         tstride = f32 ? ((max_grows * ATSC3_Z + 63) & ~(int64_t)63) : 0;
-        p0 = 256;
-        p1 = p0 + grid * stride * 4;
-        p2 = p1 + grid * stride * 4;
+        p01 = 256;
+        p2 = p01 + grid * stride * 8;
         p0b = p2 + grid * stride * 8;
         totg = p0b + (f32 ? grid * stride * 4 : 0);
         bytes = totg + grid * tstride * 4;
@@ -771,8 +815,7 @@ int32_t resident_launch(const float *llr, int32_t nblk, const int64_t *blk_off,
     a.stride = L.stride;
     a.tstride = L.tstride;
     a.queue = reinterpret_cast<int *>(base);
-    a.p0 = reinterpret_cast<unsigned *>(base + L.p0);
-    a.p1 = reinterpret_cast<unsigned *>(base + L.p1);
+    a.p01 = reinterpret_cast<uint2 *>(base + L.p01);
     a.p2 = reinterpret_cast<unsigned long long *>(base + L.p2);
     a.p0b = f32 ? reinterpret_cast<unsigned *>(base + L.p0b) : nullptr;
     a.totg = f32 ? reinterpret_cast<float *>(base + L.totg) : nullptr;

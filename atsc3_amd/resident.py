@@ -82,6 +82,7 @@ def _pack_tasks(deg, generic, pack):
 
 
 F32_LDS_ROWS = 113       # FP32 posterior rows (360 floats) a workgroup holds in LDS
+SIGN_LO = 24             # edge signs a record's word 1 holds (the rest: word 2)
 
 
 @functools.lru_cache(maxsize=None)
@@ -105,7 +106,7 @@ def _host_tables(code, f32=False):
     edge_v[2e+1]  missing variable position + 1 (0 = full circulant)
     c_steps, c_tasks, v_steps, v_tasks  the work of each phase (_pack_tasks);
                   generic: a missing entry, degree > 16, or (variables) an
-                  entry naming a check edge >= 25 (its sign is in word 2)
+                  entry naming a check edge >= SIGN_LO (its sign is in word 2)
     """
     q = code.qc()
     cg = q["cg_ptr"].astype(np.int64)
@@ -129,7 +130,7 @@ def _host_tables(code, f32=False):
     vmissg = np.array([int((vmiss[vg[t]:vg[t + 1]] > 0).any()) for t in range(len(dv))])
     c_generic = (hasmiss > 0) | (dc > FAST_MAX)
     c_tasks, c_steps = _pack_tasks(dc, c_generic, CHECK_PACK)
-    v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= 25),
+    v_tasks, v_steps = _pack_tasks(dv, (vmissg > 0) | (dv > FAST_MAX) | (vkmax >= SIGN_LO),
                                    VAR_PACK_F32 if f32 else VAR_PACK)
     # a variable step with any group that has a first region (Type A: n0 > 0
     # entries summed before the rest) carries bit 23
