@@ -144,10 +144,15 @@ extern "C" int32_t ldpc_kernel_abi(void) { return 1; }
 
 #include <dlfcn.h>
 
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -381,10 +386,22 @@ const ResCode *resident_code(int device, uint64_t hash, int n, int nc, int ne, b
     return &g_res_dev.back();
 }
 
-int32_t resident_dropin(const ResCode *rc, int device, const float *llr, int32_t nblk, int32_t n,
-                        int32_t iters, float alpha, uint8_t *bits, uint8_t *conv, int32_t *its,
-                        int32_t *bad) {
+// One caller's share of a combined decode: its host arrays.
+struct DropCall {
+    const float *llr;
+    int32_t nblk;
+    uint8_t *bits, *conv;
+    int32_t *its, *bad;
+};
+
+// The resident decode of every call's blocks in one launch: LLRs staged
+// into pinned memory call after call, results copied back to each caller's
+// arrays only when the whole decode succeeded (nothing is written on error).
+int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t iters,
+                              float alpha, const std::vector<DropCall> &calls) {
     const ResEntry &e = *rc->e;
+    int32_t nblk = 0;
+    for (const DropCall &k : calls) nblk += k.nblk;
     const int cus = atsc3_minsum_resident_grid();
     if (cus <= 0) return ATSC3_E_HIP;
     const int grid = nblk < cus ? nblk : cus;
@@ -410,7 +427,11 @@ int32_t resident_dropin(const ResCode *rc, int device, const float *llr, int32_t
     do {
         if (!grow(c, d_need, h_need)) break;
         hipStream_t st = c->st;
-        std::memcpy(c->host, llr, nb * nn * 4);
+        size_t b0 = 0;
+        for (const DropCall &k : calls) {
+            std::memcpy(c->host + b0 * nn * 4, k.llr, (size_t)k.nblk * nn * 4);
+            b0 += (size_t)k.nblk;
+        }
         int64_t *hoff = reinterpret_cast<int64_t *>(c->host + o_off);
         int32_t *hcode = reinterpret_cast<int32_t *>(c->host + o_code);
         for (size_t b = 0; b < nb; ++b) {
@@ -440,15 +461,143 @@ int32_t resident_dropin(const ResCode *rc, int device, const float *llr, int32_t
             hipMemcpyAsync(hd, dbad, nb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             break;
-        std::memcpy(bits, hb, nb * nn);
-        std::memcpy(conv, hc, nb);
-        std::memcpy(its, hi, nb * 4);
-        std::memcpy(bad, hd, nb * 4);
+        b0 = 0;
+        for (const DropCall &k : calls) {
+            const size_t kb = (size_t)k.nblk;
+            std::memcpy(k.bits, hb + b0 * nn, kb * nn);
+            std::memcpy(k.conv, hc + b0, kb);
+            std::memcpy(k.its, hi + b0, kb * 4);
+            std::memcpy(k.bad, hd + b0, kb * 4);
+            b0 += kb;
+        }
         rc_out = ATSC3_OK;
     } while (0);
     (void)hipStreamSynchronize(c->st);
     release_ctx(c);
     return rc_out;
+}
+
+// Calls that arrive together are decoded together.  The reference drives
+// its kernel from a thread pool, each call a few blocks (lab/m9_fast.py:
+// 893-925): on the GPU every call would pay a launch's whole iteration
+// latency and its own transfers.  A call that finds an open batch of the
+// same code, iteration limit, alpha and device joins it and waits.  One
+// that finds none leads a new batch; when other threads finished a call of
+// the same code within kRecentUs and are idle, it keeps the batch open until
+// as many calls as those threads have joined (or kBatchCap blocks, or
+// kJoinWindowUs pass), then runs it and wakes the members.  A lone caller
+// never waits.  Every block is decoded as if alone (the
+// resident engine's codewords are independent), so results are those of
+// separate calls; a failed decode returns its status to every member,
+// writing nothing.
+constexpr int32_t kBatchCap = 256;
+constexpr int kJoinWindowUs = 500;
+constexpr int kRecentUs = 10000;
+constexpr int kRecentCalls = 32;
+
+struct DropKey {
+    const ResCode *rc;
+    int device, n;
+    int32_t iters;
+    uint32_t alpha_bits;
+    bool operator==(const DropKey &o) const {
+        return rc == o.rc && device == o.device && n == o.n && iters == o.iters &&
+               alpha_bits == o.alpha_bits;
+    }
+};
+
+struct DropBatch {
+    DropKey key;
+    std::vector<DropCall> calls;
+    int32_t total = 0;
+    bool closed = false, done = false;
+    int32_t status = ATSC3_E_HIP;
+    std::condition_variable cv;
+};
+
+struct RecentCall {
+    std::thread::id thread;
+    DropKey key;
+    std::chrono::steady_clock::time_point end;
+};
+
+std::mutex g_batch_mu;
+// batches still accepting calls; each batch is owned by its leader and its
+// members together, so a member can read the status after the leader left
+std::vector<std::shared_ptr<DropBatch>> g_open_batches;
+std::vector<std::pair<std::thread::id, DropKey>> g_in_flight;
+RecentCall g_recent[kRecentCalls];
+int g_recent_next = 0;
+
+// Other threads that may join a batch of `key` soon: those that finished a
+// call of `key` within kRecentUs and are not inside a call now (a thread in
+// flight with the same key leads or belongs to another batch already)
+// (g_batch_mu held).
+int other_threads(const DropKey &key, std::thread::id self) {
+    const auto now = std::chrono::steady_clock::now();
+    std::vector<std::thread::id> seen;
+    for (const RecentCall &r : g_recent) {
+        if (!(r.key == key) || r.thread == self ||
+            now - r.end >= std::chrono::microseconds(kRecentUs))
+            continue;
+        bool busy = false;
+        for (const auto &f : g_in_flight) busy = busy || f.first == r.thread;
+        if (!busy && std::find(seen.begin(), seen.end(), r.thread) == seen.end())
+            seen.push_back(r.thread);
+    }
+    return (int)seen.size();
+}
+
+void finish_call(const DropKey &key, std::thread::id self) {
+    g_in_flight.erase(std::find(g_in_flight.begin(), g_in_flight.end(),
+                                std::make_pair(self, key)));
+    g_recent[g_recent_next] = RecentCall{self, key, std::chrono::steady_clock::now()};
+    g_recent_next = (g_recent_next + 1) % kRecentCalls;
+}
+
+int32_t resident_dropin(const ResCode *rc, int device, const float *llr, int32_t nblk, int32_t n,
+                        int32_t iters, float alpha, uint8_t *bits, uint8_t *conv, int32_t *its,
+                        int32_t *bad) {
+    const DropCall mine{llr, nblk, bits, conv, its, bad};
+    DropKey key{rc, device, n, iters, 0u};
+    std::memcpy(&key.alpha_bits, &alpha, sizeof(float));
+    const std::thread::id self = std::this_thread::get_id();
+    std::unique_lock<std::mutex> lk(g_batch_mu);
+    g_in_flight.emplace_back(self, key);
+    for (const std::shared_ptr<DropBatch> &open : g_open_batches) {
+        DropBatch &b = *open;
+        if (b.closed || !(b.key == key) || b.total + nblk > kBatchCap) continue;
+        const std::shared_ptr<DropBatch> keep = open;
+        b.calls.push_back(mine);
+        b.total += nblk;
+        b.cv.notify_all();
+        b.cv.wait(lk, [&b] { return b.done; });
+        finish_call(key, self);
+        return b.status;
+    }
+    const std::shared_ptr<DropBatch> batch = std::make_shared<DropBatch>();
+    DropBatch &b = *batch;
+    b.key = key;
+    b.calls.push_back(mine);
+    b.total = nblk;
+    const int others = nblk < kBatchCap ? other_threads(key, self) : 0;
+    if (others > 0) {
+        g_open_batches.push_back(batch);
+        b.cv.wait_for(lk, std::chrono::microseconds(kJoinWindowUs), [&b, others] {
+            return (int)b.calls.size() > others || b.total >= kBatchCap;
+        });
+        g_open_batches.erase(std::find(g_open_batches.begin(), g_open_batches.end(), batch));
+    }
+    b.closed = true;
+    lk.unlock();
+    // no call joins a closed batch: its call list is fixed from here on
+    const int32_t st = resident_decode_calls(rc, device, n, iters, alpha, b.calls);
+    lk.lock();
+    b.status = st;
+    b.done = true;
+    finish_call(key, self);
+    b.cv.notify_all();
+    return st;
 }
 
 uint64_t csr_hash(int32_t n, int32_t nc, int32_t nedges, const int32_t *cptr,

@@ -41,7 +41,11 @@ int32_t ldpc_kernel_abi(void);
  * Returns -1 when a check degree exceeds 128, like the upstream kernel,
  * with nothing written.  Thread-safe: concurrent calls take separate
  * pooled contexts (stream, grow-only device and pinned buffers), so after
- * the first calls none allocates.                                          */
+ * the first calls none allocates.  Calls of an A/322 code that arrive
+ * together from several threads (same code, iters, alpha) are decoded in
+ * one launch, each caller getting its own blocks' results (a call waits up
+ * to 0.5 ms for other recently active threads to join; a lone caller never
+ * waits); a failure returns the same error to each, writing nothing.       */
 int32_t ldpc_minsum_f32(const float *llr, int32_t nblk, int32_t n, int32_t nc,
                         int32_t nedges, const int32_t *cptr,
                         const int32_t *cidx, const int32_t *vptr,

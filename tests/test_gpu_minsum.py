@@ -73,28 +73,38 @@ def test_host_dropin_generic_graphs(device):
 def test_host_dropin_threaded(device):
     """The upstream's calling pattern (lab/m3_ldpc.py:485-506,
     lab/m9_fast.py:893-925): a thread pool of 8 workers, each calling
-    ldpc_minsum_f32 on an 8-block sub-batch with its own buffers, two codes
-    interleaved, iters 0 among them; every result equals the C oracle."""
+    ldpc_minsum_f32 on a sub-batch with its own buffers; the library decodes
+    calls that arrive together in one launch, but only calls of the same
+    code, iteration limit and alpha: here two codes, three iteration limits
+    (0 among them), two alphas and sub-batches of 1 to 8 blocks interleave.
+    Every result equals the C oracle."""
     from concurrent.futures import ThreadPoolExecutor
     from atsc3_amd import ldpc
     jobs = []
-    for rate, n, snr, iters in (("10/15", 64800, 2.4, 50), ("6/15", 16200, 1.3, 30),
-                                ("10/15", 64800, 2.6, 0)):
+    for rate, n, snr, iters, alpha in (("10/15", 64800, 2.4, 50, 0.75),
+                                       ("6/15", 16200, 1.3, 30, 0.75),
+                                       ("10/15", 64800, 2.6, 0, 0.75),
+                                       ("10/15", 64800, 2.4, 50, 0.85),
+                                       ("10/15", 64800, 2.3, 20, 0.75)):
         code = get_code(rate, n)
         cw = synth.codewords(rate, n, 32, 700 + n % 7 + iters)
         llr = synth.llr_bpsk_like(cw, snr, 701 + iters)
-        want = OL.minsum_f32(llr, code, iters, 0.75)
-        for lo in range(0, 32, 8):
-            jobs.append((code, llr[lo:lo + 8], iters,
-                         tuple(np.asarray(w)[lo:lo + 8] for w in want)))
+        want = OL.minsum_f32(llr, code, iters, alpha)
+        lo = 0
+        for size in (8, 1, 5, 8, 3, 7):
+            jobs.append((code, llr[lo:lo + size], iters, alpha,
+                         tuple(np.asarray(w)[lo:lo + size] for w in want)))
+            lo += size
+    order = np.random.default_rng(5).permutation(len(jobs))
+    jobs = [jobs[i] for i in order]
     print(f"[gpu] drop-in threaded: {len(jobs)} calls on 8 threads", flush=True)
 
     def one(j):
-        code, llr, iters, _ = j
-        return ldpc.minsum_f32_host(llr, code.checks, code.n, iters, 0.75)
+        code, llr, iters, alpha, _ = j
+        return ldpc.minsum_f32_host(llr, code.checks, code.n, iters, alpha)
     with ThreadPoolExecutor(8) as ex:
         got = list(ex.map(one, jobs * 2))
-    for (code, llr, iters, want), g in zip(jobs * 2, got):
+    for (code, llr, iters, alpha, want), g in zip(jobs * 2, got):
         for x, y in zip(g, want):
             assert np.array_equal(np.asarray(x), np.asarray(y))
 
