@@ -419,6 +419,7 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
     const size_t o_its = o_conv + up256(nb);
     const size_t o_bad = o_its + up256(nb * 4);
     const size_t d_need = o_bad + up256(nb * 4);
+    // pinned host: blk_off | blk_code at the device offsets, then the outputs
     const size_t h_out = o_in_end;
     const size_t h_need = h_out + nb * nn + nb + 8 * nb;
     DropCtx *c = acquire_ctx(device);
@@ -427,18 +428,24 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
     do {
         if (!grow(c, d_need, h_need)) break;
         hipStream_t st = c->st;
+        // the LLRs straight from each caller's (pageable) arrays: the
+        // runtime's own staged copy beats a memcpy into pinned memory first
+        bool ok = true;
         size_t b0 = 0;
         for (const DropCall &k : calls) {
-            std::memcpy(c->host + b0 * nn * 4, k.llr, (size_t)k.nblk * nn * 4);
+            ok = ok && hipMemcpyAsync(c->dev + b0 * nn * 4, k.llr, (size_t)k.nblk * nn * 4,
+                                      hipMemcpyHostToDevice, st) == hipSuccess;
             b0 += (size_t)k.nblk;
         }
+        if (!ok) break;
         int64_t *hoff = reinterpret_cast<int64_t *>(c->host + o_off);
         int32_t *hcode = reinterpret_cast<int32_t *>(c->host + o_code);
         for (size_t b = 0; b < nb; ++b) {
             hoff[b] = (int64_t)(b * nn);
             hcode[b] = 0;
         }
-        if (hipMemcpyAsync(c->dev, c->host, o_in_end, hipMemcpyHostToDevice, st) != hipSuccess)
+        if (hipMemcpyAsync(c->dev + o_off, c->host + o_off, o_in_end - o_off,
+                           hipMemcpyHostToDevice, st) != hipSuccess)
             break;
         uint8_t *dbits = reinterpret_cast<uint8_t *>(c->dev + o_bits);
         uint8_t *dconv = reinterpret_cast<uint8_t *>(c->dev + o_conv);
