@@ -159,6 +159,9 @@ __device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, u
     if constexpr (M != 0) {
         return __half2float(reinterpret_cast<const __half *>(T.lds)[row * RES_PROW + p]);
     } else {
+        // compiled as one generic (flat) load of the selected pointer; a
+        // branch per edge instead makes every join wait for all loads in
+        // flight, which serialises the step's gathers
         return glob ? at(T.glob, row * ATSC3_Z + p)
                     : reinterpret_cast<const float *>(T.lds)[row * ATSC3_Z + p];
     }
