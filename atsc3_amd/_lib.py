@@ -100,6 +100,12 @@ _SIGS = {
     "atsc3_minsum_resident_llr_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                       c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_i32,
                                       c_vp, ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "atsc3_minsum_resident_team_ws_bytes": [c_i32, c_i32, c_i32, c_i32, c_i32],
+    "atsc3_minsum_resident_team_status_word": [],
+    "atsc3_minsum_resident_team_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
+                                       c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp,
+                                       c_i32, c_i32, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp],
     "atsc3_minsum_resident_noalloc_max_bytes": [],
     "atsc3_minsum_resident_noalloc_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                           c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp,
@@ -117,6 +123,7 @@ _SIGS = {
 
 # functions whose C return type is not int32_t
 _RESTYPES = {"atsc3_minsum_resident_ws_bytes": ctypes.c_int64,
+             "atsc3_minsum_resident_team_ws_bytes": ctypes.c_int64,
              "atsc3_minsum_utc_ws_bytes": ctypes.c_int64,
              "atsc3_minsum_resident_noalloc_max_bytes": ctypes.c_int64}
 
@@ -164,6 +171,7 @@ SIZE_QUERIES = ("atsc3_demap_ws_doubles", "atsc3_minsum_qc_ctl_words", "atsc3_ld
                 "atsc3_sp_ctl_words", "atsc3_cpe_ws_doubles", "atsc3_minsum_resident_ws_bytes",
                 "atsc3_minsum_resident_grid", "atsc3_minsum_resident_lds_rows",
                 "atsc3_minsum_utc_ws_bytes", "atsc3_minsum_resident_noalloc_max_bytes",
+                "atsc3_minsum_resident_team_ws_bytes", "atsc3_minsum_resident_team_status_word",
                 "atsc3_dropin_engine")
 
 

@@ -19,7 +19,11 @@ codewords, the batched engine when a few slow codewords would leave most
 compute units idle.  With ``xcd=True`` the batched engine's XCD-partitioned
 mode is a candidate too (slots = "xcd": every codeword of the largest part
 on each XCD): it wins when the rounds of a few slow codewords dominate, since
-its barriers span one XCD and need no L2 write-back.
+its barriers span one XCD and need no L2 write-back.  With ``team=True`` the
+resident engine's team mode is a candidate for each team size in
+``TEAM_SIZES`` (slots = "team", grid = the team size: workgroups of one XCD
+per codeword): it wins when a few slow codewords would otherwise run their
+iterations on one compute unit each.
 """
 from __future__ import annotations
 
@@ -58,7 +62,10 @@ def _store(d):
         pass
 
 
-def candidates(nblk, msg_dtype, xcd=False):
+TEAM_SIZES = (4, 8, 16)
+
+
+def candidates(nblk, msg_dtype, xcd=False, team=False):
     cap = max_grid(msg_dtype)
     grids = sorted({g for g in (cap, cap // 2, cap // 4, 128) if 1 <= g <= cap}, reverse=True)
     slots = sorted({max(1, s) for s in (nblk, (nblk + 1) // 2, (nblk + 3) // 4)}, reverse=True)
@@ -66,6 +73,8 @@ def candidates(nblk, msg_dtype, xcd=False):
     if xcd:
         capx = max_grid(msg_dtype, xcd=True)
         out += [("xcd", g) for g in sorted({capx, capx // 2} - {0}, reverse=True)]
+    if team:
+        out += [("team", t) for t in TEAM_SIZES]
     return out
 
 
@@ -83,20 +92,22 @@ def time_graph(g, reps):
 
 
 def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True, log=None,
-         resident=False, xcd=False):
+         resident=False, xcd=False, team=False):
     """build(slots, grid) -> object with .capture() and .plan; prepare(obj)
     loads the tuning input.  Returns (slots, grid, table of ms per candidate);
     slots == "resident" (grid None: one workgroup per compute unit) when the
-    resident engine won, slots == "xcd" when the XCD-partitioned mode did."""
+    resident engine won, slots == "xcd" when the XCD-partitioned mode did,
+    slots == "team" (grid = team size) when the resident team mode did."""
     dev = torch.cuda.get_device_name()
-    key = f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}{'x' if xcd else ''}"
+    key = (f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}{'x' if xcd else ''}"
+           f"{'t' if team else ''}")
     cache = _load() if use_cache else {}
     if key in cache:
         c = cache[key]
         return c["slots"], c["grid"], c.get("table", {})
     table = {}
     best = None
-    cands = candidates(nblk, msg_dtype, xcd) + ([("resident", None)] if resident else [])
+    cands = candidates(nblk, msg_dtype, xcd, team) + ([("resident", None)] if resident else [])
     for slots, grid in cands:
         obj = build(slots, grid)
         if prepare is not None:

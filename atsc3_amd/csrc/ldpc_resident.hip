@@ -148,10 +148,15 @@ __device__ __forceinline__ float h_val(unsigned short h) {
 }
 
 // Posterior rows: FP16 mode, row r of the LDS array; FP32 mode, row r of the
-// LDS array or (global) of this workgroup's global slot.
+// LDS array or (global) of this workgroup's global slot.  Team mode (several
+// workgroups on one codeword): the variable phase publishes the rows it
+// computes to `pub` (row r at r * 360 elements: FP16 halves, FP32 floats;
+// FP32 global rows stay in `glob`), and every member copies the published
+// rows into its LDS after the team barrier.
 struct Tot {
     unsigned char *lds;
     float *glob;
+    unsigned char *pub;
 };
 
 template <int M>
@@ -167,17 +172,23 @@ __device__ __forceinline__ float tot_ld(const Tot &T, unsigned row, bool glob, u
     }
 }
 
-template <int M>
+template <int M, bool PUB = false>
 __device__ __forceinline__ void tot_st(const Tot &T, unsigned row, bool glob, unsigned p,
                                        float v) {
     if constexpr (M != 0) {
-        __half *h = reinterpret_cast<__half *>(T.lds) + row * RES_PROW;
         const __half hv = __ushort_as_half(h_bits_sat(v));
-        h[p] = hv;
-        if (p < 64u) h[p + ATSC3_Z] = hv;
+        if constexpr (PUB) {
+            at(reinterpret_cast<__half *>(T.pub), row * ATSC3_Z + p) = hv;
+        } else {
+            __half *h = reinterpret_cast<__half *>(T.lds) + row * RES_PROW;
+            h[p] = hv;
+            if (p < 64u) h[p + ATSC3_Z] = hv;
+        }
     } else {
         if (glob)
             at(T.glob, row * ATSC3_Z + p) = v;
+        else if constexpr (PUB)
+            at(reinterpret_cast<float *>(T.pub), row * ATSC3_Z + p) = v;
         else
             reinterpret_cast<float *>(T.lds)[row * ATSC3_Z + p] = v;
     }
@@ -429,7 +440,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
 // (Type A codes, n0 > 0 entries); else one region.  With one region of two
 // entries the order does not matter: (0 + a) + b == (0 + b) + a in IEEE
 // arithmetic, signed zeros included.
-template <int M, int D, int G, bool TA>
+template <int M, int D, int G, bool TA, bool PUB>
 __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
                                           const float *lam, const Rec &R, unsigned lane) {
     unsigned w0[G][D], w0b[G][D], w1[G][D], dst[G], row[G], pp8[G];
@@ -495,7 +506,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
                 const unsigned h = hm | (sg & 0x8000u);
                 s = __builtin_fmaf(h_val((unsigned short)h), one, s);
             }
-            if (valid[gi]) tot_st<M>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
+            if (valid[gi]) tot_st<M, PUB>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
             continue;
         }
 #pragma unroll
@@ -530,7 +541,7 @@ __device__ __forceinline__ void var_multi(const QC_CONST int *sd, const Tot &T,
             for (int k = 0; k < D; ++k)
                 if (k >= n0[gi]) s += wr[gi][k] ? 0.0f : m[k];
         }
-        if (valid[gi]) tot_st<M>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
+        if (valid[gi]) tot_st<M, PUB>(T, row[gi], glob[gi], dst[gi], lv[gi] + s);
     }
 }
 
@@ -550,7 +561,7 @@ __device__ float var_term(const QC_CONST ResTab *ct, const Rec &R, int e, unsign
     return vm != pp + 1u ? mv : 0.0f;
 }
 
-template <int M>
+template <int M, bool PUB>
 __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float *lam,
                             const Rec &R, int t, unsigned p, bool valid) {
     const unsigned vw = (unsigned)ct->grp_v[t];
@@ -575,7 +586,109 @@ __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float
             }
         }
     }
-    if (valid) tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
+    if (valid) tot_st<M, PUB>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
+}
+
+// One CHECK phase over the check steps w0, w0 + wstep, ... (this wave's
+// share); returns the wave's count of unsatisfied checks.
+template <int M>
+__device__ __forceinline__ unsigned check_phase(const QC_CONST ResTab *ct, const Tot &T,
+                                                const Rec &R, int w0, int wstep, unsigned lane,
+                                                bool first, bool update, float alpha) {
+    unsigned unsat = 0u;
+    for (int si = w0; si < ct->nsc; si += wstep) {
+        const int sw = ct->c_steps[si];
+        const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
+        if ((sw >> 22) & 1) {
+            const int task = ct->c_tasks[start];
+            const int g = task & 255;
+            const unsigned j = (unsigned)(task >> 8) * 64u + lane;
+            const int gw = ct->grp_c[g];
+            unsat += check_generic<M>(ct, T, R, g, (gw >> 7) & 2047, gw & 127,
+                                        (gw >> 18) & 1, j, j < (unsigned)ATSC3_Z, first,
+                                        update, alpha);
+            continue;
+        }
+        const QC_CONST int *sd = ct->c_sdata + ct->c_soff[si];
+        switch (G * 128 + D) {
+#define RES_C1(DD)                                                                           \
+    case 128 + DD:                                                                           \
+        unsat += check_multi<M, DD, 1>(sd, T, R, lane, first, update, alpha);           \
+        break;
+#define RES_C2(DD)                                                                           \
+    case 256 + DD:                                                                           \
+        unsat += check_multi<M, DD, 2>(sd, T, R, lane, first, update, alpha);           \
+        break;
+            QC_CASES16(RES_C1)
+            QC_CASES16(RES_C2)
+#undef RES_C1
+#undef RES_C2
+            default: break;
+        }
+    }
+    return unsat;
+}
+
+// One VAR phase over the variable steps w0, w0 + wstep, ...; PUB: team mode
+// (posteriors published, see Tot).
+template <int M, bool PUB>
+__device__ __forceinline__ void var_phase(const QC_CONST ResTab *ct, const Tot &T,
+                                          const float *lam, const Rec &R, int w0, int wstep,
+                                          unsigned lane) {
+    for (int si = w0; si < ct->nsv; si += wstep) {
+        const int sw = ct->v_steps[si];
+        const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
+        if ((sw >> 22) & 1) {
+            const int task = ct->v_tasks[start];
+            const unsigned p = (unsigned)(task >> 8) * 64u + lane;
+            var_generic<M, PUB>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
+            continue;
+        }
+        const bool ta = M == 0 && ((sw >> 23) & 1);
+        const QC_CONST int *sd = ct->v_sdata + ct->v_soff[si];
+        switch ((ta ? 2048 : 0) + G * 128 + D) {
+#define RES_V1(DD)                                                      \
+    case 128 + DD:                                                      \
+        var_multi<M, DD, 1, false, PUB>(sd, T, lam, R, lane);       \
+        break;                                                          \
+    case 2048 + 128 + DD:                                               \
+        var_multi<M, DD, 1, true, PUB>(sd, T, lam, R, lane);        \
+        break;
+// pairs of degree 9..16 tasks: FP16 / INT8 only (FP32 keeps them single:
+// its two record words and ordered sums would not fit the register budget)
+#define RES_V2(DD)                                                      \
+    case 256 + DD:                                                      \
+        if constexpr (M != 0 || DD <= 8)                                \
+            var_multi<M, DD, 2, false, PUB>(sd, T, lam, R, lane);   \
+        break;                                                          \
+    case 2048 + 256 + DD:                                               \
+        if constexpr (M == 0 && DD <= 8)                                \
+            var_multi<M, DD, 2, true, PUB>(sd, T, lam, R, lane);    \
+        break;
+#define RES_V4(DD)                                                      \
+    case 512 + DD:                                                      \
+        var_multi<M, DD, 4, false, PUB>(sd, T, lam, R, lane);       \
+        break;                                                          \
+    case 2048 + 512 + DD:                                               \
+        var_multi<M, DD, 4, true, PUB>(sd, T, lam, R, lane);        \
+        break;
+// eight tasks of degree <= 3: FP16 / INT8 only
+#define RES_V8(DD)                                                      \
+    case 1024 + DD:                                                     \
+        if constexpr (M != 0)                                           \
+            var_multi<M, DD, 8, false, PUB>(sd, T, lam, R, lane);   \
+        break;
+            QC_CASES16(RES_V1)
+            QC_CASES16(RES_V2)
+            QC_CASES4(RES_V4)
+            RES_V8(1) RES_V8(2) RES_V8(3)
+#undef RES_V1
+#undef RES_V2
+#undef RES_V4
+#undef RES_V8
+            default: break;
+        }
+    }
 }
 
 __global__ void resident_init(int *queue) {
@@ -640,37 +753,8 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         for (int it = 0;; ++it) {
             if (tid == 0) s_unsat[(it + 1) & 1] = 0;
             const bool first = it == 0, update = it < a.iters;
-            unsigned unsat = 0u;
-            for (int si = wave; si < ct->nsc; si += RES_WAVES) {
-                const int sw = ct->c_steps[si];
-                const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
-                if ((sw >> 22) & 1) {
-                    const int task = ct->c_tasks[start];
-                    const int g = task & 255;
-                    const unsigned j = (unsigned)(task >> 8) * 64u + lane;
-                    const int gw = ct->grp_c[g];
-                    unsat += check_generic<M>(ct, T, R, g, (gw >> 7) & 2047, gw & 127,
-                                                (gw >> 18) & 1, j, j < (unsigned)ATSC3_Z, first,
-                                                update, a.alpha);
-                    continue;
-                }
-                const QC_CONST int *sd = ct->c_sdata + ct->c_soff[si];
-                switch (G * 128 + D) {
-#define RES_C1(DD)                                                                           \
-    case 128 + DD:                                                                           \
-        unsat += check_multi<M, DD, 1>(sd, T, R, lane, first, update, a.alpha);           \
-        break;
-#define RES_C2(DD)                                                                           \
-    case 256 + DD:                                                                           \
-        unsat += check_multi<M, DD, 2>(sd, T, R, lane, first, update, a.alpha);           \
-        break;
-                    QC_CASES16(RES_C1)
-                    QC_CASES16(RES_C2)
-#undef RES_C1
-#undef RES_C2
-                    default: break;
-                }
-            }
+            const unsigned unsat = check_phase<M>(ct, T, R, wave, RES_WAVES, lane, first,
+                                                  update, a.alpha);
             if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
             __syncthreads();
             const int total = s_unsat[it & 1];
@@ -693,60 +777,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 }
                 break;
             }
-            for (int si = wave; si < ct->nsv; si += RES_WAVES) {
-                const int sw = ct->v_steps[si];
-                const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
-                if ((sw >> 22) & 1) {
-                    const int task = ct->v_tasks[start];
-                    const unsigned p = (unsigned)(task >> 8) * 64u + lane;
-                    var_generic<M>(ct, T, lam, R, task & 255, p, p < (unsigned)ATSC3_Z);
-                    continue;
-                }
-                const bool ta = M == 0 && ((sw >> 23) & 1);
-                const QC_CONST int *sd = ct->v_sdata + ct->v_soff[si];
-                switch ((ta ? 2048 : 0) + G * 128 + D) {
-#define RES_V1(DD)                                                      \
-    case 128 + DD:                                                      \
-        var_multi<M, DD, 1, false>(sd, T, lam, R, lane);       \
-        break;                                                          \
-    case 2048 + 128 + DD:                                               \
-        var_multi<M, DD, 1, true>(sd, T, lam, R, lane);        \
-        break;
-// pairs of degree 9..16 tasks: FP16 / INT8 only (FP32 keeps them single:
-// its two record words and ordered sums would not fit the register budget)
-#define RES_V2(DD)                                                      \
-    case 256 + DD:                                                      \
-        if constexpr (M != 0 || DD <= 8)                                \
-            var_multi<M, DD, 2, false>(sd, T, lam, R, lane);   \
-        break;                                                          \
-    case 2048 + 256 + DD:                                               \
-        if constexpr (M == 0 && DD <= 8)                                \
-            var_multi<M, DD, 2, true>(sd, T, lam, R, lane);    \
-        break;
-#define RES_V4(DD)                                                      \
-    case 512 + DD:                                                      \
-        var_multi<M, DD, 4, false>(sd, T, lam, R, lane);       \
-        break;                                                          \
-    case 2048 + 512 + DD:                                               \
-        var_multi<M, DD, 4, true>(sd, T, lam, R, lane);        \
-        break;
-// eight tasks of degree <= 3: FP16 / INT8 only
-#define RES_V8(DD)                                                      \
-    case 1024 + DD:                                                     \
-        if constexpr (M != 0)                                           \
-            var_multi<M, DD, 8, false>(sd, T, lam, R, lane);   \
-        break;
-                    QC_CASES16(RES_V1)
-                    QC_CASES16(RES_V2)
-                    QC_CASES4(RES_V4)
-                    RES_V8(1) RES_V8(2) RES_V8(3)
-#undef RES_V1
-#undef RES_V2
-#undef RES_V4
-#undef RES_V8
-                    default: break;
-                }
-            }
+            var_phase<M, false>(ct, T, lam, R, wave, RES_WAVES, lane);
             __syncthreads();
         }
         // every thread has read s_unsat and tot before thread 0 starts the
@@ -766,6 +797,321 @@ struct WsLayout {
         totg = p0b + (f32 ? grid * stride * 4 : 0);
         bytes = totg + grid * tstride * 4;
     }
+};
+
+// ---------------------------------------------------------------------------
+// Team mode: one codeword decoded by a team of workgroups on one XCD.
+//
+// The resident kernel puts a codeword on one compute unit, so a frame whose
+// slowest codeword needs 50 iterations waits for 50 iterations of one CU.
+// Here `team` workgroups (one per CU, all on one XCD) share each codeword:
+// member m takes the check steps and variable steps m * 16 + wave, stride
+// 16 * team (the same steps the resident kernel gives its waves), the team's
+// records live in its slot in global memory, and every member keeps a full
+// copy of the posteriors in LDS.  Per iteration:
+//   CHECK (own share; old records, LDS posteriors) -> new records, the
+//     member's unsatisfied-check count added to the team's counter
+//   team barrier; total read; converged or at `iters` -> output (each member
+//     its share of the bits; member 0 conv/its/bad), next codeword
+//   VAR (own share; records of the whole team) -> posteriors published to
+//     the team's global rows
+//   team barrier; every member copies the published rows into its LDS
+// so the results are the resident kernel's, bit for bit (same steps, same
+// arithmetic, same order).
+//
+// Team barriers stay inside one XCD, whose workgroups share one L2: a writer
+// waits for its stores to complete in that L2 (s_waitcnt vmcnt(0)) and
+// arrives on the team's counter; a reader polls the counter and then drops
+// its CU's L1 (agent-scope acquire) before reading what the others wrote --
+// no L2 write-back.  Teams are formed from a census taken at launch: every
+// workgroup reads its XCD id (HW_REG_XCC_ID) and draws a rank on its XCD's
+// counter; after a counted one-time wait for the whole grid (every
+// workgroup must be co-resident: grid <= compute units, one workgroup per
+// CU), ranks r of XCD x form teams r / team (the last one smaller when the
+// XCD's count is not a multiple).  Every wait is bounded: a wait that runs
+// out sets the abort word, every workgroup leaves, and codewords never
+// finished keep its = -1 (bad = -1, conv 0) from the launch's init kernel.
+// ---------------------------------------------------------------------------
+#define RES_TEAM_MAX 32
+#define RES_XCDS 16                   // XCC_ID is a 4-bit field
+#define RES_SPIN_LIMIT (1u << 23)
+// header words of the team workspace (each group on its own 128-byte line)
+#define TH_QUEUE 0
+#define TH_ABORT 32
+#define TH_CENSUS 64                  // RES_XCDS counters
+#define TH_ARRIVE 96
+#define TH_GO 128
+#define TH_WORDS 160
+// per-team control block (words): barrier counter, then on another line the
+// three rotating unsatisfied-check counters and the two codeword words
+#define TC_BAR 0
+#define TC_UNSAT 32
+#define TC_BLK 36
+#define TC_WORDS 64
+
+struct TeamArgs {
+    unsigned *hdr;                    // TH_WORDS header words, then the control blocks
+    unsigned char *tpost;             // team posterior slots
+    long long pstride;                // bytes per team posterior slot
+    const QC_CONST int *order;        // queue position -> codeword, or NULL
+    int team;
+};
+
+// Wait until this wave's vector memory operations (stores and atomics
+// included) have completed at the L2: s_waitcnt vmcnt(0) (gfx9 counts stores
+// in vmcnt), expcnt and lgkmcnt left unconstrained.
+__device__ __forceinline__ void res_wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The XCD this wave runs on: HW_REG_XCC_ID (hardware register 20), bits [3:0].
+__device__ __forceinline__ unsigned res_xcc_id() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & (RES_XCDS - 1);
+}
+
+__device__ __forceinline__ unsigned ld_agent(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Spin (thread 0) until *w >= target; false (and the abort word set) when
+// the abort word is seen or the bound runs out.
+__device__ bool res_spin_until(const unsigned *w, unsigned target, unsigned *ab) {
+    unsigned spins = 0;
+    while (ld_agent(w) < target) {
+        if ((++spins & 15u) == 0u) {
+            if (ld_agent(ab)) return false;
+            if (spins >= RES_SPIN_LIMIT) {
+                st_agent(ab, 1u);
+                return false;
+            }
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// Team barrier number `epoch` of a team of `size` members; when `val` is
+// given, thread 0 reads it after the barrier into s_w[1].  Returns false
+// when the abort word is set.
+__device__ __forceinline__ bool team_barrier(unsigned *ctl, unsigned epoch, unsigned size,
+                                             unsigned *ab, const unsigned *val, unsigned *s_w) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    res_wait_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(ctl + TC_BAR, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = res_spin_until(ctl + TC_BAR, epoch * size, ab);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_w[0] = ok ? 0u : 1u;
+        if (val) s_w[1] = ld_agent(val);
+    }
+    __syncthreads();
+    return s_w[0] == 0u;
+}
+
+// Every member's LDS rows from the team's published rows: FP16 rows of 360
+// halves into the padded 424-half rows (positions 0..63 written twice), FP32
+// rows of 360 floats as they are; 16-byte chunks, 8 loads in flight per
+// thread.
+template <int M>
+__device__ __forceinline__ void team_refresh(unsigned char *lds, const unsigned char *pub,
+                                             int nrows) {
+    constexpr unsigned RB = M == 0 ? ATSC3_Z * 4 : ATSC3_Z * 2;   // published row bytes
+    constexpr unsigned LB = M == 0 ? ATSC3_Z * 4 : RES_PROW * 2;  // LDS row bytes
+    constexpr unsigned NC = RB / 16;
+    const unsigned total = (unsigned)nrows * NC;
+    for (unsigned c0 = threadIdx.x; c0 < total; c0 += 8u * RES_WG) {
+        uint4 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            // past the end: chunk 0 again (in bounds, not stored)
+            const unsigned c = c0 + (unsigned)q * RES_WG;
+            v[q] = at(reinterpret_cast<const uint4 *>(pub), c < total ? c : 0u);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned c = c0 + (unsigned)q * RES_WG;
+            if (c < total) {
+                const unsigned r = c / NC, k = c - r * NC;
+                *reinterpret_cast<uint4 *>(lds + r * LB + k * 16u) = v[q];
+                if (M != 0 && k < 8u) *reinterpret_cast<uint4 *>(lds + r * LB + RB + k * 16u) = v[q];
+            }
+        }
+    }
+}
+
+__global__ void resident_team_init(unsigned *hdr, int words, uint8_t *conv, int *its, int *bad,
+                                   int nblk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) hdr[i] = 0u;
+    if (i < nblk) {
+        conv[i] = 0;
+        its[i] = -1;
+        bad[i] = -1;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamArgs t) {
+    __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
+    __shared__ int s_unsat[2];
+    __shared__ int s_hist[M == 2 ? 256 : 1];
+    __shared__ unsigned s_cnt[RES_XCDS];
+    __shared__ unsigned s_w[4];
+    const unsigned tid = threadIdx.x, lane = tid & 63u;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    unsigned *hdr = t.hdr, *ab = t.hdr + TH_ABORT;
+
+    // census: rank on this XCD, then wait for the whole grid
+    const unsigned x = res_xcc_id();
+    if (tid == 0) {
+        s_w[2] = __hip_atomic_fetch_add(hdr + TH_CENSUS + x, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned old = __hip_atomic_fetch_add(hdr + TH_ARRIVE, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        bool ok = true;
+        if (old + 1u == gridDim.x) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            st_agent(hdr + TH_GO, 1u);
+        } else {
+            ok = res_spin_until(hdr + TH_GO, 1u, ab);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        for (int q = 0; q < RES_XCDS; ++q) s_cnt[q] = ld_agent(hdr + TH_CENSUS + q);
+        s_w[0] = ok ? 0u : 1u;
+    }
+    __syncthreads();
+    if (s_w[0]) return;
+    const unsigned tm = (unsigned)t.team;
+    const unsigned rank = (unsigned)__builtin_amdgcn_readfirstlane((int)s_w[2]);
+    unsigned slot = 0;
+    for (unsigned q = 0; q < x; ++q) slot += (s_cnt[q] + tm - 1u) / tm;
+    const unsigned nx = (unsigned)__builtin_amdgcn_readfirstlane((int)s_cnt[x]);
+    const unsigned tix = rank / tm, m = rank - tix * tm;
+    slot = (unsigned)__builtin_amdgcn_readfirstlane((int)(slot + tix));
+    const unsigned S = min(tm, nx - tix * tm);   // members of this team
+    unsigned *ctl = hdr + TH_WORDS + (size_t)slot * TC_WORDS;
+    const bool leader = m == 0u && tid == 0u;
+
+    Rec R;
+    R.p01 = a.p01 + (size_t)slot * a.stride;
+    R.p0b = M == 0 ? a.p0b + (size_t)slot * a.stride : nullptr;
+    R.p2 = a.p2 + (size_t)slot * a.stride;
+    Tot T;
+    T.lds = lds;
+    T.pub = t.tpost + (size_t)slot * t.pstride;
+    // FP32: the LDS rows' published copies first, then the global rows
+    T.glob = M == 0 ? reinterpret_cast<float *>(T.pub) + RES_F32_LDS_ROWS * ATSC3_Z : nullptr;
+    const int w0 = (int)m * RES_WAVES + wave, wst = (int)S * RES_WAVES;
+    unsigned epoch = 0;
+    for (unsigned cw = 0;; ++cw) {
+        if (leader)
+            st_agent(ctl + TC_BLK + (cw & 1u),
+                     (unsigned)__hip_atomic_fetch_add(hdr + TH_QUEUE, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT));
+        if (!team_barrier(ctl, ++epoch, S, ab, ctl + TC_BLK + (cw & 1u), s_w)) return;
+        const int q = __builtin_amdgcn_readfirstlane((int)s_w[1]);
+        if (q >= a.nblk) return;
+        // every member has read the last codeword's total: clear the counters
+        if (leader)
+            for (int k = 0; k < 3; ++k) st_agent(ctl + TC_UNSAT + k, 0u);
+        const int blk = t.order ? t.order[q] : q;
+        const QC_CONST ResTab *ct = &a.codes[a.blk_code[blk]];
+        const int nvg = ct->nvg;
+        const long long off = a.blk_off[blk];
+        const float *lam = a.llr + off;
+        if constexpr (M == 2) {
+            R.step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, ct->n, s_hist);
+            R.inv = 1.0f / R.step;
+        }
+        // tot = lam for this member's share of the 64-row tasks, published
+        for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int task = t8 + k * wst, tg = task / 6;
+                const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+                v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p]
+                                                               : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int task = t8 + k * wst, tg = task / 6;
+                const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+                if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
+                    const unsigned vw = (unsigned)ct->grp_v[tg];
+                    tot_st<M, true>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
+                }
+            }
+        }
+        const int nlds = M == 0 ? min(nvg, RES_F32_LDS_ROWS) : nvg;
+        if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
+        team_refresh<M>(lds, T.pub, nlds);
+        if (tid == 0) s_unsat[0] = 0;
+        __syncthreads();
+        for (int it = 0;; ++it) {
+            if (tid == 0) s_unsat[(it + 1) & 1] = 0;
+            const bool first = it == 0, update = it < a.iters;
+            const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update,
+                                                  a.alpha);
+            if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+            __syncthreads();
+            unsigned *cnt = ctl + TC_UNSAT + it % 3;
+            if (tid == 0 && s_unsat[it & 1])
+                __hip_atomic_fetch_add(cnt, (unsigned)s_unsat[it & 1], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (!team_barrier(ctl, ++epoch, S, ab, cnt, s_w)) return;
+            const int total = __builtin_amdgcn_readfirstlane((int)s_w[1]);
+            // counter (it + 1) % 3 was last read in iteration it - 2, before
+            // every member's second barrier of that iteration
+            if (leader) st_agent(ctl + TC_UNSAT + (it + 1) % 3, 0u);
+            if (total == 0 || it == a.iters) {
+                uint8_t *bout = a.bits + off;
+                for (int task = w0; task < nvg * 6; task += wst) {
+                    const int tg = task / 6;
+                    const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+                    const unsigned vw = (unsigned)ct->grp_v[tg];
+                    if (p < (unsigned)ATSC3_Z) {
+                        const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
+                        bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
+                        if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
+                    }
+                }
+                if (leader) {
+                    a.conv[blk] = (uint8_t)(total == 0);
+                    a.its[blk] = total == 0 ? it : a.iters;
+                    a.bad[blk] = total;
+                }
+                break;
+            }
+            var_phase<M, true>(ct, T, lam, R, w0, wst, lane);
+            if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
+            team_refresh<M>(lds, T.pub, nlds);
+            __syncthreads();
+        }
+    }
+}
+
+struct TeamLayout {
+    int64_t nslots, stride, pstride, hdr, p01, p0b, p2, tpost, bytes;
+    TeamLayout(int64_t grid, int64_t team, int64_t max_checks, int64_t max_grows, bool f32) {
+        // at most ceil(n_x / team) teams on each XCD x: grid / team + one per XCD
+        nslots = grid / team + RES_XCDS;
+        stride = (max_checks + 63) & ~(int64_t)63;
+        pstride = f32 ? ((RES_F32_LDS_ROWS + max_grows) * ATSC3_Z * 4 + 255) & ~(int64_t)255
+                      : ((RES_NMAX / ATSC3_Z) * ATSC3_Z * 2 + 255) & ~(int64_t)255;
+        hdr = 0;
+        const int64_t words = TH_WORDS + nslots * TC_WORDS;
+        p01 = (words * 4 + 255) & ~(int64_t)255;
+        p2 = p01 + nslots * stride * 8;
+        p0b = p2 + nslots * stride * 8;
+        tpost = p0b + (f32 ? nslots * stride * 4 : 0);
+        tpost = (tpost + 255) & ~(int64_t)255;
+        bytes = tpost + nslots * pstride;
+    }
+    int64_t words() const { return TH_WORDS + nslots * TC_WORDS; }
 };
 
 }  // namespace
@@ -897,6 +1243,84 @@ extern "C" int32_t atsc3_minsum_resident_llr_dev(
                                   max_global_rows, max_n, max_dc, max_dv, iters, alpha, msg_fmt,
                                   msg_scale, grid, ws, ws_bytes, bits, conv, its, bad, llr_out,
                                   atsc3::as_stream(stream));
+}
+
+// Team mode (see resident_team_kernel): the same decode, each codeword on a
+// team of `team` workgroups of one XCD.
+extern "C" int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t team,
+                                                       int32_t max_checks,
+                                                       int32_t max_global_rows,
+                                                       int32_t msg_fmt) {
+    if (grid < 1 || team < 1 || team > RES_TEAM_MAX || max_checks < 1 || max_global_rows < 0 ||
+        msg_fmt < 0 || msg_fmt > 2)
+        return ATSC3_E_ARG;
+    return atsc3::TeamLayout(grid, team, max_checks, max_global_rows, msg_fmt == 0).bytes;
+}
+
+// Index of the int32 abort word in the team workspace: nonzero after a
+// launch whose bounded waits ran out (its = -1 marks the codewords it left).
+extern "C" int32_t atsc3_minsum_resident_team_status_word(void) { return TH_ABORT; }
+
+extern "C" int32_t atsc3_minsum_resident_team_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
+    const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
+    int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, const int32_t *order, int32_t grid, int32_t team, void *ws,
+    int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its, int32_t *bad,
+    void *stream) {
+    const int32_t rc = atsc3::resident_check_args(llr, nblk, blk_off, blk_code, codes, ncode,
+                                                  max_checks, max_global_rows, max_n, max_dc,
+                                                  max_dv, iters, alpha, msg_fmt, grid, bits,
+                                                  conv, its, bad);
+    if (rc != ATSC3_OK) return rc;
+    if (team < 1 || team > RES_TEAM_MAX) return ATSC3_E_ARG;
+    if (nblk == 0) return ATSC3_OK;
+    if (!ws) return ATSC3_E_ARG;
+    const bool f32 = msg_fmt == 0;
+    const atsc3::TeamLayout L(grid, team, max_checks, max_global_rows, f32);
+    if (ws_bytes < L.bytes) return ATSC3_E_SIZE;
+    // every workgroup must be co-resident (one per compute unit): the census
+    // waits for the whole grid
+    const int32_t cus = atsc3_minsum_resident_grid();
+    if (cus < 0) return cus;
+    if (grid > cus) return ATSC3_E_SIZE;
+    char *base = static_cast<char *>(ws);
+    atsc3::ResArgs a{};
+    a.llr = llr;
+    a.blk_off = (const QC_CONST long long *)blk_off;
+    a.blk_code = (const QC_CONST int *)blk_code;
+    a.codes = (const QC_CONST atsc3::ResTab *)codes;
+    a.nblk = nblk;
+    a.iters = iters;
+    a.alpha = alpha;
+    a.msg_scale = msg_fmt == 2 ? msg_scale : nullptr;
+    a.stride = L.stride;
+    a.p01 = reinterpret_cast<uint2 *>(base + L.p01);
+    a.p2 = reinterpret_cast<unsigned long long *>(base + L.p2);
+    a.p0b = f32 ? reinterpret_cast<unsigned *>(base + L.p0b) : nullptr;
+    a.bits = bits;
+    a.conv = conv;
+    a.its = its;
+    a.bad = bad;
+    atsc3::TeamArgs t{};
+    t.hdr = reinterpret_cast<unsigned *>(base + L.hdr);
+    t.tpost = reinterpret_cast<unsigned char *>(base + L.tpost);
+    t.pstride = L.pstride;
+    t.order = (const QC_CONST int *)order;
+    t.team = team;
+    hipStream_t st = atsc3::as_stream(stream);
+    const int words = (int)L.words();
+    const int n0 = words > nblk ? words : nblk;
+    hipLaunchKernelGGL(atsc3::resident_team_init, dim3((n0 + 255) / 256), dim3(256), 0, st,
+                       t.hdr, words, conv, its, bad, nblk);
+    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    if (f32)
+        hipLaunchKernelGGL(atsc3::resident_team_kernel<0>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+    else if (msg_fmt == 1)
+        hipLaunchKernelGGL(atsc3::resident_team_kernel<1>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+    else
+        hipLaunchKernelGGL(atsc3::resident_team_kernel<2>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+    return atsc3::launch_status();
 }
 
 // ---------------------------------------------------------------------------

@@ -66,11 +66,13 @@ class MixedPipeline:
     """Demap + batched decode of frames with a fixed block layout."""
 
     def __init__(self, specs, frames=1, device="cuda", iters=50, alpha=0.75,
-                 msg_dtype="float32", slots=None, grid=None, engine="batch"):
+                 msg_dtype="float32", slots=None, grid=None, engine="batch", team=None):
         """engine "batch": the persistent batched engine (slots, grid);
         "xcd": the same engine partitioned per XCD (slots over all XCDs);
         "resident": one workgroup per codeword (grid = workgroups, slots
-        unused)."""
+        unused); "team": the resident engine with each codeword on a team
+        of ``team`` workgroups of one XCD (one workgroup per compute
+        unit)."""
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -86,11 +88,15 @@ class MixedPipeline:
         if engine == "resident":
             from .resident import ResidentPlan
             self.plan = ResidentPlan(codes, self.device, iters, alpha, grid, msg_dtype)
+        elif engine == "team":
+            from .resident import ResidentPlan
+            self.plan = ResidentPlan(codes, self.device, iters, alpha, None, msg_dtype,
+                                     team=8 if team is None else team)
         elif engine in ("batch", "xcd"):
             self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid,
                                   xcd=engine == "xcd")
         else:
-            raise ValueError(f"engine {engine!r}: 'batch', 'xcd' or 'resident'")
+            raise ValueError(f"engine {engine!r}: 'batch', 'xcd', 'resident' or 'team'")
         self.engine = engine
         self.demappers = {}
         self.cells, self.sigma2 = [], []

@@ -82,6 +82,7 @@ def _pack_tasks(deg, generic, pack):
 
 
 F32_LDS_ROWS = 113       # FP32 posterior rows (360 floats) a workgroup holds in LDS
+TEAM_MAX = 32            # workgroups per codeword in team mode
 SIGN_LO = 24             # edge signs a record's word 1 holds (the rest: word 2)
 
 
@@ -256,11 +257,16 @@ class ResidentPlan:
     its bits the same range of ``bits``."""
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
-                 msg_dtype="float16", msg_scale=None):
+                 msg_dtype="float16", msg_scale=None, team=None, order=None):
         """msg_dtype "float16": the FP16 form; "int8": INT8 record magnitudes
         with a per-codeword step (msg_scale, one per codeword, or None: 6 *
         median|llr| / 127 on the device); "float32": the plain decoder,
-        bit-exact with the reference kernel."""
+        bit-exact with the reference kernel.
+
+        team: None (one workgroup per codeword), or 1..32: team mode, each
+        codeword decoded by that many workgroups of one XCD (same results;
+        grid at most one workgroup per compute unit).  order: team mode's
+        queue order (codeword indices), or None (0, 1, ...)."""
         fmts = {"float32": 0, "float16": 1, "int8": 2}
         if msg_dtype not in fmts:
             raise ValueError("the resident engine stores float32, float16 or int8 records")
@@ -288,7 +294,14 @@ class ResidentPlan:
         g = default_grid() if grid is None else int(grid)
         if g < 1:
             raise ValueError("grid must be >= 1")
-        self.grid = max(1, min(g, self.nblk))
+        self.team = None if team is None else int(team)
+        if self.team is None:
+            self.grid = max(1, min(g, self.nblk))
+        else:
+            if not 1 <= self.team <= TEAM_MAX:
+                raise ValueError(f"team must be 1..{TEAM_MAX}")
+            # every workgroup must be co-resident (one per compute unit)
+            self.grid = min(g, default_grid())
         dev = self.device
         self.blk_code = torch.from_numpy(self.code_idx).to(dev)
         self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
@@ -297,10 +310,22 @@ class ResidentPlan:
         self.conv = torch.empty(self.nblk, dtype=torch.uint8, device=dev)
         self.its = torch.empty(self.nblk, dtype=torch.int32, device=dev)
         self.bad = torch.empty(self.nblk, dtype=torch.int32, device=dev)
-        nbytes = _lib.lib().atsc3_minsum_resident_ws_bytes(self.grid, self.bank.max_checks,
-                                                            self.bank.max_global_rows, self.fmt)
+        if self.team is None:
+            nbytes = _lib.lib().atsc3_minsum_resident_ws_bytes(
+                self.grid, self.bank.max_checks, self.bank.max_global_rows, self.fmt)
+        else:
+            nbytes = _lib.lib().atsc3_minsum_resident_team_ws_bytes(
+                self.grid, self.team, self.bank.max_checks, self.bank.max_global_rows, self.fmt)
         if nbytes <= 0:
             raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
+        self.order = None
+        if order is not None:
+            if self.team is None:
+                raise ValueError("order applies to team mode only")
+            o = np.asarray(order, np.int64)
+            if o.shape != (self.nblk,) or not np.array_equal(np.sort(o), np.arange(self.nblk)):
+                raise ValueError("order must be a permutation of the codeword indices")
+            self.order = torch.from_numpy(o.astype(np.int32)).to(dev)
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
         self.ws_bytes = int(nbytes)
         self.msg_scale = None
@@ -334,7 +359,13 @@ class ResidentPlan:
                 self.grid, self.ws.data_ptr(),
                 self.ws_bytes, bits.data_ptr(),
                 conv.data_ptr(), its.data_ptr(), bad.data_ptr())
-        if post is None:
+        if self.team is not None:
+            if post is not None:
+                raise ValueError("team mode writes no posteriors")
+            _lib.call("atsc3_minsum_resident_team_dev", *args[:15],
+                      None if self.order is None else self.order.data_ptr(), self.grid,
+                      self.team, *args[16:], _stream_ptr(self.device))
+        elif post is None:
             _lib.call("atsc3_minsum_resident_dev", *args, _stream_ptr(self.device))
         else:
             if (post.numel() != self.total or post.dtype != torch.float32
@@ -345,8 +376,15 @@ class ResidentPlan:
         return bits, conv, its, bad
 
     def check(self):
-        """Interface of BatchPlan.check: the resident kernel has no grid
-        barrier and always completes every codeword, so nothing can abort."""
+        """Interface of BatchPlan.check (synchronises).  One workgroup per
+        codeword never waits for another, so nothing can abort; team mode
+        raises if a bounded team wait ran out (codewords left undecoded hold
+        its = -1)."""
+        if self.team is None:
+            return None
+        w = _lib.lib().atsc3_minsum_resident_team_status_word()
+        if int(self.ws[4 * w:4 * w + 4].view(torch.int32).item()) != 0:
+            raise RuntimeError("resident team decode aborted: a bounded team wait ran out")
         return None
 
     def rounds(self):
@@ -390,10 +428,12 @@ _PLANS_MAX = 8
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16",
-                        msg_scale=None):
+                        msg_scale=None, team=None):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
     msg_dtype "float32" is the plain decoder (bit-exact with the reference);
-    "int8" takes msg_scale (one step per codeword) or derives it.
+    "int8" takes msg_scale (one step per codeword) or derives it.  team:
+    team mode (each codeword on that many workgroups of one XCD; same
+    results).
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -416,14 +456,14 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     # a plan's workspace (queue word, records) serves one stream at a time:
     # the stream is part of the key
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
-           _stream_ptr(dev))
+           _stream_ptr(dev), team)
     if msg_scale is not None:
         # caller-given steps: a plan of its own (not cached)
-        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale, team)
         return plan.run(llr.reshape(-1), (bits, conv, its, bad))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
-        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, team=team)
     _PLANS[key] = plan
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))

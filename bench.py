@@ -334,9 +334,13 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     nblk = len(specs) * F
 
     def build(fmt):
-        return lambda slots, grid: MixedPipeline(
-            specs, F, dev, 50, 0.75, fmt, None if slots in ("resident", "xcd") else slots, grid,
-            engine=slots if slots in ("resident", "xcd") else "batch")
+        def make(slots, grid):
+            if slots == "team":
+                return MixedPipeline(specs, F, dev, 50, 0.75, fmt, engine="team", team=grid)
+            return MixedPipeline(
+                specs, F, dev, 50, 0.75, fmt, None if slots in ("resident", "xcd") else slots,
+                grid, engine=slots if slots in ("resident", "xcd") else "batch")
+        return make
 
     def prepare(p):
         src = MixedSource(p, 8, args.seed + 5)
@@ -347,7 +351,7 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
     for fmt in formats:
         slots, grid, table = autotune.tune(f"mixed{len(specs)}x{F}", nblk, fmt, build(fmt),
                                            prepare, use_cache=False, log=log, resident=True,
-                                           xcd=True)
+                                           xcd=True, team=True)
         pipe = build(fmt)(slots, grid)
         src = MixedSource(pipe, 8, args.seed + 5)
         errs, blocks, its, disagree = 0, 0, [], 0

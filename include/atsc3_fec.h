@@ -374,6 +374,44 @@ int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
                                       float *llr_out, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Resident min-sum, team mode: the same decode calls (lab/m9_fast.py:843-925,
+ * lab/m44_ldm.py:2497-2576 over lab/ldpc_kernel.c:65-146) for frames whose
+ * few slow codewords would otherwise leave one compute unit decoding alone:
+ * each codeword is decoded by a team of `team` (1..32) workgroups on one
+ * XCD, which split its check and variable steps and synchronise only with
+ * each other (two team barriers per iteration, no L2 write-back); every
+ * member keeps the whole posterior vector in LDS.  Results are those of
+ * atsc3_minsum_resident_dev with the same codes, msg_fmt and msg_scale, bit
+ * for bit.  Teams are formed at launch from the workgroups each XCD holds,
+ * so every workgroup must be co-resident: grid <= atsc3_minsum_resident_grid()
+ * (one workgroup per compute unit; larger grids return ATSC3_E_SIZE).
+ *   order  NULL, or nblk codeword indices: the queue's order (a team takes
+ *          the codewords order[0], order[1], ... as it becomes free)
+ *   ws     atsc3_minsum_resident_team_ws_bytes(grid, team, max_checks,
+ *          max_global_rows, msg_fmt) bytes: control words, per-team records
+ *          and published posterior rows
+ * Every wait is bounded: if one runs out (e.g. another kernel holds compute
+ * units for seconds), the int32 word atsc3_minsum_resident_team_status_word()
+ * of ws is nonzero after the launch and the codewords left undecoded hold
+ * its = -1, bad = -1, conv = 0.  Two launches on `stream` (init, decode);
+ * runs sharing one ws must not overlap.  No upstream counterpart as an
+ * entry point (the upstream's decode calls map onto it as above).
+ * ------------------------------------------------------------------------ */
+int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t team,
+                                            int32_t max_checks,
+                                            int32_t max_global_rows,
+                                            int32_t msg_fmt);
+int32_t atsc3_minsum_resident_team_status_word(void);
+int32_t atsc3_minsum_resident_team_dev(
+    const float *llr, int32_t nblk, const int64_t *blk_off,
+    const int32_t *blk_code, const void *codes, int32_t ncode,
+    int32_t max_checks, int32_t max_global_rows, int32_t max_n, int32_t max_dc,
+    int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
+    const float *msg_scale, const int32_t *order, int32_t grid, int32_t team,
+    void *ws, int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
+    int32_t *bad, void *stream);
+
+/* ------------------------------------------------------------------------
  * Resident min-sum with no allocation.  No counterpart in the upstream: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same

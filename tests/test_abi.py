@@ -184,6 +184,59 @@ def test_resident_entry_rejects_bad_arguments():
     assert call(ws_bytes=ws_need - 1) == E_SIZE
 
 
+def test_resident_team_entry_rejects_bad_arguments():
+    """atsc3_minsum_resident_team_dev: the resident entry's checks, a team
+    size outside 1..32 and a short workspace, all before any device call; an
+    empty batch is a no-op."""
+    L = _lib().lib()
+    E_ARG, E_SIZE, OK = -2, -4, 0
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    ws_need = L.atsc3_minsum_resident_team_ws_bytes(16, 8, 21600, 0, 1)
+
+    def call(nblk=1, team=8, alpha=0.75, fmt=1, ptr=p, ws_bytes=None):
+        return L.atsc3_minsum_resident_team_dev(ptr, nblk, ptr, ptr, ptr, 1, 21600, 0, 64800,
+                                                14, 13, 50, alpha, fmt, None, None, 16, team,
+                                                ptr, ws_need if ws_bytes is None else ws_bytes,
+                                                ptr, ptr, ptr, ptr, None)
+    assert call(nblk=-1) == E_ARG
+    assert call(alpha=-1.0) == E_ARG
+    assert call(fmt=3) == E_ARG
+    assert call(team=0) == E_ARG
+    assert call(team=33) == E_ARG
+    assert call(nblk=0, ptr=None) == OK
+    assert call(ptr=None) == E_ARG
+    assert call(ws_bytes=ws_need - 1) == E_SIZE
+    assert L.atsc3_minsum_resident_team_ws_bytes(16, 0, 21600, 0, 1) == E_ARG
+    assert L.atsc3_minsum_resident_team_ws_bytes(16, 33, 21600, 0, 1) == E_ARG
+
+
+def test_resident_team_workspace_holds_every_team():
+    """Teams are formed per XCD (ranks r of XCD x: team r // team), so a
+    launch can have up to sum_x ceil(n_x / team) teams; the workspace has a
+    slot for each of them for any split of the grid over 16 XCD ids: each
+    extra slot costs one team's records and posterior rows."""
+    L = _lib().lib()
+    rng = np.random.default_rng(3)
+    for grid in (1, 7, 64, 256):
+        for team in (1, 3, 8, 32):
+            ws = {fmt: L.atsc3_minsum_resident_team_ws_bytes(grid, team, 21600, 67, fmt)
+                  for fmt in (0, 1, 2)}
+            one = {fmt: L.atsc3_minsum_resident_team_ws_bytes(grid, team, 21600, 67, fmt)
+                   - L.atsc3_minsum_resident_team_ws_bytes(max(grid - team, 1), team, 21600,
+                                                           67, fmt) for fmt in (0, 1)}
+            for _ in range(50):
+                counts = np.bincount(rng.integers(0, 16, grid), minlength=16)
+                teams = int(sum(-(-c // team) for c in counts))
+                assert teams <= grid // team + 16
+            # FP16 slot: records (2 x 8 bytes per check) + 180 rows of 360 halves
+            per_slot = 2 * 8 * 21600 + 180 * 360 * 2
+            assert ws[1] >= (grid // team + 16) * per_slot
+            assert ws[0] > ws[1] and ws[2] == ws[1]
+            if grid > team:
+                assert one[1] >= per_slot
+
+
 def test_dropin_degree_limit_writes_nothing():
     """ldpc_minsum_f32 returns -1 when a check degree exceeds 128, as
     lab/ldpc_kernel.c:73-80 does, before any device work, and leaves every

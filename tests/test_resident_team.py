@@ -1,0 +1,216 @@
+"""Resident min-sum, team mode (atsc3_minsum_resident_team_dev).
+
+Each codeword is decoded by a team of workgroups on one XCD that split its
+check and variable steps; the results must be those of the one-workgroup
+resident kernel bit for bit, so every test compares with the oracle of the
+resident decode (oracle/resident_ref.py, FP16 / INT8 storage), the
+reference kernel's golden outputs (FP32 storage) or the C restatement of the
+reference kernel (oracle/minsum_oracle.c):
+
+* mixed codes in one queue (fixture blocks and synthetic blocks of the codes
+  that take the generic paths) for team sizes 1, 3 (an uneven split of every
+  XCD's workgroups: a smaller last team), 8 and 32 (a whole XCD per
+  codeword), FP16;
+* every golden fixture in FP32 with team size 8, one launch per (iters,
+  alpha) setting, and the C oracle on the synthetic codes;
+* INT8 with the oracle's steps given, and the device-derived steps equal to
+  the explicit ones;
+* fewer codewords than teams, one codeword, iters = 0, an empty batch, a
+  queue order given as a permutation;
+* the uneven-frame pipeline captured in a HIP graph and replayed: equal to
+  the one-workgroup resident engine and, in FP32, to the batched engine.
+"""
+import numpy as np
+import pytest
+
+from oracle.resident_ref import int8_step, resident_minsum
+from tests import _fixtures as F
+from tests.test_resident import SYNTH, _case, _oracle
+
+MIXED_KEYS = [("fixture", "g16k_r2"), ("synth", "13/15 64800 5.5"), ("fixture", "s16k_r9_near"),
+              ("synth", "7/15 64800 0.8"), ("fixture", "g64k_r10_nuc20"),
+              ("synth", "8/15 16200 2.0"), ("synth", "2/15 64800 -4.0"),
+              ("fixture", "s16k_r4_a75")]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_built():
+    F.ensure_oracle_built()
+
+
+def _mixed_blocks(seed=5):
+    blocks = []
+    for kind, key in MIXED_KEYS:
+        code, llr, iters, alpha = _case(kind, key)
+        assert (iters, alpha) == (50, 0.75), key
+        want = _oracle(kind, key)
+        for b in range(len(llr)):
+            blocks.append((code, llr[b], [w[b] for w in want], f"{key}#{b}"))
+    order = np.random.default_rng(seed).permutation(len(blocks))
+    return [blocks[i] for i in order]
+
+
+def _run_plan(plan, blocks, device):
+    import torch
+    llr = np.concatenate([b[1] for b in blocks]).astype(np.float32)
+    out = plan.run(torch.from_numpy(llr).to(device))
+    torch.cuda.synchronize()
+    plan.check()
+    return [x.cpu().numpy() for x in out]
+
+
+def _check_blocks(plan, blocks, got, tag):
+    bits, conv, its, bad = got
+    for i, (code, _, want, name) in enumerate(blocks):
+        g = (bits[plan.off[i]:plan.off[i + 1]], bool(conv[i]), its[i], bad[i])
+        assert g[1] == bool(want[1]) and g[2] == want[2] and g[3] == want[3], f"{tag} {name}"
+        assert np.array_equal(g[0], want[0]), f"{tag} {name} bits"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("team", [1, 3, 8, 32])
+def test_team_fp16_mixed_codes_equal_oracle(device, team):
+    from atsc3_amd.resident import ResidentPlan
+    blocks = _mixed_blocks()
+    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="float16",
+                        team=team)
+    print(f"[gpu] team fp16 mixed {len(blocks)} blocks, team {team}, grid {plan.grid}",
+          flush=True)
+    _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"team {team}")
+
+
+@pytest.mark.gpu
+def test_team_fp32_equals_reference_fixtures(device):
+    """Every golden fixture, decoded in FP32 team mode (team 8), equals the
+    reference kernel's outputs; one mixed launch per (iters, alpha)."""
+    from atsc3_amd.resident import ResidentPlan
+    groups = {}
+    for name in F.fixture_names():
+        code, llr, iters, alpha, want = F.load(name)
+        for b in range(len(llr)):
+            groups.setdefault((iters, alpha), []).append(
+                (code, llr[b], [want[k][b] for k in ("bits", "conv", "its", "bad")],
+                 f"{name}#{b}"))
+    for (iters, alpha), blocks in sorted(groups.items()):
+        plan = ResidentPlan([b[0] for b in blocks], device, iters, alpha, msg_dtype="float32",
+                            team=8)
+        print(f"[gpu] team fp32 fixtures iters {iters} alpha {alpha}: {len(blocks)} blocks",
+              flush=True)
+        _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"fp32 {iters} {alpha}")
+
+
+@pytest.mark.gpu
+def test_team_fp32_equals_c_oracle(device):
+    """The synthetic codes of the generic paths (check degree > 16 and > 24,
+    variable degree > 16, Type A) in one FP32 team launch (team 5)."""
+    from oracle.ldpc import minsum_f32
+    from atsc3_amd.resident import ResidentPlan
+    blocks = []
+    for r, n, s in SYNTH:
+        code, llr, iters, alpha = _case("synth", f"{r} {n} {s}")
+        want = minsum_f32(llr, code, iters, alpha)
+        for b in range(len(llr)):
+            blocks.append((code, llr[b], [np.asarray(w)[b] for w in want], f"{r} {n}#{b}"))
+    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="float32", team=5)
+    print(f"[gpu] team fp32 synthetic codes {len(blocks)} blocks", flush=True)
+    _check_blocks(plan, blocks, _run_plan(plan, blocks, device), "fp32 synth")
+
+
+@pytest.mark.gpu
+def test_team_int8_equals_oracle(device):
+    """INT8 records with the oracle's per-codeword steps given: bit-exact;
+    the steps derived on the device (each member derives its codeword's)
+    give the same outputs."""
+    from atsc3_amd.resident import ResidentPlan
+    blocks = []
+    for kind, key in [("fixture", "g16k_r4"), ("fixture", "s16k_r4_a75"),
+                      ("fixture", "g64k_r10_nuc20"), ("synth", "13/15 64800 5.5"),
+                      ("synth", "7/15 64800 0.8")]:
+        code, llr, iters, alpha = _case(kind, key)
+        assert (iters, alpha) == (50, 0.75), key
+        steps = np.array([int8_step(r) for r in llr], np.float32)
+        want = resident_minsum(llr, code, iters, alpha, "int8", steps=steps)
+        for b in range(len(llr)):
+            blocks.append((code, llr[b], [w[b] for w in want], f"{key}#{b}", steps[b]))
+    steps = np.array([b[4] for b in blocks], np.float32)
+    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="int8",
+                        msg_scale=steps, team=8)
+    print(f"[gpu] team int8 {len(blocks)} blocks", flush=True)
+    got = _run_plan(plan, blocks, device)
+    _check_blocks(plan, [b[:4] for b in blocks], got, "int8")
+    auto = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="int8", team=8)
+    got2 = _run_plan(auto, blocks, device)
+    for a, b in zip(got, got2):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float16", "float32"])
+def test_team_edge_cases(device, msg):
+    """One codeword on a team of 32 (every other team finds the queue
+    empty); iters = 0 (the syndrome of the channel decisions only); an
+    empty batch; a queue order given as a permutation."""
+    import torch
+    from atsc3_amd.resident import ResidentPlan, decode_resident_dev
+    code, llr, iters, alpha = _case("fixture", "g64k_r10_nuc20")
+    t = torch.from_numpy(llr).to(device)
+    print(f"[gpu] team edge cases {msg}", flush=True)
+    ref = [x.cpu().numpy() for x in decode_resident_dev(t, code, iters, alpha, msg_dtype=msg)]
+    one = decode_resident_dev(t[:1].contiguous(), code, iters, alpha, msg_dtype=msg, team=32)
+    torch.cuda.synchronize()
+    for a, b in zip(one, ref):
+        assert np.array_equal(a.cpu().numpy(), b[:1])
+    z_ref = [x.cpu().numpy() for x in decode_resident_dev(t, code, 0, alpha, msg_dtype=msg)]
+    z = decode_resident_dev(t, code, 0, alpha, msg_dtype=msg, team=4)
+    torch.cuda.synchronize()
+    for a, b in zip(z, z_ref):
+        assert np.array_equal(a.cpu().numpy(), b)
+    empty = decode_resident_dev(t[:0].contiguous(), code, iters, alpha, msg_dtype=msg, team=8)
+    assert all(x.shape[0] == 0 for x in empty)
+    nblk = len(llr)
+    order = np.random.default_rng(1).permutation(nblk)
+    plan = ResidentPlan([code] * nblk, device, iters, alpha, msg_dtype=msg, team=2, order=order)
+    got = [x.cpu().numpy() for x in plan.run(t.reshape(-1))]
+    torch.cuda.synchronize()
+    plan.check()
+    assert np.array_equal(got[0].reshape(nblk, -1), ref[0])
+    for a, b in zip(got[1:], ref[1:]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float32", "float16"])
+def test_team_mixed_pipeline_graph_replay(device, msg):
+    """Two uneven frames (every code and constellation of the mixed
+    workload), demapped and decoded in one captured graph with the team
+    engine, replayed three times with the outputs cleared in between: every
+    replay equals the one-workgroup resident engine (and, in FP32, the
+    batched engine)."""
+    import torch
+    from atsc3_amd.mixed import MixedPipeline, MixedSource, block_esn0, frame_mix
+    specs = frame_mix(74, 3)
+    outs = {}
+    engines = ["resident", "team"] + (["batch"] if msg == "float32" else [])
+    for eng in engines:
+        print(f"[gpu] mixed pipeline {msg} {eng}", flush=True)
+        p = MixedPipeline(specs, 2, device, 50, 0.75, msg, engine=eng, team=8)
+        src = MixedSource(p, 8, 21)
+        cells, s2, _ = src.frame(block_esn0(p.specs, 23))
+        p.load(cells, s2)
+        g = p.capture()
+        for r in range(3 if eng == "team" else 1):
+            for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad):
+                x.fill_(0x5A if x.dtype == torch.uint8 else -7)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            p.plan.check()
+            cur = [x.clone() for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad)]
+            if eng == "team" and r > 0:
+                for a, b in zip(cur, outs["team"]):
+                    assert torch.equal(a, b), f"replay {r}"
+            outs[eng] = cur
+        del g, p
+    for eng in engines[1:]:
+        for a, b in zip(outs[eng], outs["resident"]):
+            assert torch.equal(a, b), eng
