@@ -102,6 +102,8 @@ _SIGS = {
                                       c_vp, ctypes.c_int64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_minsum_resident_team_ws_bytes": [c_i32, c_i32, c_i32, c_i32, c_i32],
     "atsc3_minsum_resident_team_status_word": [],
+    "atsc3_minsum_resident_team_order_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                             c_vp],
     "atsc3_minsum_resident_team_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                        c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp,
                                        c_i32, c_i32, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp,

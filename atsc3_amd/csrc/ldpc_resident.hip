@@ -1094,6 +1094,80 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
     }
 }
 
+// Queue order for team mode: the codewords that will need the most
+// iterations first, so that no slow codeword starts late.  The estimate is
+// the fraction of unsatisfied checks of the channel's hard decisions over a
+// sample of every check group's rows (positions 0..63; noise is independent
+// of the position): per code it ranks iteration counts closely, and the
+// codewords that exhaust the iteration limit sit near one half.  One
+// workgroup per codeword.  FP32 tables name a variable group's posterior
+// row, not the group: the inverse of grp_v's row map is built in LDS.
+#define RES_KEY_ROWS 64
+__global__ __launch_bounds__(256) void team_difficulty(const float *llr,
+                                                        const QC_CONST long long *blk_off,
+                                                        const QC_CONST int *blk_code,
+                                                        const QC_CONST ResTab *codes, int f32,
+                                                        float *key) {
+    __shared__ unsigned char inv[256];
+    __shared__ int s_cnt;
+    const int b = blockIdx.x;
+    const QC_CONST ResTab *ct = &codes[blk_code[b]];
+    const float *lam = llr + blk_off[b];
+    const int nvg = ct->nvg, ncg = ct->ncg;
+    if (threadIdx.x == 0) s_cnt = 0;
+    if (f32)
+        for (int t = threadIdx.x; t < nvg; t += blockDim.x) {
+            const unsigned vw = (unsigned)ct->grp_v[t];
+            inv[((vw >> 31) << 7) | ((vw >> 23) & 127u)] = (unsigned char)t;
+        }
+    __syncthreads();
+    unsigned cnt = 0;
+    for (int c = threadIdx.x; c < ncg * RES_KEY_ROWS; c += blockDim.x) {
+        const int g = c / RES_KEY_ROWS;
+        const unsigned j = (unsigned)(c - g * RES_KEY_ROWS);
+        const unsigned gw = (unsigned)ct->grp_c[g];
+        const int dc = gw & 127, b0 = (gw >> 7) & 2047;
+        const bool hasmiss = (gw >> 18) & 1u;
+        unsigned par = 0;
+        for (int k0 = 0; k0 < dc; k0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                v[q] = 0.0f;                       // past the degree or a missing entry: +0
+                if (k0 + q < dc) {
+                    const unsigned ew = (unsigned)ct->edge_c[b0 + k0 + q];
+                    if (!(hasmiss && ((ew >> 18) & 1023u) == j + 1u)) {
+                        const unsigned row = (ew >> 9) & 511u;
+                        const unsigned t = f32 ? inv[(((ew >> 28) & 1u) << 7) | (row & 127u)] : row;
+                        v[q] = lam[t * ATSC3_Z + wrap360(j + (ew & 511u))];
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) par ^= v[q] < 0.0f ? 1u : 0u;
+        }
+        cnt += par;
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63u) == 0u && cnt) atomicAdd(&s_cnt, (int)cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) key[b] = (float)s_cnt / (float)(ncg * RES_KEY_ROWS);
+}
+
+// order[r] = the codeword of rank r by decreasing key (ties: lower index
+// first): each codeword counts the ones ranked before it.
+__global__ void team_rank(const float *key, int nblk, int *order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblk) return;
+    const float ki = key[i];
+    int r = 0;
+    for (int j = 0; j < nblk; ++j) {
+        const float kj = key[j];
+        r += (kj > ki || (kj == ki && j < i)) ? 1 : 0;
+    }
+    order[r] = i;
+}
+
 struct TeamLayout {
     int64_t nslots, stride, pstride, hdr, p01, p0b, p2, tpost, bytes;
     TeamLayout(int64_t grid, int64_t team, int64_t max_checks, int64_t max_grows, bool f32) {
@@ -1260,6 +1334,28 @@ extern "C" int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t tea
 // Index of the int32 abort word in the team workspace: nonzero after a
 // launch whose bounded waits ran out (its = -1 marks the codewords it left).
 extern "C" int32_t atsc3_minsum_resident_team_status_word(void) { return TH_ABORT; }
+
+// The team queue order (see team_difficulty): order[0..nblk) = the codewords
+// by decreasing sampled fraction of unsatisfied checks; key: nblk floats of
+// device scratch.
+extern "C" int32_t atsc3_minsum_resident_team_order_dev(const float *llr, int32_t nblk,
+                                                        const int64_t *blk_off,
+                                                        const int32_t *blk_code,
+                                                        const void *codes, int32_t msg_fmt,
+                                                        float *key, int32_t *order,
+                                                        void *stream) {
+    if (nblk < 0 || msg_fmt < 0 || msg_fmt > 2) return ATSC3_E_ARG;
+    if (nblk == 0) return ATSC3_OK;
+    if (!llr || !blk_off || !blk_code || !codes || !key || !order) return ATSC3_E_ARG;
+    hipStream_t st = atsc3::as_stream(stream);
+    hipLaunchKernelGGL(atsc3::team_difficulty, dim3(nblk), dim3(256), 0, st, llr,
+                       (const QC_CONST long long *)blk_off, (const QC_CONST int *)blk_code,
+                       (const QC_CONST atsc3::ResTab *)codes, msg_fmt == 0 ? 1 : 0, key);
+    if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
+    hipLaunchKernelGGL(atsc3::team_rank, dim3((nblk + 255) / 256), dim3(256), 0, st, key, nblk,
+                       order);
+    return atsc3::launch_status();
+}
 
 extern "C" int32_t atsc3_minsum_resident_team_dev(
     const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,

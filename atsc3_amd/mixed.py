@@ -72,7 +72,8 @@ class MixedPipeline:
         "resident": one workgroup per codeword (grid = workgroups, slots
         unused); "team": the resident engine with each codeword on a team
         of ``team`` workgroups of one XCD (one workgroup per compute
-        unit)."""
+        unit), the queue ranked on the device before each decode (the
+        codewords likely to need the most iterations first)."""
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -91,7 +92,7 @@ class MixedPipeline:
         elif engine == "team":
             from .resident import ResidentPlan
             self.plan = ResidentPlan(codes, self.device, iters, alpha, None, msg_dtype,
-                                     team=8 if team is None else team)
+                                     team=8 if team is None else team, order="auto")
         elif engine in ("batch", "xcd"):
             self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid,
                                   xcd=engine == "xcd")

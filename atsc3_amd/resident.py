@@ -266,7 +266,11 @@ class ResidentPlan:
         team: None (one workgroup per codeword), or 1..32: team mode, each
         codeword decoded by that many workgroups of one XCD (same results;
         grid at most one workgroup per compute unit).  order: team mode's
-        queue order (codeword indices), or None (0, 1, ...)."""
+        queue order -- None (0, 1, ...), codeword indices, or "auto": ranked
+        on the device before every decode, the codewords whose channel
+        decisions leave the most checks unsatisfied first (they tend to need
+        the most iterations, and a slow codeword that starts late sets the
+        launch's time)."""
         fmts = {"float32": 0, "float16": 1, "int8": 2}
         if msg_dtype not in fmts:
             raise ValueError("the resident engine stores float32, float16 or int8 records")
@@ -319,9 +323,13 @@ class ResidentPlan:
         if nbytes <= 0:
             raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
         self.order = None
-        if order is not None:
-            if self.team is None:
-                raise ValueError("order applies to team mode only")
+        self.order_auto = isinstance(order, str) and order == "auto"
+        if order is not None and self.team is None:
+            raise ValueError("order applies to team mode only")
+        if self.order_auto:
+            self.order = torch.empty(self.nblk, dtype=torch.int32, device=dev)
+            self.order_key = torch.empty(self.nblk, dtype=torch.float32, device=dev)
+        elif order is not None:
             o = np.asarray(order, np.int64)
             if o.shape != (self.nblk,) or not np.array_equal(np.sort(o), np.arange(self.nblk)):
                 raise ValueError("order must be a permutation of the codeword indices")
@@ -362,6 +370,11 @@ class ResidentPlan:
         if self.team is not None:
             if post is not None:
                 raise ValueError("team mode writes no posteriors")
+            if self.order_auto:
+                _lib.call("atsc3_minsum_resident_team_order_dev", src.data_ptr(), self.nblk,
+                          self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
+                          self.fmt, self.order_key.data_ptr(), self.order.data_ptr(),
+                          _stream_ptr(self.device))
             _lib.call("atsc3_minsum_resident_team_dev", *args[:15],
                       None if self.order is None else self.order.data_ptr(), self.grid,
                       self.team, *args[16:], _stream_ptr(self.device))

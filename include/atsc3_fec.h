@@ -397,6 +397,19 @@ int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
  * runs sharing one ws must not overlap.  No upstream counterpart as an
  * entry point (the upstream's decode calls map onto it as above).
  * ------------------------------------------------------------------------ */
+/* A queue order for team mode: order[0..nblk) = the codewords by
+ * decreasing fraction of unsatisfied checks of the channel's hard decisions
+ * (sign of llr), sampled on positions 0..63 of every check group (ties: lower
+ * index first), so that the codewords likely to need the most iterations
+ * start first.  codes: the atsc3_res_code array of msg_fmt's tables; key:
+ * nblk floats of device scratch (receives the fractions).  Two launches on
+ * `stream`; the order changes only scheduling, never results. */
+int32_t atsc3_minsum_resident_team_order_dev(const float *llr, int32_t nblk,
+                                             const int64_t *blk_off,
+                                             const int32_t *blk_code,
+                                             const void *codes, int32_t msg_fmt,
+                                             float *key, int32_t *order,
+                                             void *stream);
 int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t team,
                                             int32_t max_checks,
                                             int32_t max_global_rows,

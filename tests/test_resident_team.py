@@ -214,3 +214,46 @@ def test_team_mixed_pipeline_graph_replay(device, msg):
     for eng in engines[1:]:
         for a, b in zip(outs[eng], outs["resident"]):
             assert torch.equal(a, b), eng
+
+
+def _sampled_unsat_fraction(code, lam):
+    """The team queue's difficulty key restated: the fraction of unsatisfied
+    checks of the hard decisions over positions 0..63 of every check group."""
+    q = code.qc()
+    cg, be = q["cg_ptr"].astype(np.int64), q["be_word"].astype(np.int64)
+    j = np.arange(64)
+    par = np.zeros((len(cg) - 1, 64), np.int64)
+    for g in range(len(cg) - 1):
+        for b in range(cg[g], cg[g + 1]):
+            s, t, miss = be[b] & 511, (be[b] >> 9) & 511, (be[b] >> 18) & 1023
+            hard = (lam[t * 360 + (j + s) % 360] < 0).astype(np.int64)
+            par[g] ^= np.where(j + 1 == miss, 0, hard)
+    return np.float32(par.sum()) / np.float32(par.size)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float16", "float32"])
+def test_team_auto_order(device, msg):
+    """order="auto": the device ranks the queue by the sampled unsatisfied-
+    check fraction of the channel decisions (equal to its numpy restatement,
+    in decreasing order, ties by index); results equal the unranked decode
+    (and, FP16, the oracle)."""
+    from atsc3_amd.resident import ResidentPlan
+    blocks = _mixed_blocks(seed=8)
+    codes = [b[0] for b in blocks]
+    plan = ResidentPlan(codes, device, 50, 0.75, msg_dtype=msg, team=8, order="auto")
+    print(f"[gpu] team auto order {msg} {len(blocks)} blocks", flush=True)
+    got = _run_plan(plan, blocks, device)
+    key = plan.order_key.cpu().numpy()
+    order = plan.order.cpu().numpy()
+    want_key = np.array([_sampled_unsat_fraction(b[0], b[1]) for b in blocks], np.float32)
+    assert np.array_equal(key, want_key)
+    assert sorted(order.tolist()) == list(range(len(blocks)))
+    ranked = sorted(range(len(blocks)), key=lambda i: (-want_key[i], i))
+    assert order.tolist() == ranked
+    plain = ResidentPlan(codes, device, 50, 0.75, msg_dtype=msg, team=8)
+    ref = _run_plan(plain, blocks, device)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    if msg == "float16":
+        _check_blocks(plan, blocks, got, "auto order")
