@@ -22,8 +22,9 @@ on each XCD): it wins when the rounds of a few slow codewords dominate, since
 its barriers span one XCD and need no L2 write-back.  With ``team=True`` the
 resident engine's team mode is a candidate for each team size in
 ``TEAM_SIZES`` (slots = "team", grid = the team size: workgroups of one XCD
-per codeword): it wins when a few slow codewords would otherwise run their
-iterations on one compute unit each.
+per codeword; slots = "team-l2": the same without the members' LDS copies
+of the posteriors): it wins when a few slow codewords would otherwise run
+their iterations on one compute unit each.
 """
 from __future__ import annotations
 
@@ -74,7 +75,7 @@ def candidates(nblk, msg_dtype, xcd=False, team=False):
         capx = max_grid(msg_dtype, xcd=True)
         out += [("xcd", g) for g in sorted({capx, capx // 2} - {0}, reverse=True)]
     if team:
-        out += [("team", t) for t in TEAM_SIZES]
+        out += [(kind, t) for kind in ("team", "team-l2") for t in TEAM_SIZES]
     return out
 
 
@@ -97,7 +98,8 @@ def tune(signature, nblk, msg_dtype, build, prepare=None, reps=5, use_cache=True
     loads the tuning input.  Returns (slots, grid, table of ms per candidate);
     slots == "resident" (grid None: one workgroup per compute unit) when the
     resident engine won, slots == "xcd" when the XCD-partitioned mode did,
-    slots == "team" (grid = team size) when the resident team mode did."""
+    slots == "team" / "team-l2" (grid = team size) when the resident team
+    mode did."""
     dev = torch.cuda.get_device_name()
     key = (f"{dev}|{signature}|{msg_dtype}|{'r' if resident else 'b'}{'x' if xcd else ''}"
            f"{'t' if team else ''}")

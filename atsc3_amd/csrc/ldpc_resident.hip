@@ -805,8 +805,8 @@ struct WsLayout {
 // The resident kernel puts a codeword on one compute unit, so a frame whose
 // slowest codeword needs 50 iterations waits for 50 iterations of one CU.
 // Here `team` workgroups (one per CU, all on one XCD) share each codeword:
-// member m takes the check steps and variable steps m * 16 + wave, stride
-// 16 * team (the same steps the resident kernel gives its waves), the team's
+// wave w of member m takes the check steps and variable steps w * team + m,
+// stride 16 * team (together the steps the resident kernel's waves take), the team's
 // records live in its slot in global memory, and every member keeps a full
 // copy of the posteriors in LDS.  Per iteration:
 //   CHECK (own share; old records, LDS posteriors) -> new records, the
@@ -952,9 +952,10 @@ __global__ void resident_team_init(unsigned *hdr, int words, uint8_t *conv, int 
     }
 }
 
-template <int M>
+template <int M, bool DIRECT>
 __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamArgs t) {
-    __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
+    __shared__ __align__(16) unsigned char
+        lds[DIRECT ? 16 : (M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES)];
     __shared__ int s_unsat[2];
     __shared__ int s_hist[M == 2 ? 256 : 1];
     __shared__ unsigned s_cnt[RES_XCDS];
@@ -1000,11 +1001,16 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
     R.p0b = M == 0 ? a.p0b + (size_t)slot * a.stride : nullptr;
     R.p2 = a.p2 + (size_t)slot * a.stride;
     Tot T;
-    T.lds = lds;
     T.pub = t.tpost + (size_t)slot * t.pstride;
+    // DIRECT: no LDS copy; the team's rows in global memory (FP16 padded as
+    // the LDS rows are) are read and written where the LDS rows would be
+    T.lds = DIRECT ? T.pub : lds;
     // FP32: the LDS rows' published copies first, then the global rows
     T.glob = M == 0 ? reinterpret_cast<float *>(T.pub) + RES_F32_LDS_ROWS * ATSC3_Z : nullptr;
-    const int w0 = (int)m * RES_WAVES + wave, wst = (int)S * RES_WAVES;
+    // steps and tasks dealt member-major (wave w of member m takes w * S + m,
+    // then every S * 16-th): a phase with fewer steps than the team has
+    // waves then spreads them over every member's SIMDs
+    const int w0 = wave * (int)S + (int)m, wst = (int)S * RES_WAVES;
     unsigned epoch = 0;
     for (unsigned cw = 0;; ++cw) {
         if (leader)
@@ -1042,13 +1048,13 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
                 const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
                 if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
                     const unsigned vw = (unsigned)ct->grp_v[tg];
-                    tot_st<M, true>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
+                    tot_st<M, !DIRECT>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
                 }
             }
         }
         const int nlds = M == 0 ? min(nvg, RES_F32_LDS_ROWS) : nvg;
         if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
-        team_refresh<M>(lds, T.pub, nlds);
+        if (!DIRECT) team_refresh<M>(lds, T.pub, nlds);
         if (tid == 0) s_unsat[0] = 0;
         __syncthreads();
         for (int it = 0;; ++it) {
@@ -1086,10 +1092,12 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
                 }
                 break;
             }
-            var_phase<M, true>(ct, T, lam, R, w0, wst, lane);
+            var_phase<M, !DIRECT>(ct, T, lam, R, w0, wst, lane);
             if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
-            team_refresh<M>(lds, T.pub, nlds);
-            __syncthreads();
+            if (!DIRECT) {
+                team_refresh<M>(lds, T.pub, nlds);
+                __syncthreads();
+            }
         }
     }
 }
@@ -1174,8 +1182,9 @@ struct TeamLayout {
         // at most ceil(n_x / team) teams on each XCD x: grid / team + one per XCD
         nslots = grid / team + RES_XCDS;
         stride = (max_checks + 63) & ~(int64_t)63;
+        // FP16: padded rows (the direct flavor reads them as the LDS rows)
         pstride = f32 ? ((RES_F32_LDS_ROWS + max_grows) * ATSC3_Z * 4 + 255) & ~(int64_t)255
-                      : ((RES_NMAX / ATSC3_Z) * ATSC3_Z * 2 + 255) & ~(int64_t)255;
+                      : ((int64_t)RES_F16_LDS_BYTES + 255) & ~(int64_t)255;
         hdr = 0;
         const int64_t words = TH_WORDS + nslots * TC_WORDS;
         p01 = (words * 4 + 255) & ~(int64_t)255;
@@ -1325,6 +1334,7 @@ extern "C" int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t tea
                                                        int32_t max_checks,
                                                        int32_t max_global_rows,
                                                        int32_t msg_fmt) {
+    team &= ~ATSC3_TEAM_DIRECT;
     if (grid < 1 || team < 1 || team > RES_TEAM_MAX || max_checks < 1 || max_global_rows < 0 ||
         msg_fmt < 0 || msg_fmt > 2)
         return ATSC3_E_ARG;
@@ -1369,6 +1379,8 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
                                                   max_dv, iters, alpha, msg_fmt, grid, bits,
                                                   conv, its, bad);
     if (rc != ATSC3_OK) return rc;
+    const bool direct = (team & ATSC3_TEAM_DIRECT) != 0;
+    team &= ~ATSC3_TEAM_DIRECT;
     if (team < 1 || team > RES_TEAM_MAX) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!ws) return ATSC3_E_ARG;
@@ -1410,12 +1422,19 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
     hipLaunchKernelGGL(atsc3::resident_team_init, dim3((n0 + 255) / 256), dim3(256), 0, st,
                        t.hdr, words, conv, its, bad, nblk);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
-    if (f32)
-        hipLaunchKernelGGL(atsc3::resident_team_kernel<0>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+    const dim3 g(grid), b(RES_WG);
+    if (f32 && direct)
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<0, true>), g, b, 0, st, a, t);
+    else if (f32)
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<0, false>), g, b, 0, st, a, t);
+    else if (msg_fmt == 1 && direct)
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<1, true>), g, b, 0, st, a, t);
     else if (msg_fmt == 1)
-        hipLaunchKernelGGL(atsc3::resident_team_kernel<1>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<1, false>), g, b, 0, st, a, t);
+    else if (direct)
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<2, true>), g, b, 0, st, a, t);
     else
-        hipLaunchKernelGGL(atsc3::resident_team_kernel<2>, dim3(grid), dim3(RES_WG), 0, st, a, t);
+        hipLaunchKernelGGL((atsc3::resident_team_kernel<2, false>), g, b, 0, st, a, t);
     return atsc3::launch_status();
 }
 

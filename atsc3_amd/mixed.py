@@ -66,14 +66,16 @@ class MixedPipeline:
     """Demap + batched decode of frames with a fixed block layout."""
 
     def __init__(self, specs, frames=1, device="cuda", iters=50, alpha=0.75,
-                 msg_dtype="float32", slots=None, grid=None, engine="batch", team=None):
+                 msg_dtype="float32", slots=None, grid=None, engine="batch", team=None,
+                 direct=False):
         """engine "batch": the persistent batched engine (slots, grid);
         "xcd": the same engine partitioned per XCD (slots over all XCDs);
         "resident": one workgroup per codeword (grid = workgroups, slots
         unused); "team": the resident engine with each codeword on a team
         of ``team`` workgroups of one XCD (one workgroup per compute
         unit), the queue ranked on the device before each decode (the
-        codewords likely to need the most iterations first)."""
+        codewords likely to need the most iterations first); ``direct``:
+        the team mode without LDS copies of the posteriors."""
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -92,7 +94,8 @@ class MixedPipeline:
         elif engine == "team":
             from .resident import ResidentPlan
             self.plan = ResidentPlan(codes, self.device, iters, alpha, None, msg_dtype,
-                                     team=8 if team is None else team, order="auto")
+                                     team=8 if team is None else team, order="auto",
+                                     direct=direct)
         elif engine in ("batch", "xcd"):
             self.plan = BatchPlan(codes, self.device, iters, alpha, msg_dtype, slots, grid,
                                   xcd=engine == "xcd")

@@ -83,6 +83,7 @@ def _pack_tasks(deg, generic, pack):
 
 F32_LDS_ROWS = 113       # FP32 posterior rows (360 floats) a workgroup holds in LDS
 TEAM_MAX = 32            # workgroups per codeword in team mode
+TEAM_DIRECT = 0x100      # ATSC3_TEAM_DIRECT: team mode without LDS copies
 SIGN_LO = 24             # edge signs a record's word 1 holds (the rest: word 2)
 
 
@@ -257,7 +258,7 @@ class ResidentPlan:
     its bits the same range of ``bits``."""
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
-                 msg_dtype="float16", msg_scale=None, team=None, order=None):
+                 msg_dtype="float16", msg_scale=None, team=None, order=None, direct=False):
         """msg_dtype "float16": the FP16 form; "int8": INT8 record magnitudes
         with a per-codeword step (msg_scale, one per codeword, or None: 6 *
         median|llr| / 127 on the device); "float32": the plain decoder,
@@ -270,7 +271,9 @@ class ResidentPlan:
         on the device before every decode, the codewords whose channel
         decisions leave the most checks unsatisfied first (they tend to need
         the most iterations, and a slow codeword that starts late sets the
-        launch's time)."""
+        launch's time).  direct: team mode without the members' LDS copies
+        of the posteriors (every member reads the team's rows in L2; same
+        results)."""
         fmts = {"float32": 0, "float16": 1, "int8": 2}
         if msg_dtype not in fmts:
             raise ValueError("the resident engine stores float32, float16 or int8 records")
@@ -306,6 +309,9 @@ class ResidentPlan:
                 raise ValueError(f"team must be 1..{TEAM_MAX}")
             # every workgroup must be co-resident (one per compute unit)
             self.grid = min(g, default_grid())
+        self.direct = bool(direct)
+        if self.direct and self.team is None:
+            raise ValueError("direct applies to team mode only")
         dev = self.device
         self.blk_code = torch.from_numpy(self.code_idx).to(dev)
         self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
@@ -377,7 +383,8 @@ class ResidentPlan:
                           _stream_ptr(self.device))
             _lib.call("atsc3_minsum_resident_team_dev", *args[:15],
                       None if self.order is None else self.order.data_ptr(), self.grid,
-                      self.team, *args[16:], _stream_ptr(self.device))
+                      self.team | (TEAM_DIRECT if self.direct else 0), *args[16:],
+                      _stream_ptr(self.device))
         elif post is None:
             _lib.call("atsc3_minsum_resident_dev", *args, _stream_ptr(self.device))
         else:

@@ -335,8 +335,9 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
 
     def build(fmt):
         def make(slots, grid):
-            if slots == "team":
-                return MixedPipeline(specs, F, dev, 50, 0.75, fmt, engine="team", team=grid)
+            if slots in ("team", "team-l2"):
+                return MixedPipeline(specs, F, dev, 50, 0.75, fmt, engine="team", team=grid,
+                                     direct=slots == "team-l2")
             return MixedPipeline(
                 specs, F, dev, 50, 0.75, fmt, None if slots in ("resident", "xcd") else slots,
                 grid, engine=slots if slots in ("resident", "xcd") else "batch")

@@ -404,6 +404,9 @@ int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
  * start first.  codes: the atsc3_res_code array of msg_fmt's tables; key:
  * nblk floats of device scratch (receives the fractions).  Two launches on
  * `stream`; the order changes only scheduling, never results. */
+#define ATSC3_TEAM_DIRECT 0x100   /* OR-ed into `team`: no LDS copy of the
+                                     posteriors; every member reads the team's
+                                     rows in global memory (L2) instead       */
 int32_t atsc3_minsum_resident_team_order_dev(const float *llr, int32_t nblk,
                                              const int64_t *blk_off,
                                              const int32_t *blk_code,

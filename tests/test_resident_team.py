@@ -10,7 +10,8 @@ reference kernel (oracle/minsum_oracle.c):
 * mixed codes in one queue (fixture blocks and synthetic blocks of the codes
   that take the generic paths) for team sizes 1, 3 (an uneven split of every
   XCD's workgroups: a smaller last team), 8 and 32 (a whole XCD per
-  codeword), FP16;
+  codeword), FP16, with the members' LDS copies of the posteriors and
+  without them (direct: the team's rows read in L2);
 * every golden fixture in FP32 with team size 8, one launch per (iters,
   alpha) setting, and the C oracle on the synthetic codes;
 * INT8 with the oracle's steps given, and the device-derived steps equal to
@@ -68,19 +69,22 @@ def _check_blocks(plan, blocks, got, tag):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("team", [1, 3, 8, 32])
-def test_team_fp16_mixed_codes_equal_oracle(device, team):
+@pytest.mark.parametrize("team,direct", [(1, False), (3, False), (8, False), (32, False),
+                                         (1, True), (5, True), (16, True), (32, True)])
+def test_team_fp16_mixed_codes_equal_oracle(device, team, direct):
+    """direct: no LDS copies, every member reading the team's rows in L2."""
     from atsc3_amd.resident import ResidentPlan
     blocks = _mixed_blocks()
     plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="float16",
-                        team=team)
-    print(f"[gpu] team fp16 mixed {len(blocks)} blocks, team {team}, grid {plan.grid}",
-          flush=True)
-    _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"team {team}")
+                        team=team, direct=direct)
+    print(f"[gpu] team fp16 mixed {len(blocks)} blocks, team {team}, direct {direct}, "
+          f"grid {plan.grid}", flush=True)
+    _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"team {team} {direct}")
 
 
 @pytest.mark.gpu
-def test_team_fp32_equals_reference_fixtures(device):
+@pytest.mark.parametrize("direct", [False, True])
+def test_team_fp32_equals_reference_fixtures(device, direct):
     """Every golden fixture, decoded in FP32 team mode (team 8), equals the
     reference kernel's outputs; one mixed launch per (iters, alpha)."""
     from atsc3_amd.resident import ResidentPlan
@@ -93,9 +97,9 @@ def test_team_fp32_equals_reference_fixtures(device):
                  f"{name}#{b}"))
     for (iters, alpha), blocks in sorted(groups.items()):
         plan = ResidentPlan([b[0] for b in blocks], device, iters, alpha, msg_dtype="float32",
-                            team=8)
-        print(f"[gpu] team fp32 fixtures iters {iters} alpha {alpha}: {len(blocks)} blocks",
-              flush=True)
+                            team=8, direct=direct)
+        print(f"[gpu] team fp32 fixtures iters {iters} alpha {alpha}: {len(blocks)} blocks, "
+              f"direct {direct}", flush=True)
         _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"fp32 {iters} {alpha}")
 
 
@@ -111,9 +115,12 @@ def test_team_fp32_equals_c_oracle(device):
         want = minsum_f32(llr, code, iters, alpha)
         for b in range(len(llr)):
             blocks.append((code, llr[b], [np.asarray(w)[b] for w in want], f"{r} {n}#{b}"))
-    plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="float32", team=5)
-    print(f"[gpu] team fp32 synthetic codes {len(blocks)} blocks", flush=True)
-    _check_blocks(plan, blocks, _run_plan(plan, blocks, device), "fp32 synth")
+    for direct in (False, True):
+        plan = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="float32",
+                            team=5, direct=direct)
+        print(f"[gpu] team fp32 synthetic codes {len(blocks)} blocks, direct {direct}",
+              flush=True)
+        _check_blocks(plan, blocks, _run_plan(plan, blocks, device), f"fp32 synth {direct}")
 
 
 @pytest.mark.gpu
@@ -142,6 +149,11 @@ def test_team_int8_equals_oracle(device):
     got2 = _run_plan(auto, blocks, device)
     for a, b in zip(got, got2):
         assert np.array_equal(a, b)
+    l2 = ResidentPlan([b[0] for b in blocks], device, 50, 0.75, msg_dtype="int8", team=16,
+                      direct=True)
+    got3 = _run_plan(l2, blocks, device)
+    for a, b in zip(got, got3):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
@@ -160,6 +172,11 @@ def test_team_edge_cases(device, msg):
     torch.cuda.synchronize()
     for a, b in zip(one, ref):
         assert np.array_equal(a.cpu().numpy(), b[:1])
+    one = ResidentPlan([code], device, iters, alpha, msg_dtype=msg, team=32, direct=True)
+    got = [x.cpu().numpy() for x in one.run(t[:1].reshape(-1))]
+    one.check()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a.reshape(b[:1].shape), b[:1])
     z_ref = [x.cpu().numpy() for x in decode_resident_dev(t, code, 0, alpha, msg_dtype=msg)]
     z = decode_resident_dev(t, code, 0, alpha, msg_dtype=msg, team=4)
     torch.cuda.synchronize()
@@ -190,15 +207,17 @@ def test_team_mixed_pipeline_graph_replay(device, msg):
     from atsc3_amd.mixed import MixedPipeline, MixedSource, block_esn0, frame_mix
     specs = frame_mix(74, 3)
     outs = {}
-    engines = ["resident", "team"] + (["batch"] if msg == "float32" else [])
+    engines = ["resident", "team", "team-l2"] + (["batch"] if msg == "float32" else [])
     for eng in engines:
         print(f"[gpu] mixed pipeline {msg} {eng}", flush=True)
-        p = MixedPipeline(specs, 2, device, 50, 0.75, msg, engine=eng, team=8)
+        p = MixedPipeline(specs, 2, device, 50, 0.75, msg,
+                          engine="team" if eng.startswith("team") else eng,
+                          team=8 if eng == "team" else 16, direct=eng == "team-l2")
         src = MixedSource(p, 8, 21)
         cells, s2, _ = src.frame(block_esn0(p.specs, 23))
         p.load(cells, s2)
         g = p.capture()
-        for r in range(3 if eng == "team" else 1):
+        for r in range(3 if eng.startswith("team") else 1):
             for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad):
                 x.fill_(0x5A if x.dtype == torch.uint8 else -7)
             torch.cuda.synchronize()
@@ -206,9 +225,9 @@ def test_team_mixed_pipeline_graph_replay(device, msg):
             torch.cuda.synchronize()
             p.plan.check()
             cur = [x.clone() for x in (p.plan.bits, p.plan.conv, p.plan.its, p.plan.bad)]
-            if eng == "team" and r > 0:
-                for a, b in zip(cur, outs["team"]):
-                    assert torch.equal(a, b), f"replay {r}"
+            if eng.startswith("team") and r > 0:
+                for a, b in zip(cur, outs[eng]):
+                    assert torch.equal(a, b), f"{eng} replay {r}"
             outs[eng] = cur
         del g, p
     for eng in engines[1:]:
@@ -241,7 +260,8 @@ def test_team_auto_order(device, msg):
     from atsc3_amd.resident import ResidentPlan
     blocks = _mixed_blocks(seed=8)
     codes = [b[0] for b in blocks]
-    plan = ResidentPlan(codes, device, 50, 0.75, msg_dtype=msg, team=8, order="auto")
+    plan = ResidentPlan(codes, device, 50, 0.75, msg_dtype=msg, team=8, order="auto",
+                        direct=msg == "float16")
     print(f"[gpu] team auto order {msg} {len(blocks)} blocks", flush=True)
     got = _run_plan(plan, blocks, device)
     key = plan.order_key.cpu().numpy()
