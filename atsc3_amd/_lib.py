@@ -113,6 +113,7 @@ _SIGS = {
                                           c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp,
                                           c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "atsc3_demap_ws_doubles": [c_i32, c_i32],
+    "atsc3_demap_groups_dev": [c_vp, c_i32, c_i32, c_i32, c_vp],
     "atsc3_demap_raw_dev": [c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                             c_vp],
     "atsc3_minsum_utc_ws_bytes": [c_i32, c_i32, c_i32, c_i32],

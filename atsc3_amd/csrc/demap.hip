@@ -66,6 +66,44 @@ __device__ __forceinline__ size_t cell_of(const Gather &g, int blk, int ncell, i
     }
 }
 
+// Max-log differences of one cell z against the NW = 2^(NB-2) base points w
+// (in LDS): diff[i] = min over y_i = 1 minus min over y_i = 0 (before the
+// CSI weight and sigma2), dmin = the distance to the nearest point.
+template <int NB>
+__device__ __forceinline__ void demap_cell(float2 z, const float2 *w, float (&diff)[NB],
+                                           float &dmin_out) {
+    constexpr int NW = 1 << (NB - 2);
+    const float ar = fabsf(z.x), ai = fabsf(z.y);
+    float dmin = INFINITY, opp_im = INFINITY, opp_re = INFINITY;
+    float m0[NB > 2 ? NB - 2 : 1], m1[NB > 2 ? NB - 2 : 1];
+#pragma unroll
+    for (int i = 0; i < NB - 2; ++i) m0[i] = m1[i] = INFINITY;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const float2 p = w[k];
+        const float dr = ar - p.x, sr = ar + p.x;
+        const float di = ai - p.y, si = ai + p.y;
+        const float A = dr * dr, B = sr * sr, C = di * di, D = si * si;
+        const float ac = A + C, ad = A + D, bc = B + C;
+        dmin = fminf(dmin, ac);
+        opp_im = fminf(opp_im, ad);
+        opp_re = fminf(opp_re, bc);
+#pragma unroll
+        for (int i = 2; i < NB; ++i) {
+            if ((k >> (NB - 1 - i)) & 1)
+                m1[i - 2] = fminf(m1[i - 2], ac);
+            else
+                m0[i - 2] = fminf(m0[i - 2], ac);
+        }
+    }
+    // y0: imaginary sign; y1: real sign (see header)
+    diff[0] = z.y >= 0.0f ? opp_im - dmin : dmin - opp_im;
+    diff[1] = z.x >= 0.0f ? opp_re - dmin : dmin - opp_re;
+#pragma unroll
+    for (int i = 2; i < NB; ++i) diff[i] = m1[i - 2] - m0[i - 2];
+    dmin_out = dmin;
+}
+
 template <int NB, int GM>
 __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
     const float2 *__restrict__ cells, const float *__restrict__ csi,
@@ -85,36 +123,9 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
     if (s < ncell) {
         const size_t ci = cell_of<GM>(g, blk, ncell, s);
         const float2 z = cells[ci];
-        const float ar = fabsf(z.x), ai = fabsf(z.y);
-        float dmin = INFINITY, opp_im = INFINITY, opp_re = INFINITY;
-        float m0[NB > 2 ? NB - 2 : 1], m1[NB > 2 ? NB - 2 : 1];
-#pragma unroll
-        for (int i = 0; i < NB - 2; ++i) m0[i] = m1[i] = INFINITY;
-#pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            const float2 p = w[k];
-            const float dr = ar - p.x, sr = ar + p.x;
-            const float di = ai - p.y, si = ai + p.y;
-            const float A = dr * dr, B = sr * sr, C = di * di, D = si * si;
-            const float ac = A + C, ad = A + D, bc = B + C;
-            dmin = fminf(dmin, ac);
-            opp_im = fminf(opp_im, ad);
-            opp_re = fminf(opp_re, bc);
-#pragma unroll
-            for (int i = 2; i < NB; ++i) {
-                if ((k >> (NB - 1 - i)) & 1)
-                    m1[i - 2] = fminf(m1[i - 2], ac);
-                else
-                    m0[i - 2] = fminf(m0[i - 2], ac);
-            }
-        }
+        float diff[NB], dmin;
+        demap_cell<NB>(z, w, diff, dmin);
         const float wgt = csi ? csi[ci] : 1.0f;
-        float diff[NB];
-        // y0: imaginary sign; y1: real sign (see header)
-        diff[0] = z.y >= 0.0f ? opp_im - dmin : dmin - opp_im;
-        diff[1] = z.x >= 0.0f ? opp_re - dmin : dmin - opp_re;
-#pragma unroll
-        for (int i = 2; i < NB; ++i) diff[i] = m1[i - 2] - m0[i - 2];
         const int *lq = lam_of_q + (size_t)s * NB;
         float *out = llr + (size_t)blk * N;
         if (sigma2) {
@@ -159,6 +170,57 @@ __global__ __launch_bounds__(DEMAP_WG) void demap_scale_kernel(
     const float d = s2f;
     float *out = llr + (size_t)blk * N;
     for (int i = threadIdx.x; i < N; i += DEMAP_WG) out[i] = out[i] / d;
+}
+
+// Every (code, constellation) group of a frame in one launch: workgroup
+// (x, y) takes cells x*256.. of the frame's block y, found in the group
+// table by its first-block index (groups in increasing blk0); sigma2 given
+// per block, no gather.  The arithmetic is demap_kernel's.
+struct DemapGroup {                 // mirrors struct atsc3_demap_group
+    const float2 *cells;
+    const float *csi;
+    const double *sigma2;
+    const float2 *points;
+    const int32_t *lam_of_q;
+    float *llr;
+    int32_t ncell, nbits, nblk, blk0;
+};
+static_assert(sizeof(DemapGroup) == 64, "demap group descriptor layout");
+
+template <int NB>
+__device__ __forceinline__ void demap_group_block(const DemapGroup &G, int blk, float2 *w) {
+    constexpr int NW = 1 << (NB - 2);
+    for (int k = threadIdx.x; k < NW; k += DEMAP_WG) w[k] = G.points[k];
+    __syncthreads();
+    const int s = blockIdx.x * DEMAP_WG + threadIdx.x;
+    if (s >= G.ncell) return;
+    const size_t ci = (size_t)blk * G.ncell + s;
+    float diff[NB], dmin;
+    demap_cell<NB>(G.cells[ci], w, diff, dmin);
+    const float wgt = G.csi ? G.csi[ci] : 1.0f;
+    const int *lq = G.lam_of_q + (size_t)s * NB;
+    float *out = G.llr + (size_t)blk * G.ncell * NB;
+    const float s2 = (float)G.sigma2[blk];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) out[lq[i]] = (diff[i] * wgt) / s2;
+}
+
+__global__ __launch_bounds__(DEMAP_WG) void demap_groups_kernel(
+    const DemapGroup *__restrict__ grp, int ngroups) {
+    __shared__ float2 w[64];
+    const int gb = blockIdx.y;
+    int gi = 0;
+    while (gi + 1 < ngroups && grp[gi + 1].blk0 <= gb) ++gi;
+    const DemapGroup G = grp[gi];
+    const int blk = gb - G.blk0;
+    if (blk >= G.nblk || (int)blockIdx.x * DEMAP_WG >= G.ncell) return;
+    switch (G.nbits) {
+        case 2: demap_group_block<2>(G, blk, w); break;
+        case 4: demap_group_block<4>(G, blk, w); break;
+        case 6: demap_group_block<6>(G, blk, w); break;
+        case 8: demap_group_block<8>(G, blk, w); break;
+        default: break;
+    }
 }
 
 __global__ void copy_sigma2_kernel(const double *src, double *dst, int nblk) {
@@ -246,6 +308,18 @@ static int32_t demap_any(const float *cells, const float *csi, const double *sig
 }
 
 }  // namespace atsc3
+
+extern "C" int32_t atsc3_demap_groups_dev(const void *groups, int32_t ngroups,
+                                          int32_t nblk_total, int32_t max_ncell, void *stream) {
+    if (ngroups < 0 || nblk_total < 0 || max_ncell < 0) return ATSC3_E_ARG;
+    if (ngroups == 0 || nblk_total == 0 || max_ncell == 0) return ATSC3_OK;
+    if (!groups || nblk_total > 65535) return ATSC3_E_ARG;
+    hipLaunchKernelGGL(atsc3::demap_groups_kernel,
+                       dim3((max_ncell + DEMAP_WG - 1) / DEMAP_WG, nblk_total), dim3(DEMAP_WG),
+                       0, atsc3::as_stream(stream),
+                       static_cast<const atsc3::DemapGroup *>(groups), ngroups);
+    return atsc3::launch_status();
+}
 
 extern "C" int32_t atsc3_demap_ws_doubles(int32_t nblk, int32_t ncell) {
     if (nblk < 0 || ncell < 0) return ATSC3_E_ARG;

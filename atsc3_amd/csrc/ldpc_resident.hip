@@ -1111,7 +1111,8 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
 // workgroup per codeword.  FP32 tables name a variable group's posterior
 // row, not the group: the inverse of grp_v's row map is built in LDS.
 #define RES_KEY_ROWS 64
-__global__ __launch_bounds__(256) void team_difficulty(const float *llr,
+#define RES_KEY_WG 1024
+__global__ __launch_bounds__(RES_KEY_WG) void team_difficulty(const float *llr,
                                                         const QC_CONST long long *blk_off,
                                                         const QC_CONST int *blk_code,
                                                         const QC_CONST ResTab *codes, int f32,
@@ -1137,10 +1138,11 @@ __global__ __launch_bounds__(256) void team_difficulty(const float *llr,
         const int dc = gw & 127, b0 = (gw >> 7) & 2047;
         const bool hasmiss = (gw >> 18) & 1u;
         unsigned par = 0;
-        for (int k0 = 0; k0 < dc; k0 += 8) {
-            float v[8];
+        // 16 loads in flight (one round for the common degrees <= 16)
+        for (int k0 = 0; k0 < dc; k0 += 16) {
+            float v[16];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < 16; ++q) {
                 v[q] = 0.0f;                       // past the degree or a missing entry: +0
                 if (k0 + q < dc) {
                     const unsigned ew = (unsigned)ct->edge_c[b0 + k0 + q];
@@ -1152,7 +1154,7 @@ __global__ __launch_bounds__(256) void team_difficulty(const float *llr,
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) par ^= v[q] < 0.0f ? 1u : 0u;
+            for (int q = 0; q < 16; ++q) par ^= v[q] < 0.0f ? 1u : 0u;
         }
         cnt += par;
     }
@@ -1358,7 +1360,7 @@ extern "C" int32_t atsc3_minsum_resident_team_order_dev(const float *llr, int32_
     if (nblk == 0) return ATSC3_OK;
     if (!llr || !blk_off || !blk_code || !codes || !key || !order) return ATSC3_E_ARG;
     hipStream_t st = atsc3::as_stream(stream);
-    hipLaunchKernelGGL(atsc3::team_difficulty, dim3(nblk), dim3(256), 0, st, llr,
+    hipLaunchKernelGGL(atsc3::team_difficulty, dim3(nblk), dim3(RES_KEY_WG), 0, st, llr,
                        (const QC_CONST long long *)blk_off, (const QC_CONST int *)blk_code,
                        (const QC_CONST atsc3::ResTab *)codes, msg_fmt == 0 ? 1 : 0, key);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;

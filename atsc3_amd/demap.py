@@ -53,8 +53,11 @@ def _stream(dev):
     return torch.cuda.current_stream(dev).cuda_stream
 
 
-def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=None):
-    """Core launcher.  cells (B, C) complex64 device tensor."""
+def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=None,
+                want_sigma2=True):
+    """Core launcher.  cells (B, C) complex64 device tensor.  want_sigma2
+    False: no sigma2 tensor is returned (None), which saves the copy of a
+    given sigma2 (one small launch) in per-frame graphs."""
     if not (isinstance(cells, torch.Tensor) and cells.is_cuda):
         raise TypeError("cells must be a device tensor")
     cells = cells.to(torch.complex64).contiguous()
@@ -62,7 +65,8 @@ def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=Non
     dev = cells.device
     N = C * nb
     llr = out if out is not None else torch.empty((B, N), dtype=torch.float32, device=dev)
-    s2out = torch.empty(B, dtype=torch.float64, device=dev)
+    s2out = (torch.empty(B, dtype=torch.float64, device=dev)
+             if want_sigma2 or sigma2 is None else None)
     if B == 0 or C == 0:
         return llr, s2out
     csi_p = None
@@ -83,8 +87,72 @@ def demap_q_dev(cells, pts_dev, nb, lam_of_q_dev, csi=None, sigma2=None, out=Non
         ws_p = ws.data_ptr()
     _lib.call("atsc3_demap_dev", torch.view_as_real(cells).data_ptr(), csi_p, s2_p, B, C,
               pts_dev.data_ptr(), nb, lam_of_q_dev.data_ptr(), llr.data_ptr(),
-              s2out.data_ptr(), ws_p, _stream(dev))
+              None if s2out is None else s2out.data_ptr(), ws_p, _stream(dev))
     return llr, s2out
+
+
+class _DemapGroup(ctypes.Structure):
+    """struct atsc3_demap_group (include/atsc3_fec.h)."""
+    _fields_ = ([(f, ctypes.c_void_p) for f in ("cells", "csi", "sigma2", "points", "lam_of_q",
+                                                 "llr")]
+                + [(f, ctypes.c_int32) for f in ("ncell", "nbits", "nblk", "blk0")])
+
+
+assert ctypes.sizeof(_DemapGroup) == 64
+
+
+class GroupedDemap:
+    """The demaps of several (Demapper, cells, sigma2, out[, csi]) groups --
+    a frame's blocks of different (N, constellation, rate) -- as ONE launch
+    (``atsc3_demap_groups_dev``), with Demapper.demap_dev's LLRs.  The
+    tensors are referenced, not copied: refill them in place between runs
+    (the descriptor table holds their addresses), so the launch can be
+    captured in a HIP graph."""
+
+    def __init__(self, groups, device):
+        self.device = torch.device(device)
+        self.keep = []
+        arr = (_DemapGroup * len(groups))()
+        blk0, max_ncell = 0, 0
+        for i, g in enumerate(groups):
+            dm, cells, sigma2, out = g[:4]
+            csi = g[4] if len(g) > 4 else None
+            nblk = cells.shape[0]
+            if (cells.dtype != torch.complex64 or not cells.is_contiguous()
+                    or cells.shape[1] != dm.ncell):
+                raise ValueError(f"group {i}: cells must be contiguous complex64 (B, ncell)")
+            if (sigma2.dtype != torch.float64 or sigma2.numel() != nblk
+                    or not sigma2.is_contiguous()):
+                raise ValueError(f"group {i}: sigma2 must be contiguous float64 (B,)")
+            if (out.dtype != torch.float32 or not out.is_contiguous()
+                    or out.numel() != nblk * dm.ncell * dm.nb):
+                raise ValueError(f"group {i}: out must be contiguous float32 (B, n)")
+            if csi is not None and (csi.dtype != torch.float32 or csi.shape != cells.shape
+                                    or not csi.is_contiguous()):
+                raise ValueError(f"group {i}: csi must be contiguous float32 like the cells")
+            for t in (cells, sigma2, out) + ((csi,) if csi is not None else ()):
+                if t.device != self.device:
+                    raise ValueError(f"group {i}: tensors must be on {self.device}")
+            r = arr[i]
+            r.cells = torch.view_as_real(cells).data_ptr()
+            r.csi = None if csi is None else csi.data_ptr()
+            r.sigma2 = sigma2.data_ptr()
+            r.points = dm.points.data_ptr()
+            r.lam_of_q = dm.lam_of_q.data_ptr()
+            r.llr = out.data_ptr()
+            r.ncell, r.nbits, r.nblk, r.blk0 = dm.ncell, dm.nb, nblk, blk0
+            blk0 += nblk
+            max_ncell = max(max_ncell, dm.ncell)
+            self.keep.append((dm, cells, sigma2, out, csi))
+        if blk0 > 65535:
+            raise ValueError("at most 65535 blocks per grouped demap")
+        self.ngroups, self.nblk, self.max_ncell = len(groups), blk0, max_ncell
+        raw = np.frombuffer(bytes(arr), np.uint8).copy()
+        self.table = torch.from_numpy(raw).to(self.device)
+
+    def run(self):
+        _lib.call("atsc3_demap_groups_dev", self.table.data_ptr(), self.ngroups, self.nblk,
+                  self.max_ncell, _stream(self.device))
 
 
 class GatherSpec(ctypes.Structure):
@@ -179,11 +247,13 @@ class Demapper:
         lq = bicm.bicm_map(self.n, mod, rate)
         self.lam_of_q = torch.from_numpy(lq.astype(np.int32)).to(self.device)
 
-    def demap_dev(self, cells, csi=None, sigma2=None, out=None):
-        """(B, ncell) cells -> ((B, n) f32 codeword-order LLRs, sigma2 (B,) f64)."""
+    def demap_dev(self, cells, csi=None, sigma2=None, out=None, want_sigma2=True):
+        """(B, ncell) cells -> ((B, n) f32 codeword-order LLRs, sigma2 (B,) f64;
+        None when sigma2 is given and want_sigma2 is False)."""
         if cells.shape[1] != self.ncell:
             raise ValueError(f"{cells.shape[1]} cells per block, expected {self.ncell}")
-        return demap_q_dev(cells, self.points, self.nb, self.lam_of_q, csi, sigma2, out)
+        return demap_q_dev(cells, self.points, self.nb, self.lam_of_q, csi, sigma2, out,
+                           want_sigma2)
 
     def demap_gather_dev(self, pool, spec, nblk, csi=None, sigma2=None, out=None):
         """Blocks read from ``pool`` through a GatherSpec (the time

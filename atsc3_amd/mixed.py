@@ -7,9 +7,10 @@ constellation) drawn in seeded random proportions from 64800-bit rates
 FEC stage (lab/m9_fast.py:1087-1166: demap, de-interleave, LDPC decode of
 every block), here for all of a frame's blocks at once:
 
-  * one demap launch per (N, constellation, rate) group present in the frame,
-    writing codeword-order LLRs straight into the batched decoder's flat LLR
-    buffer (the groups' blocks are laid out contiguously);
+  * the demaps of every (N, constellation, rate) group present in the frame
+    as ONE launch (atsc3_amd.demap.GroupedDemap), writing codeword-order
+    LLRs straight into the decoder's flat LLR buffer (the groups' blocks are
+    laid out contiguously);
   * ONE persistent batched decode (atsc3_amd.batch) over the whole mix.
 
 ``MixedPipeline`` owns every buffer, so a frame's whole chain is captured in
@@ -29,7 +30,7 @@ import torch
 from . import bicm, synth
 from .batch import BatchPlan
 from .codes import get_code
-from .demap import Demapper
+from .demap import Demapper, GroupedDemap
 
 CODES = ((64800, "6/15"), (64800, "8/15"), (64800, "10/15"), (64800, "12/15"),
          (16200, "4/15"), (16200, "9/15"))
@@ -112,13 +113,24 @@ class MixedPipeline:
             self.cells.append(torch.zeros((cnt, nc), dtype=torch.complex64, device=self.device))
             self.sigma2.append(torch.ones(cnt, dtype=torch.float64, device=self.device))
         self.nblk = len(self.specs)
+        self.grouped = GroupedDemap(
+            [(self.demappers[(n, m, r)], z, s2, self._llr_view(b0, cnt, n))
+             for ((n, r, m), b0, cnt), z, s2 in zip(self.groups, self.cells, self.sigma2)],
+            self.device)
 
-    def stage(self):
-        """The demaps: fill the decoder's LLR buffer (enqueue only)."""
+    def _llr_view(self, b0, cnt, n):
+        lo, hi = int(self.plan.off[b0]), int(self.plan.off[b0 + cnt])
+        return self.plan.llr[lo:hi].view(cnt, n)
+
+    def stage(self, grouped=True):
+        """The demaps: fill the decoder's LLR buffer (enqueue only); one
+        launch for every group, or (grouped False) one per group."""
+        if grouped:
+            self.grouped.run()
+            return
         for ((n, r, m), b0, cnt), z, s2 in zip(self.groups, self.cells, self.sigma2):
-            lo, hi = int(self.plan.off[b0]), int(self.plan.off[b0 + cnt])
-            out = self.plan.llr[lo:hi].view(cnt, n)
-            self.demappers[(n, m, r)].demap_dev(z, sigma2=s2, out=out)
+            self.demappers[(n, m, r)].demap_dev(z, sigma2=s2, out=self._llr_view(b0, cnt, n),
+                                                want_sigma2=False)
 
     def run(self):
         self.stage()

@@ -475,6 +475,27 @@ int32_t atsc3_demap_dev(const float *cells, const float *csi,
                         const int32_t *lam_of_q, float *llr_out,
                         double *sigma2_out, double *ws, void *stream);
 
+/* Several groups of blocks, each with its own (N, constellation, rate), in
+ * ONE launch: the per-(code, constellation) demap calls of a frame's FEC
+ * stage (lab/m9_fast.py:1087-1166, one atsc3_demap_dev each) as one kernel,
+ * with atsc3_demap_dev's arithmetic (bit-identical LLRs), sigma2 given.
+ * groups: device array of ngroups descriptors in increasing blk0, where
+ * group g covers the frame's blocks blk0 .. blk0 + nblk - 1; nblk_total =
+ * the frame's blocks (<= 65535), max_ncell = the largest ncell.  Every
+ * pointer in a descriptor is a device pointer (csi may be NULL). */
+typedef struct atsc3_demap_group {
+    const float *cells;      /* [nblk*ncell] complex64                        */
+    const float *csi;        /* [nblk*ncell] f32 weights, or NULL            */
+    const double *sigma2;    /* [nblk] f64                                    */
+    const float *points;     /* [2^nbits] complex64 alphabet                  */
+    const int32_t *lam_of_q; /* [ncell*nbits]                                 */
+    float *llr;              /* [nblk*ncell*nbits] f32, codeword order        */
+    int32_t ncell, nbits, nblk, blk0;
+} atsc3_demap_group;
+int32_t atsc3_demap_groups_dev(const void *groups, int32_t ngroups,
+                               int32_t nblk_total, int32_t max_ncell,
+                               void *stream);
+
 /* Raw max-log differences and nearest-point distances, no noise scaling:
  * replaces lab/m9_fast.py:756-822 FrameDecoder.demap_batch called with
  * d2min_out (sigma^2 switched off).  llr_out as atsc3_demap_dev with

@@ -184,6 +184,22 @@ def test_resident_entry_rejects_bad_arguments():
     assert call(ws_bytes=ws_need - 1) == E_SIZE
 
 
+def test_demap_groups_entry_rejects_bad_arguments():
+    """atsc3_demap_groups_dev: negative counts and a missing table are refused,
+    more blocks than one launch's grid dimension too; nothing to do is a
+    no-op.  No device call on any of these."""
+    L = _lib().lib()
+    E_ARG, OK = -2, 0
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert L.atsc3_demap_groups_dev(p, -1, 1, 10, None) == E_ARG
+    assert L.atsc3_demap_groups_dev(p, 1, -1, 10, None) == E_ARG
+    assert L.atsc3_demap_groups_dev(p, 1, 1, -1, None) == E_ARG
+    assert L.atsc3_demap_groups_dev(None, 1, 1, 10, None) == E_ARG
+    assert L.atsc3_demap_groups_dev(p, 1, 65536, 10, None) == E_ARG
+    assert L.atsc3_demap_groups_dev(None, 0, 0, 0, None) == OK
+
+
 def test_resident_team_entry_rejects_bad_arguments():
     """atsc3_minsum_resident_team_dev: the resident entry's checks, a team
     size outside 1..32 and a short workspace, all before any device call; an
