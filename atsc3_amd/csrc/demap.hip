@@ -16,9 +16,10 @@
 //   * for y0 the subset with the same imaginary sign as z has minimum
 //     min_k (A + C) and the other min_k (A + D); likewise y1 with B + C.
 // That is the exhaustive max-log over all M points, evaluated exactly (in
-// real arithmetic) with M/4 distance evaluations, each by direct differences
-// (no |p|^2 - 2 Re(z p*) expansion, which loses relative accuracy on the
-// closely spaced 256-NUC points).
+// real arithmetic) with M/4 distance evaluations: A + C by direct
+// differences (no |p|^2 - 2 Re(z p*) expansion, which loses relative
+// accuracy on the closely spaced 256-NUC points), A + D = A + C + 4|zi|wi and
+// B + C = A + C + 4|zr|wr as one fma each (sums of non-negative terms).
 //
 // Output position of q-stream bit q = cell*nb + i is lam_of_q[q]
 // (lab/m6_bicm.py:331-335), so the LLRs land in codeword order.
@@ -69,38 +70,66 @@ __device__ __forceinline__ size_t cell_of(const Gather &g, int blk, int ncell, i
 // Max-log differences of one cell z against the NW = 2^(NB-2) base points w
 // (in LDS): diff[i] = min over y_i = 1 minus min over y_i = 0 (before the
 // CSI weight and sigma2), dmin = the distance to the nearest point.
+//
+// Per base point p = (px, py) >= 0 and z~ = (ar, ai) = (|Re z|, |Im z|):
+//   ac = (ar - px)^2 + (ai - py)^2          the point of z's quadrant
+//   ad = (ar - px)^2 + (ai + py)^2 = ac + 4 ai py   (imaginary sign flipped)
+//   bc = (ar + px)^2 + (ai - py)^2 = ac + 4 ar px   (real sign flipped)
+// -- (a + b)^2 = (a - b)^2 + 4ab, and every term is >= 0, so each distance
+// keeps its relative accuracy (no cancellation).  The minima of ac over the
+// points whose index bit j is 0 or 1 (label bits y2..y_{nb-1}) come from the
+// row and column minima of the index split as k = h * L + l: bits of h from
+// the rows R[h] = min_l ac, bits of l from the columns C[l] = min_h ac.
 template <int NB>
 __device__ __forceinline__ void demap_cell(float2 z, const float2 *w, float (&diff)[NB],
                                            float &dmin_out) {
     constexpr int NW = 1 << (NB - 2);
+    constexpr int LB = (NB - 2) / 2;             // low index bits (columns)
+    constexpr int L = 1 << LB, H = NW / L;
     const float ar = fabsf(z.x), ai = fabsf(z.y);
-    float dmin = INFINITY, opp_im = INFINITY, opp_re = INFINITY;
-    float m0[NB > 2 ? NB - 2 : 1], m1[NB > 2 ? NB - 2 : 1];
+    const float ar4 = 4.0f * ar, ai4 = 4.0f * ai;  // exact
+    float opp_im = INFINITY, opp_re = INFINITY;
+    float R[H], C[L];
 #pragma unroll
-    for (int i = 0; i < NB - 2; ++i) m0[i] = m1[i] = INFINITY;
+    for (int h = 0; h < H; ++h) R[h] = INFINITY;
+#pragma unroll
+    for (int l = 0; l < L; ++l) C[l] = INFINITY;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
         const float2 p = w[k];
-        const float dr = ar - p.x, sr = ar + p.x;
-        const float di = ai - p.y, si = ai + p.y;
-        const float A = dr * dr, B = sr * sr, C = di * di, D = si * si;
-        const float ac = A + C, ad = A + D, bc = B + C;
-        dmin = fminf(dmin, ac);
-        opp_im = fminf(opp_im, ad);
-        opp_re = fminf(opp_re, bc);
-#pragma unroll
-        for (int i = 2; i < NB; ++i) {
-            if ((k >> (NB - 1 - i)) & 1)
-                m1[i - 2] = fminf(m1[i - 2], ac);
-            else
-                m0[i - 2] = fminf(m0[i - 2], ac);
-        }
+        const float dr = ar - p.x, di = ai - p.y;
+        const float ac = __builtin_fmaf(dr, dr, di * di);
+        opp_im = fminf(opp_im, __builtin_fmaf(ai4, p.y, ac));
+        opp_re = fminf(opp_re, __builtin_fmaf(ar4, p.x, ac));
+        R[k / L] = fminf(R[k / L], ac);
+        C[k % L] = fminf(C[k % L], ac);
     }
+    float dmin = R[0];
+#pragma unroll
+    for (int h = 1; h < H; ++h) dmin = fminf(dmin, R[h]);
     // y0: imaginary sign; y1: real sign (see header)
     diff[0] = z.y >= 0.0f ? opp_im - dmin : dmin - opp_im;
     diff[1] = z.x >= 0.0f ? opp_re - dmin : dmin - opp_re;
+    // label bit i (2..NB-1) is index bit j = NB - 1 - i
 #pragma unroll
-    for (int i = 2; i < NB; ++i) diff[i] = m1[i - 2] - m0[i - 2];
+    for (int i = 2; i < NB; ++i) {
+        const int j = NB - 1 - i;
+        float m0 = INFINITY, m1 = INFINITY;
+        if (j >= LB) {
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                if ((h >> (j - LB)) & 1) m1 = fminf(m1, R[h]);
+                else m0 = fminf(m0, R[h]);
+            }
+        } else {
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                if ((l >> j) & 1) m1 = fminf(m1, C[l]);
+                else m0 = fminf(m0, C[l]);
+            }
+        }
+        diff[i] = m1 - m0;
+    }
     dmin_out = dmin;
 }
 
