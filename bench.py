@@ -53,6 +53,14 @@ sys.path.insert(0, ROOT)
 METRIC = ("FEC blocks/s at FER≤1e-4: batched FP16 vs plain GPU vs CPU; "
           "% of HBM peak")
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of one MI355X: 256 compute units x one wave64 VALU
+# instruction per cycle each (4 SIMDs, 16 lanes) x 2.4 GHz, counted in lane
+# operations
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# VALU instructions per cell of demap_kernel<8, contiguous> (256-NUC: 64
+# base points), SQ_INSTS_VALU / SQ_WAVES of a rocprofv3 counter run (one
+# cell per lane); the static count of the compiled kernel is the same
+DEMAP_VALU_PER_CELL_256 = 692
 FRAME_CELLS = 207_900
 FRAME_SEC = 0.242
 RATE, N, MOD = "10/15", 64800, "256QAM"
@@ -950,8 +958,15 @@ def main():
     hti = HtiGather(args.frames * ncols * ncell, args.frames, ncell, ncols, 0, dev)
     pool, pool_w = z.reshape(-1), w.reshape(-1)
     demap_times = {}
-    for name, fn in (("plain", lambda: dm.demap_dev(z, csi=w, sigma2=n0)),
-                     ("hti_gather", lambda: hti.demap(dm, pool, pool_w, n0))):
+    # sigma2 and the outputs resident on the device: the events time the
+    # demap kernels, not per-call host-to-device copies of sigma2
+    s2_dev = torch.full((nblk,), float(n0), dtype=torch.float64, device=dev)
+    llr_dm = torch.empty((nblk, N), dtype=torch.float32, device=dev)
+    llr_hti = torch.empty((hti.nblk, N), dtype=torch.float32, device=dev)
+    for name, fn in (("plain", lambda: dm.demap_dev(z, csi=w, sigma2=s2_dev, out=llr_dm,
+                                                    want_sigma2=False)),
+                     ("hti_gather", lambda: hti.demap(dm, pool, pool_w, s2_dev[:hti.nblk],
+                                                      llr_hti))):
         fn()
         torch.cuda.synchronize()
         e0.record(cur)
@@ -962,6 +977,28 @@ def main():
         demap_times[name] = round(e0.elapsed_time(e1) / args.steps, 4)
     log(f"[bench] demap {demap_times['plain']:.3f} ms, through the HTI gather "
         f"{demap_times['hti_gather']:.3f} ms ({hti.nblk} blocks)")
+    # the demap against both of its bounds: bytes per cell (the complex64
+    # cell, its f32 CSI weight, nb f32 LLRs written; sigma2 per block and the
+    # de-interleave table, shared by every block and L2-resident, left out)
+    # over HBM, and VALU instructions per cell over the VALU issue rate
+    cells = nblk * ncell
+    nbits = N // ncell
+    dm_bytes = 8 + 4 + 4 * nbits
+    t_dm = demap_times["plain"] / 1e3
+    dm_hbm = cells * dm_bytes / t_dm / 1e9
+    dm_valu = cells * DEMAP_VALU_PER_CELL_256 / t_dm
+    demap_roofline = {
+        "cells": cells, "ms": demap_times["plain"],
+        "hbm": {"bytes_per_cell": dm_bytes, "achieved": round(dm_hbm, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(dm_hbm / HBM_PEAK_GBS, 4)},
+        "valu": {"insts_per_cell": DEMAP_VALU_PER_CELL_256,
+                 "achieved": round(dm_valu / 1e12, 3), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3),
+                 "unit": "T lane-instructions/s", "frac": round(dm_valu / VALU_PEAK_LANE_OPS, 4)},
+    }
+    demap_roofline["bound"] = ("valu" if demap_roofline["valu"]["frac"]
+                               >= demap_roofline["hbm"]["frac"] else "hbm")
+    log(f"[bench] demap roofline: {dm_hbm:.0f} GB/s = {dm_hbm / HBM_PEAK_GBS:.3f} of HBM, "
+        f"VALU {dm_valu / VALU_PEAK_LANE_OPS:.3f} of issue: {demap_roofline['bound']}-bound")
     # common-phase-error correction of the same cells taken as a pool of
     # OFDM symbols of 5,700 cells (3 rounds, 256-NUC alphabet)
     from atsc3_amd.cpe import Cpe
@@ -1188,6 +1225,7 @@ def main():
                    "plain_fp32": plain, "iters_max": 50, "alpha": 0.75,
                    "baseband_ms": round(baseband_ms, 4),
                    "demap_ms_plain_vs_hti_gather": demap_times,
+                   "demap_roofline": demap_roofline,
                    "cpe_ms": cpe_ms,
                    "blocks_per_step": nblk, "mean_its": round(float(its.mean()), 3),
                    "info_gbit_s": round(info_gbps, 3), "realtime_factor": round(rt_factor, 1),
