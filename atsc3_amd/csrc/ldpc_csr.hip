@@ -394,9 +394,20 @@ struct DropCall {
     int32_t *its, *bad;
 };
 
+// Team decodes (each codeword on several compute units of one XCD) need
+// every workgroup of their grid co-resident; two running at once could each
+// hold compute units the other waits for, so they take turns per device.
+constexpr int kTeamDevices = 64;
+std::mutex g_team_mu[kTeamDevices];
+
 // The resident decode of every call's blocks in one launch: LLRs staged
 // into pinned memory call after call, results copied back to each caller's
 // arrays only when the whole decode succeeded (nothing is written on error).
+// A launch with at most half as many codewords as compute units runs the
+// resident engine's team mode (FP32, no LDS copies: the same bits, conv,
+// its, bad), each codeword on 2..16 compute units: a round of the
+// upstream's small thread-pool calls then lasts its slowest codeword's
+// iterations at the team's rate instead of one compute unit's.
 int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t iters,
                               float alpha, const std::vector<DropCall> &calls) {
     const ResEntry &e = *rc->e;
@@ -404,9 +415,14 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
     for (const DropCall &k : calls) nblk += k.nblk;
     const int cus = atsc3_minsum_resident_grid();
     if (cus <= 0) return ATSC3_E_HIP;
-    const int grid = nblk < cus ? nblk : cus;
+    int team = 1;
+    while (team < 16 && (int64_t)nblk * team * 2 <= cus) team *= 2;
+    if (device < 0 || device >= kTeamDevices) team = 1;
+    const int grid = team > 1 ? cus : (nblk < cus ? nblk : cus);
     const int max_checks = e.ncg * ATSC3_Z;
-    const int64_t wsb = atsc3_minsum_resident_ws_bytes(grid, max_checks, e.grows, 0);
+    const int64_t wsb =
+        team > 1 ? atsc3_minsum_resident_team_ws_bytes(grid, team, max_checks, e.grows, 0)
+                 : atsc3_minsum_resident_ws_bytes(grid, max_checks, e.grows, 0);
     if (wsb <= 0) return ATSC3_E_HIP;
     const size_t nb = (size_t)nblk, nn = (size_t)n;
     // device and pinned host share the first three offsets: llr | blk_off | blk_code
@@ -425,6 +441,7 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
     DropCtx *c = acquire_ctx(device);
     if (!c) return ATSC3_E_HIP;
     int32_t rc_out = ATSC3_E_HIP;
+    std::unique_lock<std::mutex> team_turn;      // held until the decode has finished
     do {
         if (!grow(c, d_need, h_need)) break;
         hipStream_t st = c->st;
@@ -451,13 +468,23 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
         uint8_t *dconv = reinterpret_cast<uint8_t *>(c->dev + o_conv);
         int32_t *dits = reinterpret_cast<int32_t *>(c->dev + o_its);
         int32_t *dbad = reinterpret_cast<int32_t *>(c->dev + o_bad);
-        if (atsc3_minsum_resident_dev(reinterpret_cast<const float *>(c->dev), nblk,
-                                      reinterpret_cast<const int64_t *>(c->dev + o_off),
-                                      reinterpret_cast<const int32_t *>(c->dev + o_code),
-                                      rc->desc, 1, max_checks, e.grows, e.n, e.max_dc, e.max_dv,
-                                      iters, alpha, 0, nullptr, grid, c->dev + o_ws, wsb, dbits,
-                                      dconv, dits, dbad, st) != ATSC3_OK)
+        const float *dllr = reinterpret_cast<const float *>(c->dev);
+        const int64_t *doff = reinterpret_cast<const int64_t *>(c->dev + o_off);
+        const int32_t *dcode = reinterpret_cast<const int32_t *>(c->dev + o_code);
+        if (team > 1) {
+            team_turn = std::unique_lock<std::mutex>(g_team_mu[device]);
+            if (atsc3_minsum_resident_team_dev(dllr, nblk, doff, dcode, rc->desc, 1, max_checks,
+                                               e.grows, e.n, e.max_dc, e.max_dv, iters, alpha,
+                                               0, nullptr, nullptr, grid,
+                                               team | ATSC3_TEAM_DIRECT, c->dev + o_ws, wsb,
+                                               dbits, dconv, dits, dbad, st) != ATSC3_OK)
+                break;
+        } else if (atsc3_minsum_resident_dev(dllr, nblk, doff, dcode, rc->desc, 1, max_checks,
+                                             e.grows, e.n, e.max_dc, e.max_dv, iters, alpha, 0,
+                                             nullptr, grid, c->dev + o_ws, wsb, dbits, dconv,
+                                             dits, dbad, st) != ATSC3_OK) {
             break;
+        }
         uint8_t *hb = reinterpret_cast<uint8_t *>(c->host + h_out);
         uint8_t *hc = hb + nb * nn;
         int32_t *hi = reinterpret_cast<int32_t *>(hc + nb);
@@ -468,6 +495,14 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
             hipMemcpyAsync(hd, dbad, nb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             break;
+        if (team > 1) {
+            // a bounded team wait that ran out leaves its = -1: report it
+            // as a device error (nothing written), as any failed decode
+            const int32_t *hits = reinterpret_cast<const int32_t *>(hc + nb);
+            bool aborted = false;
+            for (size_t b = 0; b < nb; ++b) aborted = aborted || hits[b] < 0;
+            if (aborted) break;
+        }
         b0 = 0;
         for (const DropCall &k : calls) {
             const size_t kb = (size_t)k.nblk;
