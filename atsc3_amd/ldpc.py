@@ -154,10 +154,14 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     batches), "batch" (the persistent batched engine, atsc3_amd.batch),
     "csr" (generic kernel; FP32 only), "resident" (any format: one
     workgroup per codeword with its posteriors in LDS, atsc3_amd.resident;
-    FP32 is bit-exact with the plain decoder)
-    or "auto" (the QC engines when the code has a QC form, else csr: the
-    fused engine below PHASE_MIN_BLOCKS blocks, above it the resident
-    engine for FP32 and the batched engine for FP16/INT8).
+    FP32 is bit-exact with the plain decoder), "team" (the resident engine
+    with each codeword on 8 or 16 compute units of one XCD: the same
+    results, far lower latency for a few tens of codewords)
+    or "auto" (the QC engines when the code has a QC form, else csr: below
+    PHASE_MIN_BLOCKS blocks the team mode for FP32 and the fused engine for
+    FP16/INT8, above it the resident engine for FP32 and the batched engine
+    for FP16/INT8 -- FP16/INT8 keep the FP16/INT8 message form of the QC
+    engines, whose posteriors are FP32).
 
     Failure on the device: the batched engine leaves its = -1 (bad = -1,
     conv 0) for every codeword it did not finish (a bounded grid-barrier
@@ -187,18 +191,21 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
         if tabs.qc is None:
             engine = "csr"
         elif nblk < PHASE_MIN_BLOCKS:
-            engine = "qc"
+            # FP32: the team mode, bit-exact like every FP32 engine (32
+            # blocks of 64800 r10/15: 0.35 ms against 2.4 ms fused)
+            engine = "team" if fmt == 0 else "qc"
         else:
             # FP32: the resident engine's FP32 mode (bit-exact, ~2.4x the
             # phased engine on 2566 blocks); FP16/INT8: the batched engine
             engine = "resident" if fmt == 0 else "batch"
-    if engine == "resident":
+    if engine in ("resident", "team"):
         if msg_scale is not None and fmt != 2:
             raise ValueError("msg_scale applies to INT8 messages only")
         from .resident import decode_resident_dev
+        team = None if engine == "resident" else (16 if nblk <= 16 else 8)
         return decode_resident_dev(llr, code, iters, alpha,
                                    msg_dtype={0: "float32", 1: "float16", 2: "int8"}[fmt],
-                                   msg_scale=msg_scale)
+                                   msg_scale=msg_scale, team=team, direct=team is not None)
     if engine == "batch":
         if tabs.qc is None:
             raise ValueError("this code has no QC form with variable degree <= 32")

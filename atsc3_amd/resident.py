@@ -208,6 +208,33 @@ def _host_tables(code, f32=False):
                                     1).ravel()))
 
 
+# Team launches need every workgroup of their grid co-resident; two running
+# at once on different streams could each hold compute units the other
+# waits for.  Each team launch on a device is therefore ordered after the
+# previous one (an event the next launch's stream waits on when it is
+# another stream; nothing is waited on the host).  Not during graph
+# capture, where an event recorded outside the capture cannot be waited on.
+_TEAM_LAST = {}
+
+
+def _team_turn_begin(device):
+    if torch.cuda.is_current_stream_capturing():
+        return
+    ev = _TEAM_LAST.get(str(device))
+    cur = torch.cuda.current_stream(device)
+    if ev is not None and ev[1] != cur.cuda_stream:
+        cur.wait_event(ev[0])
+
+
+def _team_turn_end(device):
+    if torch.cuda.is_current_stream_capturing():
+        return
+    cur = torch.cuda.current_stream(device)
+    ev = torch.cuda.Event()
+    ev.record(cur)
+    _TEAM_LAST[str(device)] = (ev, cur.cuda_stream)
+
+
 class ResidentBank:
     """Device tables of several codes plus their descriptor array, for one
     storage mode (msg_fmt 1 = FP16, 0 = FP32)."""
@@ -376,6 +403,7 @@ class ResidentPlan:
         if self.team is not None:
             if post is not None:
                 raise ValueError("team mode writes no posteriors")
+            _team_turn_begin(self.device)
             if self.order_auto:
                 _lib.call("atsc3_minsum_resident_team_order_dev", src.data_ptr(), self.nblk,
                           self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
@@ -385,6 +413,7 @@ class ResidentPlan:
                       None if self.order is None else self.order.data_ptr(), self.grid,
                       self.team | (TEAM_DIRECT if self.direct else 0), *args[16:],
                       _stream_ptr(self.device))
+            _team_turn_end(self.device)
         elif post is None:
             _lib.call("atsc3_minsum_resident_dev", *args, _stream_ptr(self.device))
         else:
@@ -448,12 +477,12 @@ _PLANS_MAX = 8
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16",
-                        msg_scale=None, team=None):
+                        msg_scale=None, team=None, direct=False):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
     msg_dtype "float32" is the plain decoder (bit-exact with the reference);
     "int8" takes msg_scale (one step per codeword) or derives it.  team:
     team mode (each codeword on that many workgroups of one XCD; same
-    results).
+    results); direct: team mode without LDS copies of the posteriors.
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -476,14 +505,16 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     # a plan's workspace (queue word, records) serves one stream at a time:
     # the stream is part of the key
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
-           _stream_ptr(dev), team)
+           _stream_ptr(dev), team, bool(direct))
     if msg_scale is not None:
         # caller-given steps: a plan of its own (not cached)
-        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale, team)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale, team,
+                            direct=direct)
         return plan.run(llr.reshape(-1), (bits, conv, its, bad))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
-        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, team=team)
+        plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, team=team,
+                            direct=direct)
     _PLANS[key] = plan
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))

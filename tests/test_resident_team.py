@@ -277,3 +277,32 @@ def test_team_auto_order(device, msg):
         assert np.array_equal(a, b)
     if msg == "float16":
         _check_blocks(plan, blocks, got, "auto order")
+
+
+@pytest.mark.gpu
+def test_team_launches_on_two_streams(device):
+    """Team decodes enqueued alternately on two streams (each team launch is
+    ordered after the previous one on the device by an event, so two never
+    hold compute units the other waits for): every result equals the
+    one-stream decode; and decode_dev's automatic choice for a few FP32
+    codewords (the team mode) equals the fused engine bit for bit."""
+    import torch
+    from atsc3_amd import ldpc
+    code, llr, iters, alpha = _case("fixture", "g64k_r10_nuc20")
+    t = torch.from_numpy(llr).to(device)
+    print("[gpu] team launches on two streams", flush=True)
+    ref = [x.cpu().numpy() for x in ldpc.decode_dev(t, code, iters, alpha, "float32", "team")]
+    fused = [x.cpu().numpy() for x in ldpc.decode_dev(t, code, iters, alpha, "float32", "qc")]
+    auto = [x.cpu().numpy() for x in ldpc.decode_dev(t, code, iters, alpha, "float32")]
+    for a, b, c in zip(ref, fused, auto):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    torch.cuda.synchronize()
+    streams = (torch.cuda.Stream(device), torch.cuda.Stream(device))
+    outs = []
+    for k in range(8):
+        with torch.cuda.stream(streams[k % 2]):
+            outs.append(ldpc.decode_dev(t, code, iters, alpha, "float32", "team"))
+    torch.cuda.synchronize()
+    for o in outs:
+        for a, b in zip(o, ref):
+            assert np.array_equal(a.cpu().numpy(), b)
