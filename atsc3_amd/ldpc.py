@@ -157,11 +157,11 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     FP32 is bit-exact with the plain decoder), "team" (the resident engine
     with each codeword on 8 or 16 compute units of one XCD: the same
     results, far lower latency for a few tens of codewords)
-    or "auto" (the QC engines when the code has a QC form, else csr: below
-    PHASE_MIN_BLOCKS blocks the team mode for FP32 and the fused engine for
-    FP16/INT8, above it the resident engine for FP32 and the batched engine
-    for FP16/INT8 -- FP16/INT8 keep the FP16/INT8 message form of the QC
-    engines, whose posteriors are FP32).
+    or "auto" (the QC engines when the code has a QC form, else csr: FP32
+    below PHASE_MIN_BLOCKS blocks the team mode, above it the resident
+    engine; FP16/INT8 the batched engine, which keeps the FP16/INT8 message
+    form of the QC engines, whose posteriors are FP32, and equals the fused
+    engine bit for bit at 2-3x its speed for a few tens of blocks).
 
     Failure on the device: the batched engine leaves its = -1 (bad = -1,
     conv 0) for every codeword it did not finish (a bounded grid-barrier
@@ -190,14 +190,18 @@ def decode_dev(llr, code, iters=50, alpha=0.75, msg_dtype="float32", engine="aut
     if engine == "auto":
         if tabs.qc is None:
             engine = "csr"
+        elif fmt != 0:
+            # the batched engine: the QC engines' FP16/INT8 form (32 blocks
+            # of 64800 r10/15: 0.74 ms against 2.35 ms fused)
+            engine = "batch"
         elif nblk < PHASE_MIN_BLOCKS:
-            # FP32: the team mode, bit-exact like every FP32 engine (32
-            # blocks of 64800 r10/15: 0.35 ms against 2.4 ms fused)
-            engine = "team" if fmt == 0 else "qc"
+            # the team mode, bit-exact like every FP32 engine (32 blocks:
+            # 0.35 ms against 2.4 ms fused)
+            engine = "team"
         else:
-            # FP32: the resident engine's FP32 mode (bit-exact, ~2.4x the
-            # phased engine on 2566 blocks); FP16/INT8: the batched engine
-            engine = "resident" if fmt == 0 else "batch"
+            # the resident engine's FP32 mode (bit-exact, ~2.4x the phased
+            # engine on 2566 blocks)
+            engine = "resident"
     if engine in ("resident", "team"):
         if msg_scale is not None and fmt != 2:
             raise ValueError("msg_scale applies to INT8 messages only")
