@@ -43,6 +43,12 @@
 #include "fec_common.h"
 
 #define DEMAP_WG 256
+// At least 4 waves per SIMD (<= 128 VGPRs): a demap wave then fits beside
+// the 4 waves of a resident decode workgroup (<= 96 VGPRs each, 512 per
+// SIMD), so a demap overlapped with a decode on another stream runs on
+// every compute unit in the decode's idle issue slots instead of waiting
+// for its tail.
+#define DEMAP_MIN_WAVES 4
 
 namespace atsc3 {
 
@@ -103,6 +109,18 @@ __device__ __forceinline__ void demap_cell(float2 z, const float2 *w, float (&di
         opp_re = fminf(opp_re, __builtin_fmaf(ar4, p.x, ac));
         R[k / L] = fminf(R[k / L], ac);
         C[k % L] = fminf(C[k % L], ac);
+        // two rows of base points at a time: the compiler would otherwise
+        // load every point into registers up front (128 VGPRs for 256-NUC);
+        // the rows' results are inputs of an empty statement that also
+        // orders memory, so their arithmetic stays above it and the next
+        // rows' loads below (inputs only: the values stay known to the
+        // compiler, which would otherwise re-quiet them before every fminf)
+        if (k % (2 * L) == 2 * L - 1) {
+            asm volatile("" : : "v"(opp_im), "v"(opp_re), "v"(R[k / L]), "v"(R[k / L - 1])
+                         : "memory");
+#pragma unroll
+            for (int l = 0; l < L; ++l) asm volatile("" : : "v"(C[l]));
+        }
     }
     float dmin = R[0];
 #pragma unroll
@@ -134,7 +152,7 @@ __device__ __forceinline__ void demap_cell(float2 z, const float2 *w, float (&di
 }
 
 template <int NB, int GM>
-__global__ __launch_bounds__(DEMAP_WG) void demap_kernel(
+__global__ __launch_bounds__(DEMAP_WG, DEMAP_MIN_WAVES) void demap_kernel(
     const float2 *__restrict__ cells, const float *__restrict__ csi,
     const double *__restrict__ sigma2, int ncell, const float2 *__restrict__ base,
     const int *__restrict__ lam_of_q, float *__restrict__ llr, double *__restrict__ partial,
@@ -234,7 +252,7 @@ __device__ __forceinline__ void demap_group_block(const DemapGroup &G, int blk, 
     for (int i = 0; i < NB; ++i) out[lq[i]] = (diff[i] * wgt) / s2;
 }
 
-__global__ __launch_bounds__(DEMAP_WG) void demap_groups_kernel(
+__global__ __launch_bounds__(DEMAP_WG, DEMAP_MIN_WAVES) void demap_groups_kernel(
     const DemapGroup *__restrict__ grp, int ngroups) {
     __shared__ float2 w[64];
     const int gb = blockIdx.y;

@@ -60,7 +60,7 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # VALU instructions per cell of demap_kernel<8, contiguous> (256-NUC: 64
 # base points), SQ_INSTS_VALU / SQ_WAVES of a rocprofv3 counter run (one
 # cell per lane); the static count of the compiled kernel is the same
-DEMAP_VALU_PER_CELL_256 = 692
+DEMAP_VALU_PER_CELL_256 = 699
 FRAME_CELLS = 207_900
 FRAME_SEC = 0.242
 RATE, N, MOD = "10/15", 64800, "256QAM"
