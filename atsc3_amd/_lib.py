@@ -106,8 +106,8 @@ _SIGS = {
                                              c_vp],
     "atsc3_minsum_resident_team_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                        c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp, c_vp,
-                                       c_i32, c_i32, c_vp, ctypes.c_int64, c_vp, c_vp, c_vp,
-                                       c_vp, c_vp],
+                                       c_vp, c_i32, c_i32, c_i32, c_vp, ctypes.c_int64, c_vp,
+                                       c_vp, c_vp, c_vp, c_vp],
     "atsc3_minsum_resident_noalloc_max_bytes": [],
     "atsc3_minsum_resident_noalloc_dev": [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32,
                                           c_i32, c_i32, c_i32, c_i32, c_f32, c_i32, c_vp,

@@ -475,7 +475,7 @@ int32_t resident_decode_calls(const ResCode *rc, int device, int32_t n, int32_t 
             team_turn = std::unique_lock<std::mutex>(g_team_mu[device]);
             if (atsc3_minsum_resident_team_dev(dllr, nblk, doff, dcode, rc->desc, 1, max_checks,
                                                e.grows, e.n, e.max_dc, e.max_dv, iters, alpha,
-                                               0, nullptr, nullptr, grid,
+                                               0, nullptr, nullptr, nullptr, 0, grid,
                                                team | ATSC3_TEAM_DIRECT, c->dev + o_ws, wsb,
                                                dbits, dconv, dits, dbad, st) != ATSC3_OK)
                 break;

@@ -849,12 +849,16 @@ struct WsLayout {
 #define TC_BLK 36
 #define TC_WORDS 64
 
+#define RES_MAX_RUNGS 4
+
 struct TeamArgs {
     unsigned *hdr;                    // TH_WORDS header words, then the control blocks
     unsigned char *tpost;             // team posterior slots
     long long pstride;                // bytes per team posterior slot
     const QC_CONST int *order;        // queue position -> codeword, or NULL
     int team;
+    int nrungs;                       // 0: one decode at ResArgs::alpha; else the ladder
+    float ladder[RES_MAX_RUNGS];
 };
 
 // Wait until this wave's vector memory operations (stores and atomics
@@ -1032,72 +1036,94 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
             R.step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, ct->n, s_hist);
             R.inv = 1.0f / R.step;
         }
-        // tot = lam for this member's share of the 64-row tasks, published
-        for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
-            float v[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int task = t8 + k * wst, tg = task / 6;
-                const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-                v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p]
-                                                               : 0.0f;
+        // the alpha ladder (lab/m9_fast.py:843-891): every rung decodes the
+        // codeword afresh from its LLRs; a rung's result is taken when it
+        // converged or leaves fewer unsatisfied checks than the best so far
+        // (the first rung's always), and the ladder stops at convergence
+        const int nr = t.nrungs > 0 ? t.nrungs : 1;
+        int best_bad = 0x7fffffff;
+        for (int r = 0; r < nr; ++r) {
+            const float alpha = t.nrungs > 0 ? t.ladder[r] : a.alpha;
+            if (r > 0) {
+                // every member has read the last rung's total and written its
+                // share of that rung's outputs before the rows are rewritten
+                if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
+                if (leader)
+                    for (int k = 0; k < 3; ++k) st_agent(ctl + TC_UNSAT + k, 0u);
             }
+            bool converged = false;
+            // tot = lam for this member's share of the 64-row tasks, published
+            for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
+                float v[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int task = t8 + k * wst, tg = task / 6;
-                const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-                if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
-                    const unsigned vw = (unsigned)ct->grp_v[tg];
-                    tot_st<M, !DIRECT>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
-                }
-            }
-        }
-        const int nlds = M == 0 ? min(nvg, RES_F32_LDS_ROWS) : nvg;
-        if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
-        if (!DIRECT) team_refresh<M>(lds, T.pub, nlds);
-        if (tid == 0) s_unsat[0] = 0;
-        __syncthreads();
-        for (int it = 0;; ++it) {
-            if (tid == 0) s_unsat[(it + 1) & 1] = 0;
-            const bool first = it == 0, update = it < a.iters;
-            const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update,
-                                                  a.alpha);
-            if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
-            __syncthreads();
-            unsigned *cnt = ctl + TC_UNSAT + it % 3;
-            if (tid == 0 && s_unsat[it & 1])
-                __hip_atomic_fetch_add(cnt, (unsigned)s_unsat[it & 1], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            if (!team_barrier(ctl, ++epoch, S, ab, cnt, s_w)) return;
-            const int total = __builtin_amdgcn_readfirstlane((int)s_w[1]);
-            // counter (it + 1) % 3 was last read in iteration it - 2, before
-            // every member's second barrier of that iteration
-            if (leader) st_agent(ctl + TC_UNSAT + (it + 1) % 3, 0u);
-            if (total == 0 || it == a.iters) {
-                uint8_t *bout = a.bits + off;
-                for (int task = w0; task < nvg * 6; task += wst) {
-                    const int tg = task / 6;
+                for (int k = 0; k < 8; ++k) {
+                    const int task = t8 + k * wst, tg = task / 6;
                     const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-                    const unsigned vw = (unsigned)ct->grp_v[tg];
-                    if (p < (unsigned)ATSC3_Z) {
-                        const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
-                        bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
-                        if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
+                    v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p]
+                                                                   : 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int task = t8 + k * wst, tg = task / 6;
+                    const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+                    if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
+                        const unsigned vw = (unsigned)ct->grp_v[tg];
+                        tot_st<M, !DIRECT>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
                     }
                 }
-                if (leader) {
-                    a.conv[blk] = (uint8_t)(total == 0);
-                    a.its[blk] = total == 0 ? it : a.iters;
-                    a.bad[blk] = total;
-                }
-                break;
             }
-            var_phase<M, !DIRECT>(ct, T, lam, R, w0, wst, lane);
+            const int nlds = M == 0 ? min(nvg, RES_F32_LDS_ROWS) : nvg;
             if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
-            if (!DIRECT) {
-                team_refresh<M>(lds, T.pub, nlds);
+            if (!DIRECT) team_refresh<M>(lds, T.pub, nlds);
+            if (tid == 0) s_unsat[0] = 0;
+            __syncthreads();
+            for (int it = 0;; ++it) {
+                if (tid == 0) s_unsat[(it + 1) & 1] = 0;
+                const bool first = it == 0, update = it < a.iters;
+                const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update,
+                                                      alpha);
+                if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                 __syncthreads();
+                unsigned *cnt = ctl + TC_UNSAT + it % 3;
+                if (tid == 0 && s_unsat[it & 1])
+                    __hip_atomic_fetch_add(cnt, (unsigned)s_unsat[it & 1], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                if (!team_barrier(ctl, ++epoch, S, ab, cnt, s_w)) return;
+                const int total = __builtin_amdgcn_readfirstlane((int)s_w[1]);
+                // counter (it + 1) % 3 was last read in iteration it - 2, before
+                // every member's second barrier of that iteration
+                if (leader) st_agent(ctl + TC_UNSAT + (it + 1) % 3, 0u);
+                if (total == 0 || it == a.iters) {
+                    converged = total == 0;
+                    if (r == 0 || converged || total < best_bad) {
+                        best_bad = total;
+                        uint8_t *bout = a.bits + off;
+                        for (int task = w0; task < nvg * 6; task += wst) {
+                            const int tg = task / 6;
+                            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+                            const unsigned vw = (unsigned)ct->grp_v[tg];
+                            if (p < (unsigned)ATSC3_Z) {
+                                const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
+                                bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
+                                if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
+                            }
+                        }
+                        if (leader) {
+                            a.conv[blk] = (uint8_t)(total == 0);
+                            a.its[blk] = total == 0 ? it : a.iters;
+                            a.bad[blk] = total;
+                        }
+                    }
+                    break;
+                }
+                var_phase<M, !DIRECT>(ct, T, lam, R, w0, wst, lane);
+                if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
+                if (!DIRECT) {
+                    team_refresh<M>(lds, T.pub, nlds);
+                    __syncthreads();
+                }
             }
+            if (converged) break;
         }
     }
 }
@@ -1373,9 +1399,9 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
     const float *llr, int32_t nblk, const int64_t *blk_off, const int32_t *blk_code,
     const void *codes, int32_t ncode, int32_t max_checks, int32_t max_global_rows,
     int32_t max_n, int32_t max_dc, int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
-    const float *msg_scale, const int32_t *order, int32_t grid, int32_t team, void *ws,
-    int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its, int32_t *bad,
-    void *stream) {
+    const float *msg_scale, const int32_t *order, const float *ladder, int32_t nrungs,
+    int32_t grid, int32_t team, void *ws, int64_t ws_bytes, uint8_t *bits, uint8_t *conv,
+    int32_t *its, int32_t *bad, void *stream) {
     const int32_t rc = atsc3::resident_check_args(llr, nblk, blk_off, blk_code, codes, ncode,
                                                   max_checks, max_global_rows, max_n, max_dc,
                                                   max_dv, iters, alpha, msg_fmt, grid, bits,
@@ -1384,6 +1410,9 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
     const bool direct = (team & ATSC3_TEAM_DIRECT) != 0;
     team &= ~ATSC3_TEAM_DIRECT;
     if (team < 1 || team > RES_TEAM_MAX) return ATSC3_E_ARG;
+    if (nrungs < 0 || nrungs > RES_MAX_RUNGS || (nrungs > 0 && !ladder)) return ATSC3_E_ARG;
+    for (int r = 0; r < nrungs; ++r)
+        if (!(ladder[r] >= 0.0f)) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!ws) return ATSC3_E_ARG;
     const bool f32 = msg_fmt == 0;
@@ -1418,6 +1447,8 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
     t.pstride = L.pstride;
     t.order = (const QC_CONST int *)order;
     t.team = team;
+    t.nrungs = nrungs;
+    for (int r = 0; r < nrungs; ++r) t.ladder[r] = ladder[r];
     hipStream_t st = atsc3::as_stream(stream);
     const int words = (int)L.words();
     const int n0 = words > nblk ? words : nblk;

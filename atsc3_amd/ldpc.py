@@ -420,11 +420,21 @@ def ladder_dev(llr, code, ladder=(1.0, 0.85, 0.75), iters=50, msg_dtype="float32
                engine="auto"):
     """Alpha ladder on device tensors; returns device (bits, conv, its, bad).
 
-    Engines "auto" and "batch" (up to 4 rungs) run the whole ladder inside
-    the batched persistent kernel: no host sync.  Otherwise one decode per
-    rung and one host sync per rung (the number of rows still failing)."""
+    Up to 4 rungs run inside one launch, with no host sync: engine "team"
+    (the resident engine's team mode; "auto" picks it for FP32 batches
+    below PHASE_MIN_BLOCKS codewords, where it is the faster) or "batch" /
+    "auto" (the batched persistent kernel).  Otherwise one decode per rung
+    and one host sync per rung (the number of rows still failing)."""
     nb = llr.shape[0]
     dev = llr.device
+    if len(ladder) <= 4 and (engine == "team" or (engine == "auto" and msg_dtype == "float32"
+                                                  and nb < PHASE_MIN_BLOCKS)):
+        from .resident import decode_resident_dev
+        llr = llr.contiguous()
+        if llr.dtype != torch.float32:
+            llr = llr.float()
+        return decode_resident_dev(llr, code, iters, ladder[0], msg_dtype=msg_dtype,
+                                   team=16 if nb <= 16 else 8, direct=True, ladder=ladder)
     if engine in ("auto", "batch") and len(ladder) <= 4:
         from .batch import decode_batch_dev
         return decode_batch_dev(llr, code, iters, ladder[0], msg_dtype, ladder=ladder)

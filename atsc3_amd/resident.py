@@ -285,7 +285,8 @@ class ResidentPlan:
     its bits the same range of ``bits``."""
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
-                 msg_dtype="float16", msg_scale=None, team=None, order=None, direct=False):
+                 msg_dtype="float16", msg_scale=None, team=None, order=None, direct=False,
+                 ladder=None):
         """msg_dtype "float16": the FP16 form; "int8": INT8 record magnitudes
         with a per-codeword step (msg_scale, one per codeword, or None: 6 *
         median|llr| / 127 on the device); "float32": the plain decoder,
@@ -300,7 +301,8 @@ class ResidentPlan:
         the most iterations, and a slow codeword that starts late sets the
         launch's time).  direct: team mode without the members' LDS copies
         of the posteriors (every member reads the team's rows in L2; same
-        results)."""
+        results).  ladder: team mode only, 1..4 alphas: the alpha ladder
+        inside the launch (``alpha`` unused), the results of ldpc.ladder_dev."""
         fmts = {"float32": 0, "float16": 1, "int8": 2}
         if msg_dtype not in fmts:
             raise ValueError("the resident engine stores float32, float16 or int8 records")
@@ -339,6 +341,13 @@ class ResidentPlan:
         self.direct = bool(direct)
         if self.direct and self.team is None:
             raise ValueError("direct applies to team mode only")
+        self.ladder = None
+        if ladder is not None:
+            if self.team is None:
+                raise ValueError("the ladder runs in team mode only")
+            self.ladder = [float(np.float32(a)) for a in ladder]
+            if not 1 <= len(self.ladder) <= 4 or min(self.ladder) < 0:
+                raise ValueError("ladder needs 1..4 alphas >= 0")
         dev = self.device
         self.blk_code = torch.from_numpy(self.code_idx).to(dev)
         self.blk_off = torch.from_numpy(self.off[:-1].copy()).to(dev)
@@ -409,8 +418,12 @@ class ResidentPlan:
                           self.blk_off.data_ptr(), self.blk_code.data_ptr(), b.desc.data_ptr(),
                           self.fmt, self.order_key.data_ptr(), self.order.data_ptr(),
                           _stream_ptr(self.device))
+            lad = None
+            if self.ladder is not None:
+                lad = (ctypes.c_float * len(self.ladder))(*self.ladder)
             _lib.call("atsc3_minsum_resident_team_dev", *args[:15],
-                      None if self.order is None else self.order.data_ptr(), self.grid,
+                      None if self.order is None else self.order.data_ptr(), lad,
+                      0 if self.ladder is None else len(self.ladder), self.grid,
                       self.team | (TEAM_DIRECT if self.direct else 0), *args[16:],
                       _stream_ptr(self.device))
             _team_turn_end(self.device)
@@ -477,12 +490,13 @@ _PLANS_MAX = 8
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16",
-                        msg_scale=None, team=None, direct=False):
+                        msg_scale=None, team=None, direct=False, ladder=None):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
     msg_dtype "float32" is the plain decoder (bit-exact with the reference);
     "int8" takes msg_scale (one step per codeword) or derives it.  team:
     team mode (each codeword on that many workgroups of one XCD; same
-    results); direct: team mode without LDS copies of the posteriors.
+    results); direct: team mode without LDS copies of the posteriors;
+    ladder: team mode, 1..4 alphas run as the alpha ladder (alpha unused).
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -504,17 +518,18 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
         return bits, conv, its, bad
     # a plan's workspace (queue word, records) serves one stream at a time:
     # the stream is part of the key
+    lad = None if ladder is None else tuple(float(np.float32(a)) for a in ladder)
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
-           _stream_ptr(dev), team, bool(direct))
+           _stream_ptr(dev), team, bool(direct), lad)
     if msg_scale is not None:
         # caller-given steps: a plan of its own (not cached)
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale, team,
-                            direct=direct)
+                            direct=direct, ladder=lad)
         return plan.run(llr.reshape(-1), (bits, conv, its, bad))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, team=team,
-                            direct=direct)
+                            direct=direct, ladder=lad)
     _PLANS[key] = plan
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))

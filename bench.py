@@ -508,8 +508,9 @@ def ldm_cti_bench(args, dev, fmt="float16", nrows=1024, every=4, engine="auto"):
 def frame_chain_bench(args, dev, esn0=19.0, frames=(1, 8)):
     """One RF33-like frame through the whole device chain, per frame: CPE of
     the 211,472-cell pool, the 74-block 64QAM 11/15 PLP demapped through the
-    hybrid time de-interleaver's gather, the alpha ladder in the batched
-    decoder, the sum-product rescue, BCH + descrambling; the QPSK 2/15 PLP
+    hybrid time de-interleaver's gather, the alpha ladder inside one decode
+    launch (the team mode for one frame, the batched engine for eight), the
+    sum-product rescue, BCH + descrambling; the QPSK 2/15 PLP
     demapped and decoded beside it.  Synthetic frames (random information
     bits, so the BCH verdict is not the check here; block errors are counted
     against the sent codewords), per-symbol gain and AWGN.  Timed for one
@@ -570,8 +571,7 @@ def frame_chain_bench(args, dev, esn0=19.0, frames=(1, 8)):
             ev[1].record(cur)
             llr0, _ = hti.demap(dm0, z)
             ev[2].record(cur)
-            bits, conv, its, bad = ldpc.ladder_dev(llr0, code0, (1.0, 0.85, 0.75), 50,
-                                                   engine="batch")
+            bits, conv, its, bad = ldpc.ladder_dev(llr0, code0, (1.0, 0.85, 0.75), 50)
             ev[3].record(cur)
             rescue_dev(llr0, bits, conv, its, bad, code0, budget_s=0.12, skip_if_idle=True)
             ev[4].record(cur)

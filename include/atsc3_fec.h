@@ -387,6 +387,14 @@ int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
  * (one workgroup per compute unit; larger grids return ATSC3_E_SIZE).
  *   order  NULL, or nblk codeword indices: the queue's order (a team takes
  *          the codewords order[0], order[1], ... as it becomes free)
+ *   ladder, nrungs  nrungs = 0: one decode at `alpha`; 1..4: the alpha
+ *          ladder of lab/m9_fast.py:843-891 inside the launch, `ladder` the
+ *          host array of the rungs' alphas (each >= 0; `alpha` unused): each
+ *          rung decodes the codeword afresh from its LLRs, a rung's result
+ *          is taken when it converged or leaves fewer unsatisfied checks
+ *          than the best so far (the first rung's always), and a codeword
+ *          stops at its first converged rung -- the results of
+ *          atsc3_amd.ldpc.ladder_dev / the batched engine's ladder
  *   ws     atsc3_minsum_resident_team_ws_bytes(grid, team, max_checks,
  *          max_global_rows, msg_fmt) bytes: control words, per-team records
  *          and published posterior rows
@@ -423,9 +431,9 @@ int32_t atsc3_minsum_resident_team_dev(
     const int32_t *blk_code, const void *codes, int32_t ncode,
     int32_t max_checks, int32_t max_global_rows, int32_t max_n, int32_t max_dc,
     int32_t max_dv, int32_t iters, float alpha, int32_t msg_fmt,
-    const float *msg_scale, const int32_t *order, int32_t grid, int32_t team,
-    void *ws, int64_t ws_bytes, uint8_t *bits, uint8_t *conv, int32_t *its,
-    int32_t *bad, void *stream);
+    const float *msg_scale, const int32_t *order, const float *ladder,
+    int32_t nrungs, int32_t grid, int32_t team, void *ws, int64_t ws_bytes,
+    uint8_t *bits, uint8_t *conv, int32_t *its, int32_t *bad, void *stream);
 
 /* ------------------------------------------------------------------------
  * Resident min-sum with no allocation.  No counterpart in the upstream: it

@@ -202,7 +202,8 @@ def test_demap_groups_entry_rejects_bad_arguments():
 
 def test_resident_team_entry_rejects_bad_arguments():
     """atsc3_minsum_resident_team_dev: the resident entry's checks, a team
-    size outside 1..32 and a short workspace, all before any device call; an
+    size outside 1..32, a ladder of more than 4 rungs, without alphas or with
+    a negative one, and a short workspace, all before any device call; an
     empty batch is a no-op."""
     L = _lib().lib()
     E_ARG, E_SIZE, OK = -2, -4, 0
@@ -210,10 +211,13 @@ def test_resident_team_entry_rejects_bad_arguments():
     p = ctypes.cast(buf, ctypes.c_void_p)
     ws_need = L.atsc3_minsum_resident_team_ws_bytes(16, 8, 21600, 0, 1)
 
-    def call(nblk=1, team=8, alpha=0.75, fmt=1, ptr=p, ws_bytes=None):
+    lad = (ctypes.c_float * 4)(1.0, 0.85, 0.75, -1.0)
+
+    def call(nblk=1, team=8, alpha=0.75, fmt=1, ptr=p, ws_bytes=None, ladder=None, nrungs=0):
         return L.atsc3_minsum_resident_team_dev(ptr, nblk, ptr, ptr, ptr, 1, 21600, 0, 64800,
-                                                14, 13, 50, alpha, fmt, None, None, 16, team,
-                                                ptr, ws_need if ws_bytes is None else ws_bytes,
+                                                14, 13, 50, alpha, fmt, None, None, ladder,
+                                                nrungs, 16, team, ptr,
+                                                ws_need if ws_bytes is None else ws_bytes,
                                                 ptr, ptr, ptr, ptr, None)
     assert call(nblk=-1) == E_ARG
     assert call(alpha=-1.0) == E_ARG
@@ -223,6 +227,10 @@ def test_resident_team_entry_rejects_bad_arguments():
     assert call(nblk=0, ptr=None) == OK
     assert call(ptr=None) == E_ARG
     assert call(ws_bytes=ws_need - 1) == E_SIZE
+    assert call(ladder=lad, nrungs=5) == E_ARG          # more than 4 rungs
+    assert call(ladder=None, nrungs=2) == E_ARG         # rungs without a ladder
+    assert call(ladder=lad, nrungs=4) == E_ARG          # a negative rung alpha
+    assert call(nblk=0, ptr=None, ladder=lad, nrungs=3) == OK
     assert L.atsc3_minsum_resident_team_ws_bytes(16, 0, 21600, 0, 1) == E_ARG
     assert L.atsc3_minsum_resident_team_ws_bytes(16, 33, 21600, 0, 1) == E_ARG
 

@@ -306,3 +306,38 @@ def test_team_launches_on_two_streams(device):
     for o in outs:
         for a, b in zip(o, ref):
             assert np.array_equal(a.cpu().numpy(), b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float32", "float16"])
+def test_team_ladder_equals_rung_by_rung(device, msg):
+    """The alpha ladder inside one team launch (1.0, 0.85, 0.75; 20
+    iterations) equals the host-driven ladder -- one resident decode per
+    rung on the rows still failing, a rung's result taken when it converged
+    or left fewer unsatisfied checks -- on blocks spread over the waterfall
+    (some converge at the first rung, some at a later one, some never); in
+    FP32 also the batched engine's in-kernel ladder."""
+    import torch
+    from atsc3_amd import ldpc, synth
+    from atsc3_amd.codes import get_code
+    code = get_code("10/15", 64800)
+    cw = synth.codewords("10/15", 64800, 24, 77)
+    llr = np.concatenate([synth.llr_bpsk_like(cw[i:i + 1], 3.0 + 0.08 * i, 78 + i)
+                          for i in range(24)]).astype(np.float32)
+    t = torch.from_numpy(llr).to(device)
+    lad = (1.0, 0.85, 0.75)
+    print(f"[gpu] team ladder {msg}", flush=True)
+    got = [x.cpu().numpy() for x in ldpc.ladder_dev(t, code, lad, 20, msg, engine="team")]
+    ref = [x.cpu().numpy() for x in ldpc.ladder_dev(t, code, lad, 20, msg, engine="resident")]
+    for a, b, nm in zip(got, ref, ("bits", "conv", "its", "bad")):
+        assert np.array_equal(a, b), nm
+    # the blocks exercise every branch of the ladder
+    single = [x.cpu().numpy() for x in ldpc.decode_dev(t, code, 20, 1.0, msg, "resident")]
+    assert single[1].any() and not single[1].all()
+    assert (got[1].astype(bool) & ~single[1].astype(bool)).any()
+    assert not got[1].all()
+    if msg == "float32":
+        bat = [x.cpu().numpy() for x in ldpc.ladder_dev(t, code, lad, 20, msg, engine="batch")]
+        auto = [x.cpu().numpy() for x in ldpc.ladder_dev(t, code, lad, 20, msg)]
+        for a, b, c in zip(got, bat, auto):
+            assert np.array_equal(a, b) and np.array_equal(a, c)
