@@ -1128,6 +1128,374 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Grow mode: the resident kernel's one workgroup per codeword, whose last
+// codewords grow into teams.
+//
+// Once the queue is empty, a workgroup that finishes its codeword has
+// nothing left to start, while the codewords still running (the slow ones
+// of the launch's last round) hold one compute unit each: the launch ends
+// with most units idle.  Here a workgroup that finds the queue empty
+// becomes a helper: it takes a seat on its XCD's seat counter and waits.
+// After every variable phase a codeword's owner reads its XCD's seat and
+// claim counters; when `team` - 1 unclaimed seats wait, it claims them (one
+// compare-and-swap on the claim counter), copies its LDS posterior rows to
+// its global slot (the layout the direct team flavor reads) and posts a
+// request; the seated helpers read it, and the team (the owner is member 0)
+// runs the codeword's remaining iterations as resident_team_kernel's direct
+// flavor does: the same steps dealt member-major, the posteriors in the
+// slot's global rows, the records in the owner's record slot.  Seat s of an
+// XCD serves that XCD's request s / (team - 1) as member 1 + s % (team - 1).
+// When the team finishes every member becomes a helper again.  A helper
+// leaves when every codeword is finished (the done counter reaches nblk),
+// when its seat lies past the request table, or on the abort word.  A
+// codeword grows at most once, so each owner's control block serves one
+// team.  Results are the resident kernel's bit for bit: every step computes
+// the same values whichever wave runs it.  Owners never wait for helpers
+// they have not claimed, and claimed helpers are running, so no wait
+// depends on workgroups that are not resident.
+// ---------------------------------------------------------------------------
+#define GH_DONE 0                     // codewords finished (grow header words)
+#define GH_XCD 32                     // per XCD, its own line: seats taken, seats claimed
+#define GH_REQ (GH_XCD + RES_XCDS * 32)
+#define GROW_RMAX 64                  // requests per XCD
+#define GR_WORDS 8                    // request: posted, codeword, owner, first iteration, step
+#define GH_WORDS (GH_REQ + RES_XCDS * GROW_RMAX * GR_WORDS)
+
+// tot = lam for the 64-row tasks w0, w0 + wst, ... (8 loads in flight)
+template <int M>
+__device__ __forceinline__ void res_tot_init(const QC_CONST ResTab *ct, const Tot &T,
+                                             const float *lam, int w0, int wst, unsigned lane) {
+    const int nvg = ct->nvg;
+    for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int task = t8 + k * wst, tg = task / 6;
+            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+            v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int task = t8 + k * wst, tg = task / 6;
+            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+            if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
+                const unsigned vw = (unsigned)ct->grp_v[tg];
+                tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
+            }
+        }
+    }
+}
+
+// Hard decisions (and posteriors when asked) of the 64-row tasks w0, w0 +
+// wst, ...
+template <int M>
+__device__ __forceinline__ void res_write_out(const ResArgs &a, const QC_CONST ResTab *ct,
+                                              const Tot &T, long long off, int w0, int wst,
+                                              unsigned lane) {
+    uint8_t *bout = a.bits + off;
+    for (int task = w0; task < ct->nvg * 6; task += wst) {
+        const int tg = task / 6;
+        const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+        const unsigned vw = (unsigned)ct->grp_v[tg];
+        if (p < (unsigned)ATSC3_Z) {
+            const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
+            bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
+            if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
+        }
+    }
+}
+
+// The owner's LDS posterior rows, byte for byte, to its global slot: 16-byte
+// chunks, 8 in flight per thread.
+__device__ __forceinline__ void grow_publish(const unsigned char *lds, unsigned char *pub,
+                                             unsigned bytes) {
+    const unsigned total = bytes / 16u;
+    for (unsigned c0 = threadIdx.x; c0 < total; c0 += 8u * RES_WG) {
+        uint4 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned c = c0 + (unsigned)q * RES_WG;
+            // past the end: chunk 0 again (in bounds, not stored)
+            v[q] = reinterpret_cast<const uint4 *>(lds)[c < total ? c : 0u];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned c = c0 + (unsigned)q * RES_WG;
+            if (c < total) at(reinterpret_cast<uint4 *>(pub), c) = v[q];
+        }
+    }
+}
+
+// Thread 0 of a helper: 1 once the request at rq is posted, 0 once every
+// codeword is finished, -1 on the abort word or when the bound runs out.
+__device__ int grow_wait(const unsigned *rq, const unsigned *done, unsigned nblk, unsigned *ab) {
+    for (unsigned spins = 1;; ++spins) {
+        if (ld_agent(rq)) return 1;
+        // a posted request's codeword cannot finish without this member, so
+        // done == nblk means no request will come for this seat
+        if (ld_agent(done) >= nblk) return 0;
+        if ((spins & 15u) == 0u) {
+            if (ld_agent(ab)) return -1;
+            if (spins >= RES_SPIN_LIMIT) {
+                st_agent(ab, 1u);
+                return -1;
+            }
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+// A grown codeword's iterations from `from` on: member m of a team of S,
+// the owner's slot `own` (records, global rows, control block).  False on
+// the abort word.
+template <int M>
+__device__ __forceinline__ bool grow_team(const ResArgs &a, const TeamArgs &t, unsigned *ctl,
+                                          unsigned *done, unsigned *ab, int blk, unsigned own,
+                                          unsigned m, unsigned S, int from, float step,
+                                          int *s_unsat, unsigned *s_w) {
+    const unsigned tid = threadIdx.x, lane = tid & 63u;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    Rec R;
+    R.p01 = a.p01 + (size_t)own * a.stride;
+    R.p0b = M == 0 ? a.p0b + (size_t)own * a.stride : nullptr;
+    R.p2 = a.p2 + (size_t)own * a.stride;
+    R.step = step;
+    R.inv = 1.0f / step;
+    Tot T;
+    T.pub = t.tpost + (size_t)own * t.pstride;
+    T.lds = T.pub;
+    T.glob = M == 0 ? a.totg + (size_t)own * a.tstride : nullptr;
+    const QC_CONST ResTab *ct = &a.codes[a.blk_code[blk]];
+    const long long off = a.blk_off[blk];
+    const float *lam = a.llr + off;
+    const int w0 = wave * (int)S + (int)m, wst = (int)S * RES_WAVES;
+    const bool leader = m == 0u && tid == 0u;
+    unsigned epoch = 0;
+    if (tid == 0) {
+        s_unsat[0] = 0;
+        s_unsat[1] = 0;
+    }
+    __syncthreads();
+    for (int it = from;; ++it) {
+        if (tid == 0) s_unsat[(it + 1) & 1] = 0;
+        const bool first = it == 0, update = it < a.iters;
+        const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update, a.alpha);
+        if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+        __syncthreads();
+        unsigned *cnt = ctl + TC_UNSAT + it % 3;
+        if (tid == 0 && s_unsat[it & 1])
+            __hip_atomic_fetch_add(cnt, (unsigned)s_unsat[it & 1], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (!team_barrier(ctl, ++epoch, S, ab, cnt, s_w)) return false;
+        const int total = __builtin_amdgcn_readfirstlane((int)s_w[1]);
+        // counter (it + 1) % 3 was last read in iteration it - 2, before
+        // every member's second barrier of that iteration
+        if (leader) st_agent(ctl + TC_UNSAT + (it + 1) % 3, 0u);
+        if (total == 0 || it == a.iters) {
+            res_write_out<M>(a, ct, T, off, w0, wst, lane);
+            if (leader) {
+                a.conv[blk] = (uint8_t)(total == 0);
+                a.its[blk] = total == 0 ? it : a.iters;
+                a.bad[blk] = total;
+                __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return true;
+        }
+        var_phase<M, false>(ct, T, lam, R, w0, wst, lane);
+        if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return false;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamArgs t) {
+    __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
+    __shared__ int s_blk;
+    __shared__ int s_unsat[2];
+    __shared__ int s_hist[M == 2 ? 256 : 1];
+    __shared__ unsigned s_w[8];
+    const unsigned tid = threadIdx.x, lane = tid & 63u;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    unsigned *hdr = t.hdr, *ab = hdr + TH_ABORT, *gh = hdr + TH_WORDS;
+    unsigned *done = gh + GH_DONE;
+    unsigned *ctl0 = gh + GH_WORDS;
+    const unsigned x = res_xcc_id();
+    unsigned *xw = gh + GH_XCD + x * 32u;
+    unsigned *reqs = gh + GH_REQ + x * (GROW_RMAX * GR_WORDS);
+    const unsigned S = (unsigned)t.team, need = S - 1u;
+    const unsigned me = blockIdx.x;
+    bool solo = true;                 // still taking codewords from the queue
+    for (;;) {
+        // the next team this workgroup joins: (codeword, owner, member,
+        // first iteration, INT8 step)
+        int tblk = -1, tfrom = 0;
+        unsigned town = 0, tm = 0;
+        float tstep = 1.0f;
+        if (solo) {
+            Rec R;
+            R.p01 = a.p01 + (size_t)me * a.stride;
+            R.p0b = M == 0 ? a.p0b + (size_t)me * a.stride : nullptr;
+            R.p2 = a.p2 + (size_t)me * a.stride;
+            R.step = 1.0f;
+            R.inv = 1.0f;
+            Tot T;
+            T.pub = t.tpost + (size_t)me * t.pstride;
+            T.lds = lds;
+            // FP32: the rows beyond LDS, where the team reads them after
+            // growing
+            T.glob = M == 0 ? a.totg + (size_t)me * a.tstride : nullptr;
+            for (;;) {
+                if (tid == 0) {
+                    s_blk = (int)__hip_atomic_fetch_add(hdr + TH_QUEUE, 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    s_unsat[0] = 0;
+                    s_unsat[1] = 0;
+                }
+                __syncthreads();
+                const int q = __builtin_amdgcn_readfirstlane(s_blk);
+                if (q >= a.nblk) {
+                    solo = false;
+                    break;
+                }
+                const int blk = t.order ? t.order[q] : q;
+                const QC_CONST ResTab *ct = &a.codes[a.blk_code[blk]];
+                const int nvg = ct->nvg;
+                const long long off = a.blk_off[blk];
+                const float *lam = a.llr + off;
+                if constexpr (M == 2) {
+                    R.step = a.msg_scale ? a.msg_scale[blk]
+                                         : int8_step_of_block(lam, ct->n, s_hist);
+                    R.inv = 1.0f / R.step;
+                }
+                res_tot_init<M>(ct, T, lam, wave, RES_WAVES, lane);
+                __syncthreads();
+                int from = -1;        // grown: the team's first iteration
+                unsigned rix = 0;
+                for (int it = 0;; ++it) {
+                    if (tid == 0) s_unsat[(it + 1) & 1] = 0;
+                    const bool first = it == 0, update = it < a.iters;
+                    const unsigned unsat = check_phase<M>(ct, T, R, wave, RES_WAVES, lane, first,
+                                                          update, a.alpha);
+                    if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+                    __syncthreads();
+                    const int total = s_unsat[it & 1];
+                    if (total == 0 || it == a.iters) {
+                        res_write_out<M>(a, ct, T, off, wave, RES_WAVES, lane);
+                        if (tid == 0) {
+                            a.conv[blk] = (uint8_t)(total == 0);
+                            a.its[blk] = total == 0 ? it : a.iters;
+                            a.bad[blk] = total;
+                            __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        break;
+                    }
+                    // the XCD's counters, loaded before the variable phase and
+                    // looked at after it (a codeword in its last iteration
+                    // does not grow)
+                    const bool look = need != 0u && tid == 0 && it + 1 < a.iters;
+                    unsigned sv = 0u, cv = 0u;
+                    if (look) {
+                        sv = ld_agent(xw);
+                        cv = ld_agent(xw + 1);
+                    }
+                    var_phase<M, false>(ct, T, lam, R, wave, RES_WAVES, lane);
+                    if (tid == 0) {
+                        s_w[0] = 0u;
+                        // seats only grow and the swap re-reads the claims, so
+                        // sv >= cv + need holds at the swap in whichever order
+                        // the two loads saw the counters
+                        if (look && sv >= cv + need && cv / need < GROW_RMAX) {
+                            unsigned exp = cv;
+                            if (__hip_atomic_compare_exchange_strong(
+                                    xw + 1, &exp, cv + need, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT)) {
+                                s_w[0] = 1u;
+                                s_w[1] = cv / need;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (s_w[0]) {
+                        from = it + 1;
+                        rix = (unsigned)__builtin_amdgcn_readfirstlane((int)s_w[1]);
+                        break;
+                    }
+                }
+                if (from >= 0) {
+                    // the rows (and the records of the phases before) complete
+                    // in L2 before the request is posted
+                    const unsigned nrows =
+                        M == 0 ? (unsigned)min(nvg, RES_F32_LDS_ROWS) : (unsigned)nvg;
+                    grow_publish(lds, T.pub, nrows * (M == 0 ? ATSC3_Z * 4u : RES_PROW * 2u));
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    res_wait_vmem();
+                    __syncthreads();
+                    if (tid == 0) {
+                        unsigned *rq = reqs + rix * GR_WORDS;
+                        st_agent(rq + 1, (unsigned)blk);
+                        st_agent(rq + 2, me);
+                        st_agent(rq + 3, (unsigned)from);
+                        st_agent(rq + 4, __float_as_uint(R.step));
+                        res_wait_vmem();
+                        st_agent(rq, 1u);
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    }
+                    tblk = blk;
+                    town = me;
+                    tm = 0u;
+                    tfrom = from;
+                    tstep = R.step;
+                    // after growing, the workgroup is a helper (the queue is
+                    // empty)
+                    solo = false;
+                    break;
+                }
+                // every thread has read s_unsat and the rows before thread 0
+                // starts the next codeword
+                __syncthreads();
+            }
+        }
+        if (tblk < 0) {
+            // helper: a seat on this XCD, then its request or the end
+            if (tid == 0) {
+                s_w[0] = 0u;
+                if (need) {
+                    const unsigned s = __hip_atomic_fetch_add(xw, 1u, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+                    if (s / need < GROW_RMAX) {
+                        const unsigned *rq = reqs + (s / need) * GR_WORDS;
+                        if (grow_wait(rq, done, (unsigned)a.nblk, ab) > 0) {
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                            s_w[0] = 1u;
+                            s_w[1] = ld_agent(rq + 1);
+                            s_w[2] = ld_agent(rq + 2);
+                            s_w[3] = ld_agent(rq + 3);
+                            s_w[4] = ld_agent(rq + 4);
+                            s_w[5] = 1u + s % need;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (!s_w[0]) return;
+            tblk = __builtin_amdgcn_readfirstlane((int)s_w[1]);
+            town = (unsigned)__builtin_amdgcn_readfirstlane((int)s_w[2]);
+            tfrom = __builtin_amdgcn_readfirstlane((int)s_w[3]);
+            tstep = __uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)s_w[4]));
+            tm = (unsigned)__builtin_amdgcn_readfirstlane((int)s_w[5]);
+        }
+        // every thread has read s_w before the team run reuses it
+        __syncthreads();
+        if (!grow_team<M>(a, t, ctl0 + (size_t)town * TC_WORDS, done, ab, tblk, town, tm, S, tfrom,
+                          tstep, s_unsat, s_w))
+            return;
+        // every thread has read the team's last total (s_w) before thread 0
+        // takes the next seat
+        __syncthreads();
+    }
+}
+
 // Queue order for team mode: the codewords that will need the most
 // iterations first, so that no slow codeword starts late.  The estimate is
 // the fraction of unsatisfied checks of the channel's hard decisions over a
@@ -1205,24 +1573,32 @@ __global__ void team_rank(const float *key, int nblk, int *order) {
 }
 
 struct TeamLayout {
-    int64_t nslots, stride, pstride, hdr, p01, p0b, p2, tpost, bytes;
-    TeamLayout(int64_t grid, int64_t team, int64_t max_checks, int64_t max_grows, bool f32) {
+    int64_t nslots, stride, pstride, tstride, hdr, p01, p0b, p2, totg, tpost, bytes, gwords;
+    // grow: grow mode (one slot per workgroup, the grow header after the
+    // team header words; FP32 global rows in a region of their own, laid out
+    // as the resident kernel's, the slots holding only the LDS rows' copies)
+    TeamLayout(int64_t grid, int64_t team, int64_t max_checks, int64_t max_grows, bool f32,
+               bool grow = false) {
         // at most ceil(n_x / team) teams on each XCD x: grid / team + one per XCD
-        nslots = grid / team + RES_XCDS;
+        nslots = grow ? grid : grid / team + RES_XCDS;
+        gwords = grow ? GH_WORDS : 0;
         stride = (max_checks + 63) & ~(int64_t)63;
+        tstride = f32 && grow ? ((max_grows * ATSC3_Z + 63) & ~(int64_t)63) : 0;
         // FP16: padded rows (the direct flavor reads them as the LDS rows)
-        pstride = f32 ? ((RES_F32_LDS_ROWS + max_grows) * ATSC3_Z * 4 + 255) & ~(int64_t)255
+        pstride = f32 ? ((RES_F32_LDS_ROWS + (grow ? 0 : max_grows)) * ATSC3_Z * 4 + 255) &
+                            ~(int64_t)255
                       : ((int64_t)RES_F16_LDS_BYTES + 255) & ~(int64_t)255;
         hdr = 0;
-        const int64_t words = TH_WORDS + nslots * TC_WORDS;
+        const int64_t words = TH_WORDS + gwords + nslots * TC_WORDS;
         p01 = (words * 4 + 255) & ~(int64_t)255;
         p2 = p01 + nslots * stride * 8;
         p0b = p2 + nslots * stride * 8;
-        tpost = p0b + (f32 ? nslots * stride * 4 : 0);
+        totg = p0b + (f32 ? nslots * stride * 4 : 0);
+        tpost = totg + nslots * tstride * 4;
         tpost = (tpost + 255) & ~(int64_t)255;
         bytes = tpost + nslots * pstride;
     }
-    int64_t words() const { return TH_WORDS + nslots * TC_WORDS; }
+    int64_t words() const { return TH_WORDS + gwords + nslots * TC_WORDS; }
 };
 
 }  // namespace
@@ -1362,11 +1738,12 @@ extern "C" int64_t atsc3_minsum_resident_team_ws_bytes(int32_t grid, int32_t tea
                                                        int32_t max_checks,
                                                        int32_t max_global_rows,
                                                        int32_t msg_fmt) {
-    team &= ~ATSC3_TEAM_DIRECT;
+    const bool grow = (team & ATSC3_TEAM_GROW) != 0;
+    team &= ~(ATSC3_TEAM_DIRECT | ATSC3_TEAM_GROW);
     if (grid < 1 || team < 1 || team > RES_TEAM_MAX || max_checks < 1 || max_global_rows < 0 ||
         msg_fmt < 0 || msg_fmt > 2)
         return ATSC3_E_ARG;
-    return atsc3::TeamLayout(grid, team, max_checks, max_global_rows, msg_fmt == 0).bytes;
+    return atsc3::TeamLayout(grid, team, max_checks, max_global_rows, msg_fmt == 0, grow).bytes;
 }
 
 // Index of the int32 abort word in the team workspace: nonzero after a
@@ -1408,15 +1785,18 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
                                                   conv, its, bad);
     if (rc != ATSC3_OK) return rc;
     const bool direct = (team & ATSC3_TEAM_DIRECT) != 0;
-    team &= ~ATSC3_TEAM_DIRECT;
+    const bool grow = (team & ATSC3_TEAM_GROW) != 0;
+    team &= ~(ATSC3_TEAM_DIRECT | ATSC3_TEAM_GROW);
     if (team < 1 || team > RES_TEAM_MAX) return ATSC3_E_ARG;
     if (nrungs < 0 || nrungs > RES_MAX_RUNGS || (nrungs > 0 && !ladder)) return ATSC3_E_ARG;
+    // grow mode runs one decode per codeword (no ladder)
+    if (grow && nrungs > 0) return ATSC3_E_ARG;
     for (int r = 0; r < nrungs; ++r)
         if (!(ladder[r] >= 0.0f)) return ATSC3_E_ARG;
     if (nblk == 0) return ATSC3_OK;
     if (!ws) return ATSC3_E_ARG;
     const bool f32 = msg_fmt == 0;
-    const atsc3::TeamLayout L(grid, team, max_checks, max_global_rows, f32);
+    const atsc3::TeamLayout L(grid, team, max_checks, max_global_rows, f32, grow);
     if (ws_bytes < L.bytes) return ATSC3_E_SIZE;
     // every workgroup must be co-resident (one per compute unit): the census
     // waits for the whole grid
@@ -1437,6 +1817,8 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
     a.p01 = reinterpret_cast<uint2 *>(base + L.p01);
     a.p2 = reinterpret_cast<unsigned long long *>(base + L.p2);
     a.p0b = f32 ? reinterpret_cast<unsigned *>(base + L.p0b) : nullptr;
+    a.tstride = L.tstride;
+    a.totg = L.tstride ? reinterpret_cast<float *>(base + L.totg) : nullptr;
     a.bits = bits;
     a.conv = conv;
     a.its = its;
@@ -1456,6 +1838,15 @@ extern "C" int32_t atsc3_minsum_resident_team_dev(
                        t.hdr, words, conv, its, bad, nblk);
     if (hipGetLastError() != hipSuccess) return ATSC3_E_HIP;
     const dim3 g(grid), b(RES_WG);
+    if (grow) {
+        if (f32)
+            hipLaunchKernelGGL(atsc3::resident_grow_kernel<0>, g, b, 0, st, a, t);
+        else if (msg_fmt == 1)
+            hipLaunchKernelGGL(atsc3::resident_grow_kernel<1>, g, b, 0, st, a, t);
+        else
+            hipLaunchKernelGGL(atsc3::resident_grow_kernel<2>, g, b, 0, st, a, t);
+        return atsc3::launch_status();
+    }
     if (f32 && direct)
         hipLaunchKernelGGL((atsc3::resident_team_kernel<0, true>), g, b, 0, st, a, t);
     else if (f32)

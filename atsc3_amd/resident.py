@@ -84,6 +84,11 @@ def _pack_tasks(deg, generic, pack):
 F32_LDS_ROWS = 113       # FP32 posterior rows (360 floats) a workgroup holds in LDS
 TEAM_MAX = 32            # workgroups per codeword in team mode
 TEAM_DIRECT = 0x100      # ATSC3_TEAM_DIRECT: team mode without LDS copies
+TEAM_GROW = 0x200        # ATSC3_TEAM_GROW: one workgroup per codeword, the last grow into teams
+# grow mode's team size: the slow codewords of a launch's last round take 7
+# waiting workgroups of their XCD (2566 blocks of 64800 r10/15, FP16: 9.25 ms
+# one workgroup per codeword, 9.00 ms teams of 4, 8.97 ms teams of 8)
+GROW_TEAM = 8
 SIGN_LO = 24             # edge signs a record's word 1 holds (the rest: word 2)
 
 
@@ -286,7 +291,7 @@ class ResidentPlan:
 
     def __init__(self, block_codes, device="cuda", iters=50, alpha=0.75, grid=None,
                  msg_dtype="float16", msg_scale=None, team=None, order=None, direct=False,
-                 ladder=None):
+                 ladder=None, grow=False):
         """msg_dtype "float16": the FP16 form; "int8": INT8 record magnitudes
         with a per-codeword step (msg_scale, one per codeword, or None: 6 *
         median|llr| / 127 on the device); "float32": the plain decoder,
@@ -302,7 +307,12 @@ class ResidentPlan:
         launch's time).  direct: team mode without the members' LDS copies
         of the posteriors (every member reads the team's rows in L2; same
         results).  ladder: team mode only, 1..4 alphas: the alpha ladder
-        inside the launch (``alpha`` unused), the results of ldpc.ladder_dev."""
+        inside the launch (``alpha`` unused), the results of ldpc.ladder_dev.
+        grow: with team = S, grow mode: every codeword starts on one
+        workgroup (the resident engine's decode); once the queue is empty, a
+        codeword still running takes S - 1 waiting workgroups of its XCD and
+        finishes as a team, so the launch's slow last codewords no longer
+        leave most compute units idle (same results; no ladder)."""
         fmts = {"float32": 0, "float16": 1, "int8": 2}
         if msg_dtype not in fmts:
             raise ValueError("the resident engine stores float32, float16 or int8 records")
@@ -341,6 +351,9 @@ class ResidentPlan:
         self.direct = bool(direct)
         if self.direct and self.team is None:
             raise ValueError("direct applies to team mode only")
+        self.grow = bool(grow)
+        if self.grow and (self.team is None or ladder is not None):
+            raise ValueError("grow needs a team size and no ladder")
         self.ladder = None
         if ladder is not None:
             if self.team is None:
@@ -361,7 +374,8 @@ class ResidentPlan:
                 self.grid, self.bank.max_checks, self.bank.max_global_rows, self.fmt)
         else:
             nbytes = _lib.lib().atsc3_minsum_resident_team_ws_bytes(
-                self.grid, self.team, self.bank.max_checks, self.bank.max_global_rows, self.fmt)
+                self.grid, self.team | (TEAM_GROW if self.grow else 0), self.bank.max_checks,
+                self.bank.max_global_rows, self.fmt)
         if nbytes <= 0:
             raise _lib.FecError("atsc3_minsum_resident_ws_bytes", nbytes)
         self.order = None
@@ -424,7 +438,8 @@ class ResidentPlan:
             _lib.call("atsc3_minsum_resident_team_dev", *args[:15],
                       None if self.order is None else self.order.data_ptr(), lad,
                       0 if self.ladder is None else len(self.ladder), self.grid,
-                      self.team | (TEAM_DIRECT if self.direct else 0), *args[16:],
+                      self.team | (TEAM_DIRECT if self.direct else 0)
+                      | (TEAM_GROW if self.grow else 0), *args[16:],
                       _stream_ptr(self.device))
             _team_turn_end(self.device)
         elif post is None:
@@ -490,13 +505,15 @@ _PLANS_MAX = 8
 
 
 def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="float16",
-                        msg_scale=None, team=None, direct=False, ladder=None):
+                        msg_scale=None, team=None, direct=False, ladder=None, grow=None):
     """(nblk, n) device LLRs of one code -> device (bits (nblk, n), conv, its, bad).
     msg_dtype "float32" is the plain decoder (bit-exact with the reference);
     "int8" takes msg_scale (one step per codeword) or derives it.  team:
     team mode (each codeword on that many workgroups of one XCD; same
     results); direct: team mode without LDS copies of the posteriors;
     ladder: team mode, 1..4 alphas run as the alpha ladder (alpha unused).
+    grow: S (2..32): grow mode (one workgroup per codeword; once the queue
+    is empty a running codeword finishes on a team of S; same results).
 
     Enqueues two launches and no host synchronisation; the plan (tables,
     queue, per-workgroup records) is kept for the next call of the same
@@ -520,16 +537,20 @@ def decode_resident_dev(llr, code, iters=50, alpha=0.75, grid=None, msg_dtype="f
     # the stream is part of the key
     lad = None if ladder is None else tuple(float(np.float32(a)) for a in ladder)
     key = (id(code), nblk, int(iters), float(np.float32(alpha)), grid, str(dev), msg_dtype,
-           _stream_ptr(dev), team, bool(direct), lad)
+           _stream_ptr(dev), team, bool(direct), lad, grow)
+    if grow is not None:
+        if team is not None or ladder is not None:
+            raise ValueError("grow takes the team size itself and runs no ladder")
+        team = int(grow)
     if msg_scale is not None:
         # caller-given steps: a plan of its own (not cached)
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, msg_scale, team,
-                            direct=direct, ladder=lad)
+                            direct=direct, ladder=lad, grow=grow is not None)
         return plan.run(llr.reshape(-1), (bits, conv, its, bad))
     plan = _PLANS.pop(key, None)
     if plan is None or plan.bank.codes[0] is not code:
         plan = ResidentPlan([code] * nblk, dev, iters, alpha, grid, msg_dtype, team=team,
-                            direct=direct, ladder=lad)
+                            direct=direct, ladder=lad, grow=grow is not None)
     _PLANS[key] = plan
     while len(_PLANS) > _PLANS_MAX:
         _PLANS.pop(next(iter(_PLANS)))

@@ -731,6 +731,7 @@ def main():
     from atsc3_amd import ldpc, synth
     from atsc3_amd.codes import get_code
     from atsc3_amd.demap import Demapper
+    from atsc3_amd.resident import GROW_TEAM
 
     dev = torch.device("cuda:0")
     if args.copy_calibration:
@@ -1219,7 +1220,10 @@ def main():
                    "esn0_db": op, "messages": args.msg, "accumulation": "float32",
                    "engine": engine,
                    "headline": (f"{args.msg} check-record messages on the resident engine "
-                                f"(one codeword per compute unit, posteriors in LDS)"
+                                f"(one codeword per compute unit, posteriors in LDS"
+                                + (f"; grow mode: the codewords still running once the queue "
+                                   f"is empty finish on teams of {GROW_TEAM} compute units of "
+                                   f"their XCD)" if args.msg != "float32" else ")")
                                 if engine == "resident" else f"{args.msg} messages, {engine} engine"),
                    "posteriors": ("float16 in LDS" if engine == "resident" else "float32"),
                    "plain_fp32": plain, "iters_max": 50, "alpha": 0.75,
@@ -1239,8 +1243,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"resident_kernel<{ {'float32': 0, 'float16': 1, 'int8': 2}[args.msg]}>"
-                               if engine == "resident" else "decode launch",
+                     "kernel": ((f"resident_grow_kernel<{ {'float16': 1, 'int8': 2}[args.msg]}>"
+                                 if args.msg != "float32" else "resident_kernel<0>")
+                                if engine == "resident" else "decode launch"),
                      "model": (f"per block 4N + N + its * (2 E s + 4N), s = {s_msg} bytes per "
                                f"stored message, N = {N}, E = {E}"),
                      "bytes_per_launch": round(model_bytes(its, N, E, s_msg)),

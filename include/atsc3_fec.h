@@ -415,6 +415,15 @@ int32_t atsc3_minsum_resident_llr_dev(const float *llr, int32_t nblk,
 #define ATSC3_TEAM_DIRECT 0x100   /* OR-ed into `team`: no LDS copy of the
                                      posteriors; every member reads the team's
                                      rows in global memory (L2) instead       */
+/* ATSC3_TEAM_GROW, OR-ed into `team` (of the decode and of the ws query):
+ * grow mode.  Every codeword starts on one workgroup, as
+ * atsc3_minsum_resident_dev decodes it; once the queue is empty, workgroups
+ * with nothing left to start wait on their XCD, and a codeword still running
+ * takes `team` - 1 of them (team 1: never) after a variable phase and runs
+ * its remaining iterations as a team (the direct flavor's global rows).  It
+ * removes the idle tail of a launch whose last codewords run long.  Same
+ * results bit for bit; ladder must be absent (nrungs 0); order as above. */
+#define ATSC3_TEAM_GROW 0x200
 int32_t atsc3_minsum_resident_team_order_dev(const float *llr, int32_t nblk,
                                              const int64_t *blk_off,
                                              const int32_t *blk_code,

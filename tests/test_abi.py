@@ -235,6 +235,40 @@ def test_resident_team_entry_rejects_bad_arguments():
     assert L.atsc3_minsum_resident_team_ws_bytes(16, 33, 21600, 0, 1) == E_ARG
 
 
+def test_resident_grow_entry_arguments_and_workspace():
+    """Grow mode (ATSC3_TEAM_GROW OR-ed into team): no ladder (E_ARG before
+    any device call); its workspace, one slot per workgroup (records and
+    published rows; FP32 also the global rows beyond LDS) plus the request
+    tables, is what the decode demands (one byte less: E_SIZE)."""
+    L = _lib().lib()
+    E_ARG, E_SIZE = -2, -4
+    GROW = 0x200
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    lad = (ctypes.c_float * 2)(1.0, 0.75)
+    for fmt in (0, 1, 2):
+        need = L.atsc3_minsum_resident_team_ws_bytes(16, 8 | GROW, 21600, 67, fmt)
+        assert need > 0
+
+        def call(team=8 | GROW, ws_bytes=need, ladder=None, nrungs=0):
+            return L.atsc3_minsum_resident_team_dev(p, 1, p, p, p, 1, 21600, 67, 64800, 14, 13,
+                                                    50, 0.75, fmt, None, None, ladder, nrungs,
+                                                    16, team, p, ws_bytes, p, p, p, p, None)
+        assert call(ladder=lad, nrungs=2) == E_ARG
+        assert call(ws_bytes=need - 1) == E_SIZE
+        assert call(team=0 | GROW) == E_ARG
+        assert call(team=33 | GROW) == E_ARG
+        for grid in (1, 64, 256):
+            ws = L.atsc3_minsum_resident_team_ws_bytes(grid, 4 | GROW, 21600, 67, fmt)
+            # per workgroup: records (2 x 8 bytes per check; FP32 4 more) and
+            # the LDS rows' copies (FP16: 180 padded rows of 424 halves; FP32:
+            # 113 rows of 360 floats, then 67 global rows)
+            per = 2 * 8 * 21600 + (4 * 21600 + 180 * 360 * 4 if fmt == 0 else 180 * 424 * 2)
+            assert ws >= grid * per
+            # the grow header: 16 XCD lines and 16 x 64 requests of 8 words
+            assert ws >= grid * per + 4 * (16 * 32 + 16 * 64 * 8)
+
+
 def test_resident_team_workspace_holds_every_team():
     """Teams are formed per XCD (ranks r of XCD x: team r // team), so a
     launch can have up to sum_x ceil(n_x / team) teams; the workspace has a
