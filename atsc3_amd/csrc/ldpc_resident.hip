@@ -589,14 +589,34 @@ __device__ void var_generic(const QC_CONST ResTab *ct, const Tot &T, const float
     if (valid) tot_st<M, PUB>(T, (vw >> 23) & 255u, vw >> 31, p, lv + s);
 }
 
+// Step dealing.  Static (ctr == nullptr): this wave's steps are w0, w0 +
+// wstep, ...  Dynamic: the waves of a workgroup take their steps one at a
+// time from an LDS counter, so a wave that finds its steps slow (misses, a
+// busy SIMD) takes fewer of them and the phase ends when the work does, not
+// when the slowest share does.  Which wave computes a step does not change
+// any value.
+struct StepDeal {
+    int *ctr;
+    int w0, wstep;
+};
+__device__ __forceinline__ int step_next(const StepDeal &d, unsigned lane, int si) {
+    if (!d.ctr) return si + d.wstep;
+    int v = 0;
+    if (lane == 0u) v = atomicAdd(d.ctr, 1);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int step_first(const StepDeal &d, unsigned lane) {
+    return d.ctr ? step_next(d, lane, 0) : d.w0;
+}
+
 // One CHECK phase over the check steps w0, w0 + wstep, ... (this wave's
 // share); returns the wave's count of unsatisfied checks.
 template <int M>
 __device__ __forceinline__ unsigned check_phase(const QC_CONST ResTab *ct, const Tot &T,
-                                                const Rec &R, int w0, int wstep, unsigned lane,
+                                                const Rec &R, const StepDeal &d, unsigned lane,
                                                 bool first, bool update, float alpha) {
     unsigned unsat = 0u;
-    for (int si = w0; si < ct->nsc; si += wstep) {
+    for (int si = step_first(d, lane); si < ct->nsc; si = step_next(d, lane, si)) {
         const int sw = ct->c_steps[si];
         const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
         if ((sw >> 22) & 1) {
@@ -687,9 +707,9 @@ __device__ unsigned syndrome_generic(const QC_CONST ResTab *ct, const Tot &T, in
 // wstep, ...: the count check_phase would return, at a fraction of its work.
 template <int M>
 __device__ __forceinline__ unsigned syndrome_phase(const QC_CONST ResTab *ct, const Tot &T,
-                                                   int w0, int wstep, unsigned lane) {
+                                                   const StepDeal &d, unsigned lane) {
     unsigned unsat = 0u;
-    for (int si = w0; si < ct->nsc; si += wstep) {
+    for (int si = step_first(d, lane); si < ct->nsc; si = step_next(d, lane, si)) {
         const int sw = ct->c_steps[si];
         const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
         if ((sw >> 22) & 1) {
@@ -738,9 +758,9 @@ __device__ __forceinline__ unsigned syndrome_phase(const QC_CONST ResTab *ct, co
 // (posteriors published, see Tot).
 template <int M, bool PUB>
 __device__ __forceinline__ void var_phase(const QC_CONST ResTab *ct, const Tot &T,
-                                          const float *lam, const Rec &R, int w0, int wstep,
+                                          const float *lam, const Rec &R, const StepDeal &d,
                                           unsigned lane) {
-    for (int si = w0; si < ct->nsv; si += wstep) {
+    for (int si = step_first(d, lane); si < ct->nsv; si = step_next(d, lane, si)) {
         const int sw = ct->v_steps[si];
         const int D = sw & 127, G = 1 << ((sw >> 7) & 7), start = (sw >> 10) & 4095;
         if ((sw >> 22) & 1) {
@@ -805,6 +825,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
     __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
     __shared__ int s_blk;
     __shared__ int s_unsat[3];       // check phases (by iteration parity), syndrome pass
+    __shared__ int s_next[3];        // step counters: syndrome pass, check, variable phase
     __shared__ int s_hist[M == 2 ? 256 : 1];
     const unsigned tid = threadIdx.x, lane = tid & 63u;
     const int wave = __builtin_amdgcn_readfirstlane((int)(tid >> 6));
@@ -821,6 +842,9 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             s_unsat[0] = 0;
             s_unsat[1] = 0;
             s_unsat[2] = 0;
+            s_next[0] = 0;
+            s_next[1] = 0;
+            s_next[2] = 0;
         }
         __syncthreads();
         const int blk = __builtin_amdgcn_readfirstlane(s_blk);
@@ -857,10 +881,14 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         }
         __syncthreads();
         int prev = 1 << 30;          // the last check phase's count
+        // step counters: each is reset by thread 0 in a phase that no wave
+        // can be taking its steps in (between two barriers that enclose none
+        // of its phases)
         for (int it = 0;; ++it) {
+            if (tid == 0) s_next[2] = 0;
             bool stop = false;
             if (it > 0 && prev <= RES_SYN_FIRST) {
-                const unsigned u0 = syndrome_phase<M>(ct, T, wave, RES_WAVES, lane);
+                const unsigned u0 = syndrome_phase<M>(ct, T, StepDeal{&s_next[0], 0, 0}, lane);
                 if (u0) atomicAdd(&s_unsat[2], (int)u0);
                 __syncthreads();
                 stop = s_unsat[2] == 0;
@@ -869,8 +897,8 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             if (!stop) {
                 if (tid == 0) s_unsat[(it + 1) & 1] = 0;
                 const bool first = it == 0, update = it < a.iters;
-                const unsigned unsat = check_phase<M>(ct, T, R, wave, RES_WAVES, lane, first,
-                                                      update, a.alpha);
+                const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{&s_next[1], 0, 0}, lane,
+                                                      first, update, a.alpha);
                 if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                 __syncthreads();
                 total = s_unsat[it & 1];
@@ -898,7 +926,11 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 break;
             }
             prev = total;
-            var_phase<M, false>(ct, T, lam, R, wave, RES_WAVES, lane);
+            if (tid == 0) {
+                s_next[0] = 0;
+                s_next[1] = 0;
+            }
+            var_phase<M, false>(ct, T, lam, R, StepDeal{&s_next[2], 0, 0}, lane);
             __syncthreads();
         }
         // every thread has read s_unsat and tot before thread 0 starts the
@@ -1201,8 +1233,8 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
             for (int it = 0;; ++it) {
                 if (tid == 0) s_unsat[(it + 1) & 1] = 0;
                 const bool first = it == 0, update = it < a.iters;
-                const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update,
-                                                      alpha);
+                const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{nullptr, w0, wst}, lane,
+                                                      first, update, alpha);
                 if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                 __syncthreads();
                 unsigned *cnt = ctl + TC_UNSAT + it % 3;
@@ -1237,7 +1269,7 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
                     }
                     break;
                 }
-                var_phase<M, !DIRECT>(ct, T, lam, R, w0, wst, lane);
+                var_phase<M, !DIRECT>(ct, T, lam, R, StepDeal{nullptr, w0, wst}, lane);
                 if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return;
                 if (!DIRECT) {
                     team_refresh<M>(lds, T.pub, nlds);
@@ -1401,7 +1433,8 @@ __device__ __forceinline__ bool grow_team(const ResArgs &a, const TeamArgs &t, u
     for (int it = from;; ++it) {
         if (tid == 0) s_unsat[(it + 1) & 1] = 0;
         const bool first = it == 0, update = it < a.iters;
-        const unsigned unsat = check_phase<M>(ct, T, R, w0, wst, lane, first, update, a.alpha);
+        const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{nullptr, w0, wst}, lane, first,
+                                              update, a.alpha);
         if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
         __syncthreads();
         unsigned *cnt = ctl + TC_UNSAT + it % 3;
@@ -1423,7 +1456,7 @@ __device__ __forceinline__ bool grow_team(const ResArgs &a, const TeamArgs &t, u
             }
             return true;
         }
-        var_phase<M, false>(ct, T, lam, R, w0, wst, lane);
+        var_phase<M, false>(ct, T, lam, R, StepDeal{nullptr, w0, wst}, lane);
         if (!team_barrier(ctl, ++epoch, S, ab, nullptr, s_w)) return false;
     }
 }
@@ -1433,6 +1466,7 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
     __shared__ __align__(16) unsigned char lds[M == 0 ? RES_F32_LDS_BYTES : RES_F16_LDS_BYTES];
     __shared__ int s_blk;
     __shared__ int s_unsat[3];       // check phases (by iteration parity), syndrome pass
+    __shared__ int s_next[3];        // step counters: syndrome pass, check, variable phase
     __shared__ int s_hist[M == 2 ? 256 : 1];
     __shared__ unsigned s_w[8];
     const unsigned tid = threadIdx.x, lane = tid & 63u;
@@ -1472,6 +1506,9 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                     s_unsat[0] = 0;
                     s_unsat[1] = 0;
                     s_unsat[2] = 0;
+                    s_next[0] = 0;
+                    s_next[1] = 0;
+                    s_next[2] = 0;
                 }
                 __syncthreads();
                 const int q = __builtin_amdgcn_readfirstlane(s_blk);
@@ -1494,10 +1531,13 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                 int from = -1;        // grown: the team's first iteration
                 unsigned rix = 0;
                 int prev = 1 << 30;   // the last check phase's count
+                // step counters reset as in resident_kernel
                 for (int it = 0;; ++it) {
+                    if (tid == 0) s_next[2] = 0;
                     bool stop = false;
                     if (it > 0 && prev <= RES_SYN_FIRST) {
-                        const unsigned u0 = syndrome_phase<M>(ct, T, wave, RES_WAVES, lane);
+                        const unsigned u0 = syndrome_phase<M>(ct, T, StepDeal{&s_next[0], 0, 0},
+                                                              lane);
                         if (u0) atomicAdd(&s_unsat[2], (int)u0);
                         __syncthreads();
                         stop = s_unsat[2] == 0;
@@ -1506,7 +1546,8 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                     if (!stop) {
                         if (tid == 0) s_unsat[(it + 1) & 1] = 0;
                         const bool first = it == 0, update = it < a.iters;
-                        const unsigned unsat = check_phase<M>(ct, T, R, wave, RES_WAVES, lane,
+                        const unsigned unsat = check_phase<M>(ct, T, R,
+                                                              StepDeal{&s_next[1], 0, 0}, lane,
                                                               first, update, a.alpha);
                         if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                         __syncthreads();
@@ -1536,7 +1577,11 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                         sv = ld_agent(xw);
                         cv = ld_agent(xw + 1);
                     }
-                    var_phase<M, false>(ct, T, lam, R, wave, RES_WAVES, lane);
+                    if (tid == 0) {
+                        s_next[0] = 0;
+                        s_next[1] = 0;
+                    }
+                    var_phase<M, false>(ct, T, lam, R, StepDeal{&s_next[2], 0, 0}, lane);
                     if (tid == 0) {
                         s_w[0] = 0u;
                         // seats only grow and the swap re-reads the claims, so
