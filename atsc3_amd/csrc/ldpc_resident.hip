@@ -609,8 +609,8 @@ __device__ __forceinline__ int step_first(const StepDeal &d, unsigned lane) {
     return d.ctr ? step_next(d, lane, 0) : d.w0;
 }
 
-// One CHECK phase over the check steps w0, w0 + wstep, ... (this wave's
-// share); returns the wave's count of unsatisfied checks.
+// One CHECK phase over the check steps that d deals this wave; returns the
+// wave's count of unsatisfied checks.
 template <int M>
 __device__ __forceinline__ unsigned check_phase(const QC_CONST ResTab *ct, const Tot &T,
                                                 const Rec &R, const StepDeal &d, unsigned lane,
@@ -703,8 +703,8 @@ __device__ unsigned syndrome_generic(const QC_CONST ResTab *ct, const Tot &T, in
     return (valid && syn) ? 1u : 0u;
 }
 
-// The syndrome of the current posteriors over the check steps w0, w0 +
-// wstep, ...: the count check_phase would return, at a fraction of its work.
+// The syndrome of the current posteriors over the check steps that d deals
+// this wave: the count check_phase would return, at a fraction of its work.
 template <int M>
 __device__ __forceinline__ unsigned syndrome_phase(const QC_CONST ResTab *ct, const Tot &T,
                                                    const StepDeal &d, unsigned lane) {
@@ -754,8 +754,8 @@ __device__ __forceinline__ unsigned syndrome_phase(const QC_CONST ResTab *ct, co
 // 8.85 ms per 2566-block window; without the pass 8.99 ms).
 #define RES_SYN_FIRST 4
 
-// One VAR phase over the variable steps w0, w0 + wstep, ...; PUB: team mode
-// (posteriors published, see Tot).
+// One VAR phase over the variable steps that d deals this wave; PUB: team
+// mode (posteriors published, see Tot).
 template <int M, bool PUB>
 __device__ __forceinline__ void var_phase(const QC_CONST ResTab *ct, const Tot &T,
                                           const float *lam, const Rec &R, const StepDeal &d,
