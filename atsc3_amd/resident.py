@@ -55,12 +55,15 @@ FAST_MAX = 16
 
 def _pack_tasks(deg, generic, pack):
     """Tasks (group | c6 << 8: rows c6*64 .. c6*64+63 of a 360-row group) in
-    the order groups sorted by (generic, -degree), cut into steps
+    the order groups sorted by (not generic, -degree), cut into steps
     D | log2(G) << 7 | start << 10 | generic << 22 of G equal-degree tasks
-    (G in {1, 2, 4, 8}); a generic step holds one task."""
+    (G in {1, 2, 4, 8}); a generic step holds one task.  The waves take the
+    steps in this order, so the slowest ones (generic steps: runtime loops,
+    one task each; then the highest degrees) come first and the phase ends
+    on light steps."""
     deg = np.asarray(deg, np.int64)
     generic = np.asarray(generic, bool)
-    order = np.lexsort((np.arange(len(deg)), -deg, generic))
+    order = np.lexsort((np.arange(len(deg)), -deg, ~generic))
     tasks = np.array([g | (c6 << 8) for g in order for c6 in range(6)], np.int64)
     steps, i = [], 0
     while i < len(tasks):
