@@ -1221,9 +1221,9 @@ def main():
                    "engine": engine,
                    "headline": (f"{args.msg} check-record messages on the resident engine "
                                 f"(one codeword per compute unit, posteriors in LDS"
-                                + (f"; grow mode: the codewords still running once the queue "
-                                   f"is empty finish on teams of {GROW_TEAM} compute units of "
-                                   f"their XCD)" if args.msg != "float32" else ")")
+                                + f"; grow mode: the codewords still running once the queue "
+                                  f"is empty finish on teams of {GROW_TEAM} compute units of "
+                                  f"their XCD)"
                                 if engine == "resident" else f"{args.msg} messages, {engine} engine"),
                    "posteriors": ("float16 in LDS" if engine == "resident" else "float32"),
                    "plain_fp32": plain, "iters_max": 50, "alpha": 0.75,
@@ -1243,8 +1243,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": ((f"resident_grow_kernel<{ {'float16': 1, 'int8': 2}[args.msg]}>"
-                                 if args.msg != "float32" else "resident_kernel<0>")
+                     "kernel": (f"resident_grow_kernel<{ {'float32': 0, 'float16': 1, 'int8': 2}[args.msg]}>"
                                 if engine == "resident" else "decode launch"),
                      "model": (f"per block 4N + N + its * (2 E s + 4N), s = {s_msg} bytes per "
                                f"stored message, N = {N}, E = {E}"),
