@@ -816,6 +816,114 @@ __device__ __forceinline__ void var_phase(const QC_CONST ResTab *ct, const Tot &
     }
 }
 
+// tot = lam.  FP16 / INT8 rows (row = group, padded to RES_PROW halves in
+// LDS or in a team's published rows) with a 16-byte aligned lam: four
+// positions per lane, items i = t * 90 + j (group t, positions 4j .. 4j +
+// 3, LLRs i as one 16-byte load), chunks of 64 items dealt as w0, w0 +
+// wst, ..., 8 loads in flight; otherwise one position per lane over the
+// 64-row tasks w0, w0 + wst, ...
+template <int M>
+__device__ __forceinline__ void res_tot_init(const QC_CONST ResTab *ct, const Tot &T,
+                                             const float *lam, int w0, int wst, unsigned lane) {
+    const int nvg = ct->nvg;
+    if (M != 0 && (reinterpret_cast<uintptr_t>(lam) & 15u) == 0u) {
+        const unsigned nq = (unsigned)nvg * (ATSC3_Z / 4u), cst = (unsigned)wst * 64u;
+        for (unsigned c = (unsigned)w0 * 64u; c < nq; c += 8u * cst) {
+            float4 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const unsigned i = c + (unsigned)k * cst + lane;
+                v[k] = i < nq ? at(reinterpret_cast<const float4 *>(lam), i)
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const unsigned i = c + (unsigned)k * cst + lane;
+                if (i < nq) {
+                    const unsigned t = i / (ATSC3_Z / 4u), p = (i - t * (ATSC3_Z / 4u)) * 4u;
+                    const uint2 h = make_uint2(
+                        (unsigned)h_bits_sat(v[k].x) | ((unsigned)h_bits_sat(v[k].y) << 16),
+                        (unsigned)h_bits_sat(v[k].z) | ((unsigned)h_bits_sat(v[k].w) << 16));
+                    uint2 *row = reinterpret_cast<uint2 *>(T.lds + t * (RES_PROW * 2u));
+                    row[p / 4u] = h;
+                    if (p < 64u) row[(ATSC3_Z + p) / 4u] = h;
+                }
+            }
+        }
+        return;
+    }
+    for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int task = t8 + k * wst, tg = task / 6;
+            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+            v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int task = t8 + k * wst, tg = task / 6;
+            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+            if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
+                const unsigned vw = (unsigned)ct->grp_v[tg];
+                tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
+            }
+        }
+    }
+}
+
+// Hard decisions (and posteriors when asked).  FP16 / INT8 rows with the
+// outputs 4-byte (bits) and 16-byte (posteriors) aligned: four positions
+// per lane as in res_tot_init (one 8-byte row read, one 4-byte store of
+// four decisions), 4 items in flight; otherwise one position per lane over
+// the 64-row tasks w0, w0 + wst, ...
+template <int M>
+__device__ __forceinline__ void res_write_out(const ResArgs &a, const QC_CONST ResTab *ct,
+                                              const Tot &T, long long off, int w0, int wst,
+                                              unsigned lane) {
+    uint8_t *bout = a.bits + off;
+    float *pout = a.post ? a.post + off : nullptr;
+    if (M != 0 && (reinterpret_cast<uintptr_t>(bout) & 3u) == 0u &&
+        (reinterpret_cast<uintptr_t>(pout) & 15u) == 0u) {
+        const unsigned nq = (unsigned)ct->nvg * (ATSC3_Z / 4u), cst = (unsigned)wst * 64u;
+        for (unsigned c = (unsigned)w0 * 64u; c < nq; c += 4u * cst) {
+            uint2 h[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned i = c + (unsigned)k * cst + lane;
+                const unsigned t = i / (ATSC3_Z / 4u), p = (i - t * (ATSC3_Z / 4u)) * 4u;
+                h[k] = i < nq ? reinterpret_cast<const uint2 *>(T.lds + t * (RES_PROW * 2u))[p / 4u]
+                              : make_uint2(0u, 0u);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned i = c + (unsigned)k * cst + lane;
+                if (i < nq) {
+                    const float v0 = h_val((unsigned short)(h[k].x & 0xffffu));
+                    const float v1 = h_val((unsigned short)(h[k].x >> 16));
+                    const float v2 = h_val((unsigned short)(h[k].y & 0xffffu));
+                    const float v3 = h_val((unsigned short)(h[k].y >> 16));
+                    at(reinterpret_cast<unsigned *>(bout), i) =
+                        (unsigned)(v0 < 0.0f) | ((unsigned)(v1 < 0.0f) << 8) |
+                        ((unsigned)(v2 < 0.0f) << 16) | ((unsigned)(v3 < 0.0f) << 24);
+                    if (pout) at(reinterpret_cast<float4 *>(pout), i) = make_float4(v0, v1, v2, v3);
+                }
+            }
+        }
+        return;
+    }
+    for (int task = w0; task < ct->nvg * 6; task += wst) {
+        const int tg = task / 6;
+        const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
+        const unsigned vw = (unsigned)ct->grp_v[tg];
+        if (p < (unsigned)ATSC3_Z) {
+            const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
+            bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
+            if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
+        }
+    }
+}
+
 __global__ void resident_init(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
@@ -850,7 +958,6 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
         const int blk = __builtin_amdgcn_readfirstlane(s_blk);
         if (blk >= a.nblk) return;
         const QC_CONST ResTab *ct = &a.codes[a.blk_code[blk]];
-        const int nvg = ct->nvg;
         const long long off = a.blk_off[blk];
         const float *lam = a.llr + off;
         if constexpr (M == 2) {
@@ -858,27 +965,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             R.step = a.msg_scale ? a.msg_scale[blk] : int8_step_of_block(lam, ct->n, s_hist);
             R.inv = 1.0f / R.step;
         }
-        // tot = lam, row by row (each 64-position task of a group on one
-        // wave), the loads of 8 tasks in flight together
-        for (int t8 = wave; t8 < nvg * 6; t8 += RES_WAVES * 8) {
-            float v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int task = t8 + q * RES_WAVES, t = task / 6;
-                const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
-                v[q] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)t * ATSC3_Z + p]
-                                                               : 0.0f;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int task = t8 + q * RES_WAVES, t = task / 6;
-                const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
-                if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
-                    const unsigned vw = (unsigned)ct->grp_v[t];
-                    tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, v[q]);
-                }
-            }
-        }
+        res_tot_init<M>(ct, T, lam, wave, RES_WAVES, lane);
         __syncthreads();
         int prev = 1 << 30;          // the last check phase's count
         // step counters: each is reset by thread 0 in a phase that no wave
@@ -907,17 +994,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 if (tid == 0) s_unsat[2] = 0;
             }
             if (total == 0 || it == a.iters) {
-                uint8_t *bout = a.bits + off;
-                for (int task = wave; task < nvg * 6; task += RES_WAVES) {
-                    const int t = task / 6;
-                    const unsigned p = (unsigned)(task - t * 6) * 64u + lane;
-                    const unsigned vw = (unsigned)ct->grp_v[t];
-                    if (p < (unsigned)ATSC3_Z) {
-                        const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
-                        bout[(unsigned)t * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
-                        if (a.post) a.post[off + (unsigned)t * ATSC3_Z + p] = v;
-                    }
-                }
+                res_write_out<M>(a, ct, T, off, wave, RES_WAVES, lane);
                 if (tid == 0) {
                     a.conv[blk] = (uint8_t)(total == 0);
                     a.its[blk] = total == 0 ? it : a.iters;
@@ -1314,50 +1391,6 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
 #define GROW_RMAX 64                  // requests per XCD
 #define GR_WORDS 8                    // request: posted, codeword, owner, first iteration, step
 #define GH_WORDS (GH_REQ + RES_XCDS * GROW_RMAX * GR_WORDS)
-
-// tot = lam for the 64-row tasks w0, w0 + wst, ... (8 loads in flight)
-template <int M>
-__device__ __forceinline__ void res_tot_init(const QC_CONST ResTab *ct, const Tot &T,
-                                             const float *lam, int w0, int wst, unsigned lane) {
-    const int nvg = ct->nvg;
-    for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
-        float v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int task = t8 + k * wst, tg = task / 6;
-            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-            v[k] = task < nvg * 6 && p < (unsigned)ATSC3_Z ? lam[(unsigned)tg * ATSC3_Z + p] : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int task = t8 + k * wst, tg = task / 6;
-            const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-            if (task < nvg * 6 && p < (unsigned)ATSC3_Z) {
-                const unsigned vw = (unsigned)ct->grp_v[tg];
-                tot_st<M>(T, (vw >> 23) & 255u, vw >> 31, p, v[k]);
-            }
-        }
-    }
-}
-
-// Hard decisions (and posteriors when asked) of the 64-row tasks w0, w0 +
-// wst, ...
-template <int M>
-__device__ __forceinline__ void res_write_out(const ResArgs &a, const QC_CONST ResTab *ct,
-                                              const Tot &T, long long off, int w0, int wst,
-                                              unsigned lane) {
-    uint8_t *bout = a.bits + off;
-    for (int task = w0; task < ct->nvg * 6; task += wst) {
-        const int tg = task / 6;
-        const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
-        const unsigned vw = (unsigned)ct->grp_v[tg];
-        if (p < (unsigned)ATSC3_Z) {
-            const float v = tot_ld<M>(T, (vw >> 23) & 255u, vw >> 31, p);
-            bout[(unsigned)tg * ATSC3_Z + p] = (uint8_t)(v < 0.0f);
-            if (a.post) a.post[off + (unsigned)tg * ATSC3_Z + p] = v;
-        }
-    }
-}
 
 // The owner's LDS posterior rows, byte for byte, to its global slot: 16-byte
 // chunks, 8 in flight per thread.
