@@ -248,6 +248,11 @@ __device__ __forceinline__ unsigned shl1_add_lane_bit(unsigned neg, bool bit) {
     return r;
 }
 
+// The wave's count of lanes where `bit` holds (the same in every lane).
+__device__ __forceinline__ unsigned wave_count(bool bit) {
+    return (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(bit));
+}
+
 // Bit 31 set iff the message whose sign sits at bit sp is negative (record
 // words w1, w2; bits 0..24 of w1, then w2).
 __device__ __forceinline__ unsigned rec_sign(unsigned w1, unsigned long long w2, unsigned sp) {
@@ -368,7 +373,7 @@ __device__ __forceinline__ unsigned check_multi(const QC_CONST int *sd, const To
             rec_put<M>(R, rix[gi], m1 * alpha, m2 * alpha,
                        (sg & ((1u << RES_SIGN_LO) - 1u)) | (ix << RES_SIGN_LO));
         }
-        syn_total += (valid[gi] && syn) ? 1u : 0u;
+        syn_total += wave_count(valid[gi] && syn);
     }
     return syn_total;
 }
@@ -426,7 +431,7 @@ __device__ unsigned check_generic(const QC_CONST ResTab *ct, const Tot &T, const
                    (sg_lo & ((1u << RES_SIGN_LO) - 1u)) | (idx << RES_SIGN_LO));
         if (dc > RES_SIGN_LO) at(R.p2, rix) = par ? ~neg_hi : neg_hi;
     }
-    return valid ? syn : 0u;
+    return wave_count(valid && syn != 0u);
 }
 
 // G variable tasks of degree D <= 16 whose entries are all full circulants
@@ -610,7 +615,7 @@ __device__ __forceinline__ int step_first(const StepDeal &d, unsigned lane) {
 }
 
 // One CHECK phase over the check steps that d deals this wave; returns the
-// wave's count of unsatisfied checks.
+// wave's count of unsatisfied checks (the same in every lane).
 template <int M>
 __device__ __forceinline__ unsigned check_phase(const QC_CONST ResTab *ct, const Tot &T,
                                                 const Rec &R, const StepDeal &d, unsigned lane,
@@ -683,7 +688,7 @@ __device__ __forceinline__ unsigned syndrome_multi(const QC_CONST int *sd, const
 #pragma unroll
         for (int k = 0; k < D; ++k)
             syn ^= (M == 0 ? tv[gi][k] : __half2float(th[gi][k])) < 0.0f;
-        cnt += (valid[gi] && syn) ? 1u : 0u;
+        cnt += wave_count(valid[gi] && syn);
     }
     return cnt;
 }
@@ -700,7 +705,7 @@ __device__ unsigned syndrome_generic(const QC_CONST ResTab *ct, const Tot &T, in
         const unsigned p = wrap360(jj + (ew & 511u));
         syn ^= tot_ld<M>(T, (ew >> 9) & 511u, (ew >> 28) & 1u, p) < 0.0f;
     }
-    return (valid && syn) ? 1u : 0u;
+    return wave_count(valid && syn);
 }
 
 // The syndrome of the current posteriors over the check steps that d deals
@@ -976,7 +981,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
             bool stop = false;
             if (it > 0 && prev <= RES_SYN_FIRST) {
                 const unsigned u0 = syndrome_phase<M>(ct, T, StepDeal{&s_next[0], 0, 0}, lane);
-                if (u0) atomicAdd(&s_unsat[2], (int)u0);
+                if (lane == 0u && u0) atomicAdd(&s_unsat[2], (int)u0);
                 __syncthreads();
                 stop = s_unsat[2] == 0;
             }
@@ -986,7 +991,7 @@ __global__ __launch_bounds__(RES_WG) void resident_kernel(ResArgs a) {
                 const bool first = it == 0, update = it < a.iters;
                 const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{&s_next[1], 0, 0}, lane,
                                                       first, update, a.alpha);
-                if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+                if (lane == 0u && unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                 __syncthreads();
                 total = s_unsat[it & 1];
                 // every thread read the syndrome pass's count before this
@@ -1312,7 +1317,7 @@ __global__ __launch_bounds__(RES_WG) void resident_team_kernel(ResArgs a, TeamAr
                 const bool first = it == 0, update = it < a.iters;
                 const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{nullptr, w0, wst}, lane,
                                                       first, update, alpha);
-                if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+                if (lane == 0u && unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                 __syncthreads();
                 unsigned *cnt = ctl + TC_UNSAT + it % 3;
                 if (tid == 0 && s_unsat[it & 1])
@@ -1468,7 +1473,7 @@ __device__ __forceinline__ bool grow_team(const ResArgs &a, const TeamArgs &t, u
         const bool first = it == 0, update = it < a.iters;
         const unsigned unsat = check_phase<M>(ct, T, R, StepDeal{nullptr, w0, wst}, lane, first,
                                               update, a.alpha);
-        if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+        if (lane == 0u && unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
         __syncthreads();
         unsigned *cnt = ctl + TC_UNSAT + it % 3;
         if (tid == 0 && s_unsat[it & 1])
@@ -1571,7 +1576,7 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                     if (it > 0 && prev <= RES_SYN_FIRST) {
                         const unsigned u0 = syndrome_phase<M>(ct, T, StepDeal{&s_next[0], 0, 0},
                                                               lane);
-                        if (u0) atomicAdd(&s_unsat[2], (int)u0);
+                        if (lane == 0u && u0) atomicAdd(&s_unsat[2], (int)u0);
                         __syncthreads();
                         stop = s_unsat[2] == 0;
                     }
@@ -1582,7 +1587,7 @@ __global__ __launch_bounds__(RES_WG) void resident_grow_kernel(ResArgs a, TeamAr
                         const unsigned unsat = check_phase<M>(ct, T, R,
                                                               StepDeal{&s_next[1], 0, 0}, lane,
                                                               first, update, a.alpha);
-                        if (unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
+                        if (lane == 0u && unsat) atomicAdd(&s_unsat[it & 1], (int)unsat);
                         __syncthreads();
                         total = s_unsat[it & 1];
                         // every thread read the syndrome pass's count before
