@@ -821,42 +821,15 @@ __device__ __forceinline__ void var_phase(const QC_CONST ResTab *ct, const Tot &
     }
 }
 
-// tot = lam.  FP16 / INT8 rows (row = group, padded to RES_PROW halves in
-// LDS or in a team's published rows) with a 16-byte aligned lam: four
-// positions per lane, items i = t * 90 + j (group t, positions 4j .. 4j +
-// 3, LLRs i as one 16-byte load), chunks of 64 items dealt as w0, w0 +
-// wst, ..., 8 loads in flight; otherwise one position per lane over the
-// 64-row tasks w0, w0 + wst, ...
+// tot = lam for the 64-row tasks w0, w0 + wst, ... (8 loads in flight).
+// Four positions per lane (16-byte LLR loads) measured 0.8% faster with the
+// write-out below done the same way, but raised the grow kernel from 112 to
+// 128 VGPRs, which leaves no room for a demap wave beside a decoding
+// workgroup: the receiver pipeline's overlap is worth more.
 template <int M>
 __device__ __forceinline__ void res_tot_init(const QC_CONST ResTab *ct, const Tot &T,
                                              const float *lam, int w0, int wst, unsigned lane) {
     const int nvg = ct->nvg;
-    if (M != 0 && (reinterpret_cast<uintptr_t>(lam) & 15u) == 0u) {
-        const unsigned nq = (unsigned)nvg * (ATSC3_Z / 4u), cst = (unsigned)wst * 64u;
-        for (unsigned c = (unsigned)w0 * 64u; c < nq; c += 8u * cst) {
-            float4 v[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const unsigned i = c + (unsigned)k * cst + lane;
-                v[k] = i < nq ? at(reinterpret_cast<const float4 *>(lam), i)
-                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const unsigned i = c + (unsigned)k * cst + lane;
-                if (i < nq) {
-                    const unsigned t = i / (ATSC3_Z / 4u), p = (i - t * (ATSC3_Z / 4u)) * 4u;
-                    const uint2 h = make_uint2(
-                        (unsigned)h_bits_sat(v[k].x) | ((unsigned)h_bits_sat(v[k].y) << 16),
-                        (unsigned)h_bits_sat(v[k].z) | ((unsigned)h_bits_sat(v[k].w) << 16));
-                    uint2 *row = reinterpret_cast<uint2 *>(T.lds + t * (RES_PROW * 2u));
-                    row[p / 4u] = h;
-                    if (p < 64u) row[(ATSC3_Z + p) / 4u] = h;
-                }
-            }
-        }
-        return;
-    }
     for (int t8 = w0; t8 < nvg * 6; t8 += wst * 8) {
         float v[8];
 #pragma unroll
@@ -877,46 +850,13 @@ __device__ __forceinline__ void res_tot_init(const QC_CONST ResTab *ct, const To
     }
 }
 
-// Hard decisions (and posteriors when asked).  FP16 / INT8 rows with the
-// outputs 4-byte (bits) and 16-byte (posteriors) aligned: four positions
-// per lane as in res_tot_init (one 8-byte row read, one 4-byte store of
-// four decisions), 4 items in flight; otherwise one position per lane over
-// the 64-row tasks w0, w0 + wst, ...
+// Hard decisions (and posteriors when asked) of the 64-row tasks w0, w0 +
+// wst, ...
 template <int M>
 __device__ __forceinline__ void res_write_out(const ResArgs &a, const QC_CONST ResTab *ct,
                                               const Tot &T, long long off, int w0, int wst,
                                               unsigned lane) {
     uint8_t *bout = a.bits + off;
-    float *pout = a.post ? a.post + off : nullptr;
-    if (M != 0 && (reinterpret_cast<uintptr_t>(bout) & 3u) == 0u &&
-        (reinterpret_cast<uintptr_t>(pout) & 15u) == 0u) {
-        const unsigned nq = (unsigned)ct->nvg * (ATSC3_Z / 4u), cst = (unsigned)wst * 64u;
-        for (unsigned c = (unsigned)w0 * 64u; c < nq; c += 4u * cst) {
-            uint2 h[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const unsigned i = c + (unsigned)k * cst + lane;
-                const unsigned t = i / (ATSC3_Z / 4u), p = (i - t * (ATSC3_Z / 4u)) * 4u;
-                h[k] = i < nq ? reinterpret_cast<const uint2 *>(T.lds + t * (RES_PROW * 2u))[p / 4u]
-                              : make_uint2(0u, 0u);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const unsigned i = c + (unsigned)k * cst + lane;
-                if (i < nq) {
-                    const float v0 = h_val((unsigned short)(h[k].x & 0xffffu));
-                    const float v1 = h_val((unsigned short)(h[k].x >> 16));
-                    const float v2 = h_val((unsigned short)(h[k].y & 0xffffu));
-                    const float v3 = h_val((unsigned short)(h[k].y >> 16));
-                    at(reinterpret_cast<unsigned *>(bout), i) =
-                        (unsigned)(v0 < 0.0f) | ((unsigned)(v1 < 0.0f) << 8) |
-                        ((unsigned)(v2 < 0.0f) << 16) | ((unsigned)(v3 < 0.0f) << 24);
-                    if (pout) at(reinterpret_cast<float4 *>(pout), i) = make_float4(v0, v1, v2, v3);
-                }
-            }
-        }
-        return;
-    }
     for (int task = w0; task < ct->nvg * 6; task += wst) {
         const int tg = task / 6;
         const unsigned p = (unsigned)(task - tg * 6) * 64u + lane;
