@@ -19,9 +19,9 @@ engine:
   device;
 * one codeword, iters 0 and 1 (no iteration left to grow into), an empty
   batch, a queue order given as a permutation;
-* decode_dev's resident engine (grow mode for FP16 and INT8) on 300
-  blocks of 64800 r10/15 over a spread of SNRs equals the plain resident
-  kernel, with launches enqueued on two streams alternately.
+* decode_dev's resident engine (grow mode, every format) on 300 blocks of
+  64800 r10/15 over a spread of SNRs equals the plain resident kernel,
+  with launches enqueued on two streams alternately.
 """
 import numpy as np
 import pytest
@@ -145,10 +145,10 @@ def test_grow_edge_cases(device, msg):
 
 @pytest.mark.gpu
 def test_grow_decode_dev_window_two_streams(device):
-    """decode_dev's resident engine (grow mode for FP16 / INT8) on 300
-    blocks of 64800 r10/15 (iteration counts spread by SNR, so the queue's
-    last codewords run long) equals the plain resident kernel; launches
-    enqueued on two streams alternately give the same results."""
+    """decode_dev's resident engine (grow mode, every format) on 300 blocks
+    of 64800 r10/15 (iteration counts spread by SNR, so the queue's last
+    codewords run long) equals the plain resident kernel; launches enqueued
+    on two streams alternately give the same results."""
     import torch
     from atsc3_amd import ldpc, synth
     from atsc3_amd.codes import get_code
@@ -160,7 +160,7 @@ def test_grow_decode_dev_window_two_streams(device):
     llr = np.concatenate([synth.llr_bpsk_like(cw[i % 20:i % 20 + 1], snr[i], 1000 + i)
                           for i in range(300)]).astype(np.float32)
     t = torch.from_numpy(llr).to(device)
-    for msg in ("float16", "int8"):
+    for msg in ("float16", "int8", "float32"):
         print(f"[gpu] grow decode_dev window {msg}", flush=True)
         ref = [x.cpu().numpy() for x in decode_resident_dev(t, code, 50, 0.75, msg_dtype=msg)]
         print(f"[gpu] its median {np.median(ref[2])}, max {ref[2].max()}", flush=True)
