@@ -391,13 +391,27 @@ def mixed_bench(args, dev, formats=("float32", "float16", "int8")):
             pipe.run()
         torch.cuda.synchronize()
         ms_eager = (time.perf_counter() - t0) * 1e3 / max(args.steps, 5)
-        kbits = sum(__import__("atsc3_amd.codes", fromlist=["get_code"]).get_code(r, n).K
-                    for n, r, m in pipe.specs)
+        from atsc3_amd.codes import get_code
+        kbits = sum(get_code(r, n).K for n, r, m in pipe.specs)
+        # the compulsory-traffic model over the frame's blocks at the
+        # iteration counts of the timed (prepared) frame, each at its own N, E
+        its_t = pipe.plan.its.cpu().numpy()
+        s_fmt = {"float32": 4, "float16": 2, "int8": 1}[fmt]
+        frame_bytes = sum(model_bytes([its_t[b]], n, get_code(r, n).nedges, s_fmt)
+                          for b, (n, r, m) in enumerate(pipe.specs))
         res[fmt] = {"engine": pipe.engine, "slots": slots, "grid": grid, "tune_ms": table,
                     "ms_per_launch_graph": round(ms_graph, 4),
                     "ms_per_launch_eager": round(ms_eager, 4),
                     "blocks_per_s": round(nblk / (ms_graph / 1e3), 1),
                     "info_gbit_s": round(kbits / (ms_graph / 1e3) / 1e9, 3),
+                    "roofline": {"bound": "hbm", "model_bytes": round(frame_bytes),
+                                 "achieved": round(frame_bytes / (ms_graph / 1e3) / 1e9, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(frame_bytes / (ms_graph / 1e3) / 1e9
+                                               / HBM_PEAK_GBS, 4),
+                                 "note": "per block 4N + N + its * (2 E s + 4N) at the timed "
+                                         "frame's iteration counts, over one graph replay "
+                                         "(demaps included)"},
                     "fer_blocks": blocks, "fer_errors": errs,
                     "its_min": int(its.min().item()), "its_max": int(its.max().item()),
                     "its_mean": round(float(its.mean().item()), 2),
