@@ -561,3 +561,30 @@ def test_soft_output_equals_oracle(device, msg):
     assert np.array_equal(p.view(np.uint32), want[4].view(np.uint32))
     assert np.array_equal(outs[0].cpu().numpy().reshape(6, -1), (p < 0).astype(np.uint8))
     assert np.array_equal(outs[2].cpu().numpy(), want[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msg", ["float16", "float32"])
+def test_stop_test_at_every_iteration_limit(device, msg):
+    """The speculative stop test (once a check phase leaves at most a few
+    unsatisfied checks, the next iteration counts them alone first and stops
+    there when none is left) against the oracle at every iteration limit
+    from 0 to past convergence: the stop can land on the limit itself
+    (converged at its = iters), just after it (bad counted by the check
+    phase with no update), or anywhere before; conv, its, bad and bits must
+    be the plain schedule's each time, for the one-workgroup kernel and in
+    grow mode."""
+    import torch
+    from atsc3_amd.resident import decode_resident_dev
+    code, llr = _synth_llr("9/15", 16200, 2.8, nblk=6, seed=41)
+    t = torch.from_numpy(llr).to(device)
+    full = resident_minsum(llr, code, 50, 0.75, msg)
+    top = int(np.max(full[2])) + 2
+    assert 3 <= top <= 30, f"test blocks converge at {full[2]}"
+    for iters in range(top):
+        want = resident_minsum(llr, code, iters, 0.75, msg)
+        print(f"[gpu] stop test {msg} iters {iters}", flush=True)
+        for grow in (None, 8):
+            got = [x.cpu().numpy() for x in decode_resident_dev(t, code, iters, 0.75,
+                                                                msg_dtype=msg, grow=grow)]
+            _check(got, want, f"{msg} iters {iters} grow {grow}")
