@@ -65,7 +65,8 @@ def _store(d):
 
 # team sizes tried: a partial team takes an XCD's remaining workgroups (32
 # per XCD: 12 gives teams of 12, 12 and 8); on the 74-block uneven frame
-# FP32 replays in 0.94 ms with teams of 12 against 1.03-1.05 with 8 or 16
+# FP32 replays measured 0.94-1.05 ms with teams of 12 against 1.03-1.12
+# with 8 or 16 (FP16: 0.92 with 12, 0.91 with 16)
 TEAM_SIZES = (4, 8, 12, 16)
 
 
